@@ -1,0 +1,432 @@
+// aux.hip -- once-per-solve and once-per-rotation kernels around the
+// conduction kernels: layout packing, initial temperatures, shape->crater
+// coupling (RGI/RGH), crater and facet sunlight scattering, the facet
+// self-heating SpMM and the per-set status reduction.
+#include <hip/hip_runtime.h>
+
+#include "btpm_internal.h"
+
+// ---------------------------------------------------------------------------
+// layout: facet-major x[c][ntau]  <->  blocked x[b][c][BLK]
+// ---------------------------------------------------------------------------
+
+// out[b][c][k] = scale[c] * in[c][8b+k]  (scale may be null = 1); steps past
+// ntau are padded with the last step's value.
+extern "C" __global__ void btpm_pack_blocked(const float* __restrict__ in, float* __restrict__ out,
+                                             const float* __restrict__ scale, int ncols, int ntau,
+                                             int nb) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)ncols * nb) return;
+  const int b = (int)(i / ncols);
+  const int c = (int)(i - (size_t)b * ncols);
+  const float sc = scale ? scale[c] : 1.0f;
+  float v[BTPM_BLK];
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k) {
+    const int s = min(b * BTPM_BLK + k, ntau - 1);
+    v[k] = sc * in[(size_t)c * ntau + s];
+  }
+  float4* o = reinterpret_cast<float4*>(out + ((size_t)b * ncols + c) * BTPM_BLK);
+  o[0] = make_float4(v[0], v[1], v[2], v[3]);
+  o[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+extern "C" __global__ void btpm_unpack_blocked(const float* __restrict__ in, float* __restrict__ out,
+                                               int ncols, int ntau, int nb) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)ncols * nb) return;
+  const int b = (int)(i / ncols);
+  const int c = (int)(i - (size_t)b * ncols);
+  const float* p = in + ((size_t)b * ncols + c) * BTPM_BLK;
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k) {
+    const int s = b * BTPM_BLK + k;
+    if (s < ntau) out[(size_t)c * ntau + s] = p[k];
+  }
+}
+
+// crater reference layout x[f][s][j] <-> blocked x[b][f*NS+j][k]
+extern "C" __global__ void btpm_pack_crater(const float* __restrict__ in, float* __restrict__ out,
+                                            int nfac, int ntau, int nb) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncols = nfac * BTPM_NS;
+  if (i >= (size_t)ncols * nb) return;
+  const int b = (int)(i / ncols);
+  const int c = (int)(i - (size_t)b * ncols);
+  const int f = c / BTPM_NS, j = c - f * BTPM_NS;
+  float v[BTPM_BLK];
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k) {
+    const int s = min(b * BTPM_BLK + k, ntau - 1);
+    v[k] = in[((size_t)f * ntau + s) * BTPM_NS + j];
+  }
+  float4* o = reinterpret_cast<float4*>(out + ((size_t)b * ncols + c) * BTPM_BLK);
+  o[0] = make_float4(v[0], v[1], v[2], v[3]);
+  o[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+extern "C" __global__ void btpm_unpack_crater(const float* __restrict__ in, float* __restrict__ out,
+                                              int nfac, int ntau, int nb) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncols = nfac * BTPM_NS;
+  if (i >= (size_t)ncols * nb) return;
+  const int b = (int)(i / ncols);
+  const int c = (int)(i - (size_t)b * ncols);
+  const int f = c / BTPM_NS, j = c - f * BTPM_NS;
+  const float* p = in + ((size_t)b * ncols + c) * BTPM_BLK;
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k) {
+    const int s = b * BTPM_BLK + k;
+    if (s < ntau) out[((size_t)f * ntau + s) * BTPM_NS + j] = p[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// initial temperatures
+// ---------------------------------------------------------------------------
+
+// initialisetemperatures, T175706/thermal.cpp:1861-1889, from the absorbed
+// flux src = (1-A)F (blocked):  T = (mean_s src / es)^(1/4) everywhere.
+extern "C" __global__ void btpm_init_smooth(btpm_set_dev S) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ncols) return;
+  double sum = 0.0;
+  for (int b = 0; b < S.nb; ++b) {
+    const float* p = S.src + ((size_t)b * S.ncols + c) * BTPM_BLK;
+    const int kend = min(BTPM_BLK, S.ntau - b * BTPM_BLK);
+    for (int k = 0; k < kend; ++k) sum += (double)p[k];
+  }
+  const float t = (float)pow(sum / S.ntau / (double)S.es, 0.25);
+  for (int i = 0; i < BTPM_PSTRIDE; ++i) S.prof[(size_t)c * BTPM_PSTRIDE + i] = t;
+  const float e = S.esia * t * t * t * t;
+  for (int b = 0; b < S.nb; ++b) {
+    float* p = S.T + ((size_t)b * S.ncols + c) * BTPM_BLK;
+    for (int k = 0; k < BTPM_BLK; ++k) p[k] = t;
+    if (S.emis) {
+      float* q = S.emis + ((size_t)b * S.ncols + c) * BTPM_BLK;
+      for (int k = 0; k < BTPM_BLK; ++k) q[k] = e;
+    }
+  }
+  S.frozen[c] = 0;
+  S.dmax[c] = 0.0f;
+}
+
+// roughness_initialise_temperatures, T175706/thermal_GPU.cu:374-423: two
+// passes per crater; src = (1-A) illum + RGH (blocked).  vf32[i*NS+j] =
+// VF(j->i).  One workgroup (128 lanes) per parent facet.
+extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev S,
+                                                                   const float* __restrict__ vf32) {
+  __shared__ float e1[BTPM_NS];
+  const int f = blockIdx.x;
+  const int j = threadIdx.x;
+  const bool ok = j < BTPM_NS;
+  const int c = f * BTPM_NS + (ok ? j : 0);
+  double sum = 0.0;
+  if (ok) {
+    for (int b = 0; b < S.nb; ++b) {
+      const float* p = S.src + ((size_t)b * S.ncols + c) * BTPM_BLK;
+      const int kend = min(BTPM_BLK, S.ntau - b * BTPM_BLK);
+      for (int k = 0; k < kend; ++k) sum += (double)p[k];
+    }
+  }
+  const double mean = sum / S.ntau;
+  const double t1 = pow(mean / (double)S.es, 0.25);
+  if (ok) e1[j] = (float)((double)S.esia * t1 * t1 * t1 * t1);
+  __syncthreads();
+  if (!ok) return;
+  double self = 0.0;
+  for (int i = 0; i < BTPM_NS; ++i) self += (double)e1[i] * (double)vf32[i * BTPM_NS + j];
+  const float t = (float)pow((mean + self) / (double)S.es, 0.25);
+  for (int i = 0; i < BTPM_PSTRIDE; ++i) S.prof[(size_t)c * BTPM_PSTRIDE + i] = t;
+  for (int b = 0; b < S.nb; ++b) {
+    float* p = S.T + ((size_t)b * S.ncols + c) * BTPM_BLK;
+    for (int k = 0; k < BTPM_BLK; ++k) p[k] = t;
+  }
+  if (j == 0) {
+    S.frozen[f] = 0;
+    S.dmax[f] = 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// shape -> crater coupling (load_GPU_roughness_globalillumination /
+// _globalheating, T175706/thermal.cpp:2693-2803) as a per-facet GEMM
+//   out[j][s] = sum_k drf[k][j] w[g_k][s]        (j < 100, s in a 64-step tile)
+// w = weights in blocked layout over the smooth columns:
+//   RGI: w = A_g F_g(s);  RGH: w = es T_g(s)^4 (mode 1 computes it from T).
+// The result is written blocked over crater columns c = f*NS+j as
+//   dst[b][c][k] = base_scale_f * base[b][c][k] + out      (base may be null).
+// Workgroup: 256 lanes = 4 waves; wave w owns j in [25w, 25w+25), lane = s.
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
+    const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ drf,
+    const float* __restrict__ w, int mode, float es, int nsmooth, int nfac, int ntau, int nb,
+    const float* __restrict__ base, const float* __restrict__ base_scale, float* __restrict__ dst) {
+  constexpr int KC = 16;
+  __shared__ float ds[KC][BTPM_NS + 4];
+  __shared__ float ws[KC][64];
+  const int f = blockIdx.y;
+  const int s0 = blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int jw = (tid >> 6) * 25;
+  const int s = s0 + lane;
+  float acc[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) acc[q] = 0.0f;
+  const int k0 = rowptr[f], k1 = rowptr[f + 1];
+  for (int kb = k0; kb < k1; kb += KC) {
+    const int kn = min(KC, k1 - kb);
+    __syncthreads();
+    for (int e = tid; e < KC * BTPM_NS; e += 256) {
+      const int kk = e / BTPM_NS, jj = e - kk * BTPM_NS;
+      ds[kk][jj] = kk < kn ? drf[(size_t)(kb + kk) * BTPM_NS + jj] : 0.0f;
+    }
+    for (int e = tid; e < KC * 64; e += 256) {
+      const int kk = e >> 6, ss = min(s0 + (e & 63), ntau - 1);
+      float v = 0.0f;
+      if (kk < kn) {
+        const int g = colidx[kb + kk];
+        v = w[((size_t)(ss / BTPM_BLK) * nsmooth + g) * BTPM_BLK + (ss % BTPM_BLK)];
+        if (mode == 1) v = es * (v * v) * (v * v);
+      }
+      ws[kk][e & 63] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < KC; ++kk) {
+      const float wv = ws[kk][lane];
+#pragma unroll
+      for (int q = 0; q < 25; ++q) acc[q] = fmaf(ds[kk][jw + q], wv, acc[q]);
+    }
+  }
+  if (s >= nb * BTPM_BLK) return;
+  const int ncols = nfac * BTPM_NS;
+  const float bsc = base_scale ? base_scale[f] : 1.0f;
+  for (int q = 0; q < 25; ++q) {
+    const int c = f * BTPM_NS + jw + q;
+    const size_t o = ((size_t)(s / BTPM_BLK) * ncols + c) * BTPM_BLK + (s % BTPM_BLK);
+    float v = acc[q];
+    if (base) v += bsc * base[o];
+    dst[o] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// crater sunlight scattering (roughness_single_scattering /
+// roughness_multiple_scattering, T175706/thermal_GPU.cu:241-341).
+// Inputs shadow, cosang, rgi in the reference layout [f][s][j]; output
+// illum blocked [b][f*NS+j][k].  One workgroup (128 lanes) per (facet, block):
+// lane j holds the 8 steps of sub-facet j.
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(128) void btpm_crater_scatter(
+    const float* __restrict__ vf32, const float* __restrict__ albedo,
+    const float* __restrict__ shadow, const float* __restrict__ cosang,
+    const float* __restrict__ rgi, float solar, int multiple, float acc, int max_iter, int nfac,
+    int ntau, int nb, float* __restrict__ illum_out) {
+  __shared__ float vfs[BTPM_NS * BTPM_NS];
+  __shared__ float il[BTPM_BLK][BTPM_NS];
+  const int f = blockIdx.y;
+  const int b = blockIdx.x;
+  const int j = threadIdx.x;
+  const bool ok = j < BTPM_NS;
+  for (int e = j; e < BTPM_NS * BTPM_NS; e += 128) vfs[e] = vf32[e];
+  const float a = albedo[f];
+  float sp[BTPM_BLK], res[BTPM_BLK];
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k) {
+    const int s = min(b * BTPM_BLK + k, ntau - 1);
+    sp[k] = 0.0f;
+    if (ok) {
+      const size_t o = ((size_t)f * ntau + s) * BTPM_NS + j;
+      sp[k] = solar * shadow[o] * cosang[o] + (rgi ? rgi[o] : 0.0f);
+      il[k][j] = sp[k];
+    }
+  }
+  __syncthreads();
+  // single scattering: tot_j = il_j + A sum_i VF(j->i) il_i, clamped >= 0
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k) {
+    float t = sp[k];
+    if (ok)
+      for (int i = 0; i < BTPM_NS; ++i) t = fmaf(vfs[i * BTPM_NS + j] * a, il[k][i], t);
+    res[k] = t > 0.0f ? t : 0.0f;
+  }
+  if (multiple && a > 0.0f) {
+    // Jacobi on p = A (sp + VF p) from p = A * single; stop when every
+    // |p_new - p_old| <= acc over the block's 8 steps
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BTPM_BLK; ++k)
+      if (ok) il[k][j] = a * res[k];
+    for (int it = 0; it < max_iter; ++it) {
+      __syncthreads();  // the a*res / previous sweep writes are visible
+      float pn[BTPM_BLK];
+      int ch = 0;
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k) {
+        float sum = 0.0f;
+        if (ok)
+          for (int i = 0; i < BTPM_NS; ++i) sum = fmaf(vfs[i * BTPM_NS + j], il[k][i], sum);
+        pn[k] = a * (sp[k] + sum);
+        if (ok && fabsf(pn[k] - il[k][j]) > acc) ch = 1;
+      }
+      __syncthreads();  // every lane has read the old sweep
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k)
+        if (ok) il[k][j] = pn[k];
+      if (!__syncthreads_or(ch)) break;  // block-uniform exit
+    }
+#pragma unroll
+    for (int k = 0; k < BTPM_BLK; ++k) res[k] = ok ? il[k][j] / a : 0.0f;
+  } else if (multiple) {
+#pragma unroll
+    for (int k = 0; k < BTPM_BLK; ++k) res[k] = sp[k];
+  }
+  if (!ok) return;
+  const int ncols = nfac * BTPM_NS;
+  float4* o = reinterpret_cast<float4*>(illum_out + ((size_t)b * ncols + f * BTPM_NS + j) * BTPM_BLK);
+  o[0] = make_float4(res[0], res[1], res[2], res[3]);
+  o[1] = make_float4(res[4], res[5], res[6], res[7]);
+}
+
+// ---------------------------------------------------------------------------
+// sparse facet coupling over blocked time series (CSR rows = receivers):
+//   y[b][f][k] = alpha_f * (x0[b][f][k] + sum_{(f,g)} vf * x[b][g][k])
+// x0, alpha may be null (0 and 1).  Lane = receiving facet, each lane covers
+// BPT consecutive time blocks.  Used for the self-heating flux of the next
+// rotation (x = emis) and for the facet scattering sweeps.
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(256) void btpm_spmm_blocked(
+    const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ vals,
+    const float* __restrict__ x, const float* __restrict__ x0, const float* __restrict__ alpha,
+    float* __restrict__ y, int nrows, int nb, int bpt, const btpm_status_dev* __restrict__ st) {
+  if (st && st->done) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b0 = blockIdx.y * bpt;
+  if (f >= nrows) return;
+  const int k0 = rowptr[f], k1 = rowptr[f + 1];
+  const float al = alpha ? alpha[f] : 1.0f;
+  for (int bb = b0; bb < min(b0 + bpt, nb); bb += 4) {
+    const int nbb = min(4, min(b0 + bpt, nb) - bb);
+    float acc[4][BTPM_BLK];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k) acc[q][k] = 0.0f;
+    for (int kk = k0; kk < k1; ++kk) {
+      const int g = colidx[kk];
+      const float v = vals[kk];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q < nbb) {
+          const float4* p = reinterpret_cast<const float4*>(x + ((size_t)(bb + q) * nrows + g) * BTPM_BLK);
+          const float4 lo = p[0], hi = p[1];
+          acc[q][0] = fmaf(v, lo.x, acc[q][0]);
+          acc[q][1] = fmaf(v, lo.y, acc[q][1]);
+          acc[q][2] = fmaf(v, lo.z, acc[q][2]);
+          acc[q][3] = fmaf(v, lo.w, acc[q][3]);
+          acc[q][4] = fmaf(v, hi.x, acc[q][4]);
+          acc[q][5] = fmaf(v, hi.y, acc[q][5]);
+          acc[q][6] = fmaf(v, hi.z, acc[q][6]);
+          acc[q][7] = fmaf(v, hi.w, acc[q][7]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < nbb) {
+        const size_t o = ((size_t)(bb + q) * nrows + f) * BTPM_BLK;
+        float r[BTPM_BLK];
+#pragma unroll
+        for (int k = 0; k < BTPM_BLK; ++k) r[k] = al * ((x0 ? x0[o + k] : 0.0f) + acc[q][k]);
+        float4* yp = reinterpret_cast<float4*>(y + o);
+        yp[0] = make_float4(r[0], r[1], r[2], r[3]);
+        yp[1] = make_float4(r[4], r[5], r[6], r[7]);
+      }
+    }
+  }
+}
+
+// elementwise helpers over blocked arrays of n floats
+extern "C" __global__ void btpm_scale_cols(float* __restrict__ y, const float* __restrict__ x,
+                                           const float* __restrict__ scale, int ncols, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)((i / BTPM_BLK) % ncols);
+  y[i] = scale[c] * x[i];
+}
+
+// p_new = alpha*(d + q) has been written to y; record max |y - p| per
+// workgroup into part[] and copy y into p.
+extern "C" __global__ void btpm_delta_max(float* __restrict__ p, const float* __restrict__ y,
+                                          float* __restrict__ part, size_t n) {
+  __shared__ float red[256];
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float d = 0.0f;
+  if (i < n) {
+    d = fabsf(y[i] - p[i]);
+    p[i] = y[i];
+  }
+  red[threadIdx.x] = d;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// F = p / A  (A > 0)  else direct d
+extern "C" __global__ void btpm_finish_scatter(float* __restrict__ F, const float* __restrict__ p,
+                                               const float* __restrict__ d,
+                                               const float* __restrict__ albedo, int ncols,
+                                               size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)((i / BTPM_BLK) % ncols);
+  const float a = albedo[c];
+  F[i] = a > 0.0f ? p[i] / a : d[i];
+}
+
+// ---------------------------------------------------------------------------
+// per-set status after a rotation (one workgroup per set)
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(256) void btpm_status_kernel(
+    const int* __restrict__ set_ids, const btpm_set_dev* __restrict__ sets,
+    btpm_status_dev* __restrict__ status) {
+  __shared__ long long red_w[256];
+  __shared__ int red_a[256];
+  const int sid = set_ids[blockIdx.x];
+  const btpm_set_dev& S = sets[sid];
+  btpm_status_dev* st = status + sid;
+  if (st->done) return;
+  long long w = 0;
+  int a = 0;
+  for (int t = threadIdx.x; t < S.ntiles; t += 256) {
+    a += S.tile_active[t];
+    w += S.tile_work[t];
+  }
+  red_w[threadIdx.x] = w;
+  red_a[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      red_w[threadIdx.x] += red_w[threadIdx.x + s];
+      red_a[threadIdx.x] += red_a[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int rev = st->rev;          // index of the rotation just executed
+    st->active = red_a[0];
+    st->colsteps += red_w[0] * (long long)S.ntau;
+    st->rev = rev + 1;
+    if (rev > S.check_after && red_a[0] == 0) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (rev + 1 >= S.max_rev) {
+      st->done = 1;
+    }
+  }
+}

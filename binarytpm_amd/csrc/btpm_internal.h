@@ -1,0 +1,72 @@
+// btpm_internal.h -- device-visible descriptors shared by the HIP kernels and
+// the C++ host runtime of libbtpm.
+//
+// HBM layout of one column set (a smooth body or the crater columns of one
+// solve).  "Blocked SoA": time is cut into blocks of BTPM_BLK steps and each
+// block stores the BTPM_BLK values of a column contiguously,
+//     x[b][c][k]   (b = step / BLK, c = column, k = step % BLK),
+// so a wavefront of 64 consecutive columns moves 64 x 32 B = 2 KB of
+// contiguous memory per block (fully coalesced), while a single column's
+// steps still come in 32-byte runs for the once-per-rotation gathers.
+#pragma once
+#include <stdint.h>
+
+#define BTPM_NX 56          // depth nodes below the surface (reference NX)
+#define BTPM_BLK 8          // steps per time block
+#define BTPM_NS 100         // crater sub-facets per parent facet
+#define BTPM_SMOOTH_WG 256  // columns per smooth-conduction workgroup
+#define BTPM_CRATERS_WG 10  // craters per crater-conduction workgroup (1000 columns)
+#define BTPM_CRATER_THREADS 1024
+#define BTPM_PSTRIDE 64     // floats per column in the profile array (57 used)
+
+// kind of a column set
+enum { BTPM_SET_SMOOTH = 0, BTPM_SET_CRATER = 1 };
+
+// A set of columns advanced together rotation by rotation.
+struct btpm_set_dev {
+  int kind;           // BTPM_SET_SMOOTH or BTPM_SET_CRATER
+  int ncols;          // columns (crater: ngroups * 100)
+  int ngroups;        // freeze groups: smooth = ncols, crater = parent facets
+  int ntau;           // steps per cycle
+  int nb;             // time blocks = ceil(ntau / BLK)
+  int max_rev;        // rotation cap
+  int check_after;    // freeze test when rev > check_after
+  int inst;           // 1: instantaneous equilibrium (TI = 0)
+  float es;           // eps * sigma
+  float esia;         // ir_abs * eps * sigma (self-heating emission)
+  float tacc;         // freeze tolerance (K)
+  float newton_tol;   // Newton stop tolerance (K)
+  int newton_cap;     // Newton iteration cap
+  int pad_;
+  // arrays (device pointers)
+  const float* src;   // [nb][ncols][BLK] absorbed flux (1-A)F (+RGH)
+  float* T;           // [nb][ncols][BLK] surface T, last rotation (in place)
+  float* prof;        // [ncols][PSTRIDE] depth profile, K (node i at +i)
+  const float* r;     // [ngroups] FTCS number dtau/dz^2
+  const float* m1;    // [ngroups] Gamma sqrt(2 pi n / P) / dz
+  uint8_t* frozen;    // [ngroups]
+  float* dmax;        // [ngroups] max |dT| of the last executed rotation
+  const float* qext;  // [nb][ncols][BLK] facet self-heating flux (or null)
+  float* emis;        // [nb][ncols][BLK] esia T^4 for the next rotation (or null)
+  const float* vfA;   // crater: fp16 hi/lo view-factor tiles (see crater kernel)
+  // status, written by the kernels
+  int* tile_active;   // [ntiles] active groups after this rotation's test
+  int* tile_work;     // [ntiles] columns integrated during the rotation
+  int ntiles;
+  int tile_cols;      // columns per tile (smooth) / groups per tile (crater)
+};
+
+// per-set progress, device resident, advanced by btpm_status_kernel
+struct btpm_status_dev {
+  int rev;            // rotations completed
+  int active;         // groups not frozen
+  int done;           // 1 once every group froze or rev reached max_rev
+  int converged;      // 1 if done by freezing (not by the cap)
+  long long colsteps; // integrated column-steps (executed, excluding frozen)
+};
+
+// one workgroup's work item in a batched launch
+struct btpm_tile {
+  int set;            // index into the set table
+  int tile;           // tile index inside the set
+};

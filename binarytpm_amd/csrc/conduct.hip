@@ -1,0 +1,489 @@
+// conduct.hip -- persistent per-rotation conduction kernels for gfx950.
+//
+// One launch advances every listed tile by one full rotation (ntau steps):
+// the 57-node depth profile of each column lives in VGPRs for the whole
+// rotation, inputs/outputs stream through HBM in 8-step blocks (blocked SoA,
+// see btpm_internal.h) with the next block prefetched while the current one
+// is integrated, and the per-column rotation-to-rotation change is reduced
+// in-kernel for the freeze test.
+//
+// Algorithm per column and step (the reference's inertiadifference,
+// T175706/thermal.cpp:1895-1979, and the crater kernels finite_difference +
+// boundary_condition, T175706/thermal_GPU.cu:430-574):
+//   FTCS   T_i += r (T_{i+1} - 2 T_i + T_{i-1}),  i = 1..NX-1 (old values)
+//   bottom T_NX = T_{NX-1}
+//   SBC    Newton on (1-A)F + Q + m1 (T_1 - T_0) - es T_0^4 = 0 from the old
+//          T_0 until |dT| < tol, in FP64
+// where Q (self-heating) always uses the PREVIOUS rotation's surface
+// temperatures at the same step.
+//
+// Precision: the profile is held as offsets u_i = T_i - T_ref with a
+// per-column reference T_ref (the bottom node at rotation start).  The FTCS
+// operator is exact on constants, so the offsets integrate the same
+// recurrence while their FP32 rounding is set by the size of the offsets
+// (K) rather than of the absolute temperature (hundreds of K).
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "btpm_internal.h"
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+// Pointers read from the set descriptor are generic; casting them to the
+// global address space lets the compiler emit global (not flat) memory ops,
+// whose completion is tracked by vmcnt alone so prefetches stay in flight.
+#define GLOBAL_AS __attribute__((address_space(1)))
+typedef GLOBAL_AS float gfloat;
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// Newton root of the surface energy balance in FP64.  Returns the new
+// surface temperature; *iters receives the iteration count.
+__device__ __forceinline__ double sbc_newton(double src, double m1, double t1, double es,
+                                             double x, double tol, int cap) {
+  for (int it = 0; it < cap; ++it) {
+    const double x3 = x * x * x;
+    const double g = src + m1 * (t1 - x) - es * x3 * x;
+    const double gp = m1 + 4.0 * es * x3;
+    const double xn = x + g / gp;
+    const double d = fabs(xn - x);
+    x = xn;
+    if (d < tol) break;
+  }
+  return x;
+}
+
+// FTCS on the offset profile with the zero-flux bottom.
+template <int NX>
+__device__ __forceinline__ void ftcs(float (&u)[NX + 1], float r) {
+  float before = u[0];
+#pragma unroll
+  for (int i = 1; i < NX; ++i) {
+    const float cur = u[i];
+    const float lap = fmaf(-2.0f, cur, u[i + 1]) + before;
+    u[i] = fmaf(r, lap, cur);
+    before = cur;
+  }
+  u[NX] = u[NX - 1];
+}
+
+__device__ __forceinline__ void load8(const gfloat* p, float (&v)[BTPM_BLK]) {
+  const GLOBAL_AS float4_t* q = reinterpret_cast<const GLOBAL_AS float4_t*>(p);
+  const float4_t a = q[0], b = q[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+__device__ __forceinline__ void load4(const gfloat* p, float (&v)[BTPM_BLK], int k0) {
+  const float4_t a = *reinterpret_cast<const GLOBAL_AS float4_t*>(p + k0);
+  v[k0] = a.x; v[k0 + 1] = a.y; v[k0 + 2] = a.z; v[k0 + 3] = a.w;
+}
+
+// FTCS + bottom boundary + FP64 Newton SBC for one step; returns T_0.
+template <int NX>
+__device__ __forceinline__ float conduct_step(float (&u)[NX + 1], float r, double src, double m1,
+                                              double es, double Tref, double tol, int cap) {
+  ftcs<NX>(u, r);
+  const double x = sbc_newton(src, m1, Tref + (double)u[1], es, Tref + (double)u[0], tol, cap);
+  u[0] = (float)(x - Tref);
+  return (float)x;
+}
+
+// profile of one column: 57 nodes in a 64-float record (one base address,
+// immediate offsets), held as offsets from its bottom node
+template <int NX>
+__device__ __forceinline__ float load_profile(const gfloat* rec, float (&u)[NX + 1]) {
+#pragma unroll
+  for (int i = 0; i + 3 <= NX; i += 4) {
+    const float4_t a = *reinterpret_cast<const GLOBAL_AS float4_t*>(rec + i);
+    u[i] = a.x; u[i + 1] = a.y; u[i + 2] = a.z; u[i + 3] = a.w;
+  }
+#pragma unroll
+  for (int i = (NX + 1) & ~3; i <= NX; ++i) u[i] = rec[i];
+  const float tref = u[NX];
+#pragma unroll
+  for (int i = 0; i <= NX; ++i) u[i] -= tref;
+  return tref;
+}
+
+template <int NX>
+__device__ __forceinline__ void store_profile(gfloat* rec, const float (&u)[NX + 1], float tref) {
+#pragma unroll
+  for (int i = 0; i + 3 <= NX; i += 4)
+    *reinterpret_cast<GLOBAL_AS float4_t*>(rec + i) =
+        float4_t{tref + u[i], tref + u[i + 1], tref + u[i + 2], tref + u[i + 3]};
+#pragma unroll
+  for (int i = (NX + 1) & ~3; i <= NX; ++i) rec[i] = tref + u[i];
+}
+
+__device__ __forceinline__ void store8(gfloat* p, const float (&v)[BTPM_BLK]) {
+  GLOBAL_AS float4_t* q = reinterpret_cast<GLOBAL_AS float4_t*>(p);
+  q[0] = float4_t{v[0], v[1], v[2], v[3]};
+  q[1] = float4_t{v[4], v[5], v[6], v[7]};
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Smooth facets: one lane per column, BTPM_SMOOTH_WG columns per workgroup.
+// ---------------------------------------------------------------------------
+extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smooth(
+    const btpm_tile* __restrict__ tiles, const btpm_set_dev* __restrict__ sets,
+    const btpm_status_dev* __restrict__ status) {
+  constexpr int NX = BTPM_NX;
+  const btpm_tile tl = tiles[blockIdx.x];
+  const btpm_set_dev& S = sets[tl.set];
+  const btpm_status_dev st = status[tl.set];
+  if (st.done || st.rev >= S.max_rev) return;
+  const bool check = st.rev > S.check_after;
+
+  const int ncols = S.ncols;
+  const int c = tl.tile * BTPM_SMOOTH_WG + (int)threadIdx.x;
+  const bool valid = c < ncols;
+  const int cc = valid ? c : ncols - 1;  // clamped index for loads
+  const bool active = valid && (S.inst || !S.frozen[cc]);
+  const int nwork = __syncthreads_count(active);
+  if (nwork == 0) {
+    if (threadIdx.x == 0) {
+      S.tile_active[tl.tile] = 0;
+      S.tile_work[tl.tile] = 0;
+    }
+    return;
+  }
+
+  const float r = S.r[cc];
+  const double m1 = (double)S.m1[cc];
+  const double es = (double)S.es;
+  const double tol = (double)S.newton_tol;
+  const int cap = S.newton_cap;
+  // with facet self-heating the SpMM has written src + Q into qext
+  const gfloat* __restrict__ srcp = (const gfloat*)(S.qext != nullptr ? S.qext : S.src);
+  const gfloat* __restrict__ Tin = (const gfloat*)S.T;
+  gfloat* __restrict__ Tout = (gfloat*)S.T;
+  gfloat* __restrict__ Eout = (gfloat*)S.emis;
+  const bool has_e = S.emis != nullptr;
+  const float esia = S.esia;
+
+  float u[NX + 1];
+  const float Tref = load_profile<NX>((const gfloat*)S.prof + (size_t)cc * BTPM_PSTRIDE, u);
+  const double Trefd = (double)Tref;
+
+  const int nb = S.nb;
+  const int ntau = S.ntau;
+  const size_t bstride = (size_t)ncols * BTPM_BLK;
+  const size_t coff = (size_t)cc * BTPM_BLK;
+
+  // previous-rotation T of the block, parked in LDS (lane-private slots)
+  __shared__ float lds_p[BTPM_BLK][BTPM_SMOOTH_WG];
+  const int tid = (int)threadIdx.x;
+  float nsrc[BTPM_BLK], nprev[BTPM_BLK];
+  load8(srcp + coff, nsrc);
+  load8(Tin + coff, nprev);
+
+  float dm = 0.0f;
+  // the TI = 0 loop is a separate instantiation so the main loop stays lean
+  auto rotation = [&](auto inst_tag) {
+    constexpr bool INST = decltype(inst_tag)::value;
+    for (int b = 0; b < nb; ++b) {
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k) lds_p[k][tid] = nprev[k];
+      const bool more = b + 1 < nb;
+      const size_t o = (size_t)b * bstride + coff;
+      const size_t on = (size_t)(b + 1) * bstride + coff;
+      if (more) load8(Tin + on, nprev);
+      const int kend = min(BTPM_BLK, ntau - b * BTPM_BLK);
+      float out[BTPM_BLK];
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k) {
+        const float tp = lds_p[k][tid];
+        float t = tp;
+        if (k < kend) {
+          if constexpr (INST)  // instantaneous equilibrium (TI = 0), thermal.cpp:1985-2033
+            t = (float)sqrt(sqrt((double)nsrc[k] / es));
+          else
+            t = conduct_step<NX>(u, r, (double)nsrc[k], m1, es, Trefd, tol, cap);
+          dm = fmaxf(dm, fabsf(t - tp));
+        }
+        out[k] = t;
+        if (k == BTPM_BLK / 2 - 1 && more) load4(srcp + on, nsrc, 0);
+        if (k == BTPM_BLK - 1 && more) load4(srcp + on, nsrc, 4);
+      }
+      if (active) {
+        store8(Tout + o, out);
+        if (has_e) {
+#pragma unroll
+          for (int k = 0; k < BTPM_BLK; ++k) {
+            const float t = out[k];
+            out[k] = esia * (t * t) * (t * t);
+          }
+          store8(Eout + o, out);
+        }
+      }
+    }
+  };
+  if (S.inst) rotation(std::true_type{});
+  else rotation(std::false_type{});
+
+  bool still = active;
+  if (active) {
+    if (!S.inst) {
+      int cw = c;
+      asm volatile("" : "+v"(cw));  // keep the record address out of the step loop
+      store_profile<NX>((gfloat*)S.prof + (size_t)cw * BTPM_PSTRIDE, u, Tref);
+    }
+    S.dmax[c] = dm;
+    if (check && dm < S.tacc) {
+      still = false;
+      if (!S.inst) S.frozen[c] = 1;
+    }
+  }
+  const int nact = __syncthreads_count(still);
+  if (threadIdx.x == 0) {
+    S.tile_active[tl.tile] = nact;
+    S.tile_work[tl.tile] = S.inst ? 0 : nwork;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Crater columns: BTPM_CRATERS_WG craters x 100 sub-facets per workgroup of
+// 1024 lanes (lane -> sub-facet column, 24 idle lanes).  Every 8-step block
+// the workgroup evaluates the crater self-heating of all its craters for the
+// 8 steps at once as one MFMA product
+//     Q[j][n] = sum_i VF(j->i) E[i][n],   n = (crater, step in block)
+// with M = 100 (padded 112), K = 100 (padded 112), N = 80, using
+// v_mfma_f32_16x16x16_f16 and the three-product split
+//     VF E ~ VFhi Ehi + VFhi Elo + VFlo Ehi   (FP32 accumulation),
+// where x_hi = fp16(x) and x_lo = fp16(x - x_hi).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kMP = 112;           // padded M (receiving sub-facet j)
+constexpr int kKP = 112;           // padded K (emitting sub-facet i)
+constexpr int kLdsK = 116;         // LDS row stride in halves (bank spread)
+constexpr int kNP = BTPM_CRATERS_WG * BTPM_BLK;  // 80 columns of the product
+constexpr int kLdsQ = 116;         // Q row stride in floats
+constexpr int kMT = kMP / 16;      // 7 M tiles
+constexpr int kNT = kNP / 16;      // 5 N tiles
+constexpr int kKT = kKP / 16;      // 7 K chunks
+
+// Q[n][m] = sum_k A[m][k] B[n][k] over the padded 112 x 112 x 80 tile held in
+// LDS (A = VF^T hi/lo, B = E hi/lo), 7 x 5 output tiles spread over the 16
+// waves.  Fragment layout of v_mfma_f32_16x16x16f16: lane l supplies
+// A[m = l%16][k = 4(l/16) .. +3] and B[k = 4(l/16) .. +3][n = l%16] and
+// receives D[m = 4(l/16) + i][n = l%16], i = 0..3.
+__device__ __forceinline__ void crater_product(const _Float16 (*lds_a)[kMP * kLdsK],
+                                               const _Float16 (*lds_b)[kNP * kLdsK],
+                                               float* lds_q, int wave, int lane) {
+  for (int pt = wave; pt < kMT * kNT; pt += BTPM_CRATER_THREADS / 64) {
+    const int mt = pt / kNT, nt = pt - (pt / kNT) * kNT;
+    float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int am = mt * 16 + (lane & 15);
+    const int bn = nt * 16 + (lane & 15);
+    const int kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int kt = 0; kt < kKT; ++kt) {
+      const int k0 = kt * 16 + kq;
+      const half4_t ahi = *reinterpret_cast<const half4_t*>(&lds_a[0][am * kLdsK + k0]);
+      const half4_t alo = *reinterpret_cast<const half4_t*>(&lds_a[1][am * kLdsK + k0]);
+      const half4_t bhi = *reinterpret_cast<const half4_t*>(&lds_b[0][bn * kLdsK + k0]);
+      const half4_t blo = *reinterpret_cast<const half4_t*>(&lds_b[1][bn * kLdsK + k0]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(alo, bhi, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, blo, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, bhi, acc, 0, 0, 0);
+    }
+    float* q = &lds_q[bn * kLdsQ + mt * 16 + kq];
+    *reinterpret_cast<float4*>(q) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  }
+}
+
+// Stage VF^T hi/lo ([2][112][112] fp16 in global memory) into LDS.
+__device__ __forceinline__ void stage_vf(const _Float16* src, _Float16 (*lds_a)[kMP * kLdsK],
+                                         int tid) {
+  for (int e = tid; e < 2 * kMP * kKP; e += BTPM_CRATER_THREADS) {
+    const int h = e / (kMP * kKP);
+    const int rem = e - h * kMP * kKP;
+    const int m = rem / kKP, k = rem - m * kKP;
+    lds_a[h][m * kLdsK + k] = src[e];
+  }
+}
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_crater(
+    const btpm_tile* __restrict__ tiles, const btpm_set_dev* __restrict__ sets,
+    const btpm_status_dev* __restrict__ status) {
+  constexpr int NX = BTPM_NX;
+  constexpr int NS = BTPM_NS;
+  constexpr int P = BTPM_CRATERS_WG;
+  __shared__ _Float16 lds_a[2][kMP * kLdsK];   // VF^T hi / lo  (52 KB)
+  __shared__ _Float16 lds_b[2][kNP * kLdsK];   // E hi / lo     (37 KB)
+  __shared__ float lds_q[kNP * kLdsQ];         // product       (37 KB)
+  __shared__ float lds_p[BTPM_BLK][BTPM_CRATER_THREADS];  // T_prev (32 KB)
+  __shared__ unsigned lds_cmax[P];
+  __shared__ int lds_cnt;
+
+  const btpm_tile tl = tiles[blockIdx.x];
+  const btpm_set_dev& S = sets[tl.set];
+  const btpm_status_dev st = status[tl.set];
+  if (st.done || st.rev >= S.max_rev) return;
+  const bool check = st.rev > S.check_after;
+
+  const int tid = (int)threadIdx.x;
+  const int lc = tid < P * NS ? tid : P * NS - 1;  // local column (clamped)
+  const int lcr = lc / NS;                         // local crater
+  const int j = lc - lcr * NS;                     // sub-facet
+  const int g = tl.tile * P + lcr;                 // global crater
+  const bool valid = tid < P * NS && g < S.ngroups;
+  const int gg = valid ? g : 0;
+  const int c = gg * NS + j;                       // global column (clamped)
+  const bool active = valid && !S.frozen[gg];
+  const int nwork = __syncthreads_count(active);
+  if (nwork == 0) {
+    if (tid == 0) {
+      S.tile_active[tl.tile] = 0;
+      S.tile_work[tl.tile] = 0;
+    }
+    return;
+  }
+
+  // stage VF^T (hi, lo) once per launch; zero the padded E rows
+  {
+    stage_vf(reinterpret_cast<const _Float16*>(S.vfA), lds_a, tid);
+    for (int e = tid; e < 2 * kNP * kLdsK; e += BTPM_CRATER_THREADS)
+      (&lds_b[0][0])[e] = (_Float16)0.0f;
+    if (tid < P) lds_cmax[tid] = 0u;
+  }
+  __syncthreads();
+
+  const float r = S.r[gg];
+  const double m1 = (double)S.m1[gg];
+  const double es = (double)S.es;
+  const double tol = (double)S.newton_tol;
+  const int cap = S.newton_cap;
+  const float esia = S.esia;
+  const int ncols = S.ncols;
+
+  float u[NX + 1];
+  const float Tref = load_profile<NX>((const gfloat*)S.prof + (size_t)c * BTPM_PSTRIDE, u);
+  const double Trefd = (double)Tref;
+
+  const int nb = S.nb;
+  const int ntau = S.ntau;
+  const size_t bstride = (size_t)ncols * BTPM_BLK;
+  const size_t coff = (size_t)c * BTPM_BLK;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+
+  const gfloat* __restrict__ srcp = (const gfloat*)S.src;
+  const gfloat* __restrict__ Tin = (const gfloat*)S.T;
+  gfloat* __restrict__ Tout = (gfloat*)S.T;
+  float nsrc[BTPM_BLK], nprev[BTPM_BLK];
+  load8(srcp + coff, nsrc);
+  load8(Tin + coff, nprev);
+
+  float dm = 0.0f;
+  auto rotation = [&](auto inst_tag) {
+    constexpr bool INST = decltype(inst_tag)::value;
+    for (int b = 0; b < nb; ++b) {
+      // previous-rotation emission of the block -> LDS (fp16 hi/lo split);
+      // T_prev itself is parked in lane-private LDS slots
+      if (tid < P * NS) {
+#pragma unroll
+        for (int k = 0; k < BTPM_BLK; ++k) {
+          const float t = nprev[k];
+          const float e = esia * (t * t) * (t * t);
+          const _Float16 hi = (_Float16)e;
+          const _Float16 lo = (_Float16)(e - (float)hi);
+          const int n = lcr * BTPM_BLK + k;
+          lds_b[0][n * kLdsK + j] = hi;
+          lds_b[1][n * kLdsK + j] = lo;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k) lds_p[k][tid] = nprev[k];
+      const bool more = b + 1 < nb;
+      const size_t o = (size_t)b * bstride + coff;
+      const size_t on = (size_t)(b + 1) * bstride + coff;
+      if (more) load8(Tin + on, nprev);
+      __syncthreads();
+      crater_product(lds_a, lds_b, lds_q, wave, lane);
+      __syncthreads();
+
+      const int kend = min(BTPM_BLK, ntau - b * BTPM_BLK);
+      float out[BTPM_BLK];
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k) {
+        const float tp = lds_p[k][tid];
+        float t = tp;
+        if (k < kend) {
+          const double tot = (double)(nsrc[k] + lds_q[(lcr * BTPM_BLK + k) * kLdsQ + j]);
+          if constexpr (INST)  // roughness_inst_equilibrium, thermal_GPU.cu:581-641
+            t = (float)sqrt(sqrt(tot / es));
+          else
+            t = conduct_step<NX>(u, r, tot, m1, es, Trefd, tol, cap);
+          dm = fmaxf(dm, fabsf(t - tp));
+        }
+        out[k] = t;
+        if (k == BTPM_BLK / 2 - 1 && more) load4(srcp + on, nsrc, 0);
+        if (k == BTPM_BLK - 1 && more) load4(srcp + on, nsrc, 4);
+      }
+      if (active) store8(Tout + o, out);
+    }
+  };
+  if (S.inst) rotation(std::true_type{});
+  else rotation(std::false_type{});
+
+  // per-crater freeze test (converged kernel, T175706/thermal_GPU.cu:648-692)
+  if (active) {
+    if (!S.inst) {
+      int cw = c;
+      asm volatile("" : "+v"(cw));  // keep the record address out of the step loop
+      store_profile<NX>((gfloat*)S.prof + (size_t)cw * BTPM_PSTRIDE, u, Tref);
+    }
+    atomicMax(&lds_cmax[lcr], __float_as_uint(dm));
+  }
+  if (tid == 0) lds_cnt = 0;
+  __syncthreads();
+  if (active && j == 0) {
+    const float cm = __uint_as_float(lds_cmax[lcr]);
+    S.dmax[g] = cm;
+    bool still = true;
+    if (check && cm < S.tacc) {
+      S.frozen[g] = 1;
+      still = false;
+    }
+    if (still) atomicAdd(&lds_cnt, 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    S.tile_active[tl.tile] = lds_cnt;
+    S.tile_work[tl.tile] = S.inst ? 0 : nwork;
+  }
+}
+
+// Standalone crater self-heating product (tests and tools):
+//   q[n][j] = sum_i VF(j->i) e[n][i],  n < ncol, with the kernel's own split.
+extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_contract_kernel(
+    const _Float16* __restrict__ vfA, const float* __restrict__ e, float* __restrict__ q,
+    int ncol) {
+  __shared__ _Float16 lds_a[2][kMP * kLdsK];
+  __shared__ _Float16 lds_b[2][kNP * kLdsK];
+  __shared__ float lds_q[kNP * kLdsQ];
+  const int tid = (int)threadIdx.x;
+  const int n0 = blockIdx.x * kNP;
+  stage_vf(vfA, lds_a, tid);
+  for (int x = tid; x < kNP * kLdsK; x += BTPM_CRATER_THREADS) {
+    const int n = x / kLdsK, i = x - n * kLdsK;
+    float v = 0.0f;
+    if (i < BTPM_NS && n0 + n < ncol) v = e[(size_t)(n0 + n) * BTPM_NS + i];
+    const _Float16 hi = (_Float16)v;
+    lds_b[0][x] = hi;
+    lds_b[1][x] = (_Float16)(v - (float)hi);
+  }
+  __syncthreads();
+  crater_product(lds_a, lds_b, lds_q, tid >> 6, tid & 63);
+  __syncthreads();
+  for (int x = tid; x < kNP * BTPM_NS; x += BTPM_CRATER_THREADS) {
+    const int n = x / BTPM_NS, jj = x - n * BTPM_NS;
+    if (n0 + n < ncol) q[(size_t)(n0 + n) * BTPM_NS + jj] = lds_q[n * kLdsQ + jj];
+  }
+}

@@ -1,0 +1,1081 @@
+// host.hip -- C++ runtime of libbtpm: contexts, device-resident shapes and
+// solves, the batched rotation scheduler (hipGraph-captured launches,
+// HBM-budgeted admission, on-device convergence status) and the
+// continuous-batching server thread.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/btpm.h"
+#include "btpm_internal.h"
+
+// ---- kernels (conduct.hip, aux.hip) ----------------------------------------
+extern "C" __global__ void btpm_conduct_smooth(const btpm_tile*, const btpm_set_dev*,
+                                               const btpm_status_dev*);
+extern "C" __global__ void btpm_conduct_crater(const btpm_tile*, const btpm_set_dev*,
+                                               const btpm_status_dev*);
+extern "C" __global__ void btpm_contract_kernel(const _Float16*, const float*, float*, int);
+extern "C" __global__ void btpm_pack_blocked(const float*, float*, const float*, int, int, int);
+extern "C" __global__ void btpm_unpack_blocked(const float*, float*, int, int, int);
+extern "C" __global__ void btpm_pack_crater(const float*, float*, int, int, int);
+extern "C" __global__ void btpm_unpack_crater(const float*, float*, int, int, int);
+extern "C" __global__ void btpm_init_smooth(btpm_set_dev);
+extern "C" __global__ void btpm_init_crater(btpm_set_dev, const float*);
+extern "C" __global__ void btpm_shape_to_crater(const int*, const int*, const float*, const float*,
+                                                int, float, int, int, int, int, const float*,
+                                                const float*, float*);
+extern "C" __global__ void btpm_crater_scatter(const float*, const float*, const float*,
+                                               const float*, const float*, float, int, float, int,
+                                               int, int, int, float*);
+extern "C" __global__ void btpm_spmm_blocked(const int*, const int*, const float*, const float*,
+                                             const float*, const float*, float*, int, int, int,
+                                             const btpm_status_dev*);
+extern "C" __global__ void btpm_scale_cols(float*, const float*, const float*, int, size_t);
+extern "C" __global__ void btpm_delta_max(float*, const float*, float*, size_t);
+extern "C" __global__ void btpm_finish_scatter(float*, const float*, const float*, const float*,
+                                               int, size_t);
+extern "C" __global__ void btpm_status_kernel(const int*, const btpm_set_dev*, btpm_status_dev*);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+struct HipError {
+  int code;
+};
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      fail(e_ == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP, "%s: %s (%s:%d)", \
+           #x, hipGetErrorString(e_), __FILE__, __LINE__);                          \
+      throw HipError{e_ == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP};    \
+    }                                                                               \
+  } while (0)
+
+#define API_BEGIN try {
+#define API_END                                   \
+  }                                               \
+  catch (const HipError& e) {                     \
+    return e.code;                                \
+  }                                               \
+  catch (const std::bad_alloc&) {                 \
+    return fail(BTPM_ERR_NOMEM, "host allocation failed"); \
+  }                                               \
+  return BTPM_OK;
+
+constexpr int kMaxSets = 1024;
+const double kPi = 3.14159265358979323846;
+
+inline int nblocks(int ntau) { return (ntau + BTPM_BLK - 1) / BTPM_BLK; }
+inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// device buffer owned by RAII
+struct DBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return;
+    HIPCHK(hipMalloc(&p, bytes));
+    n = bytes;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <class T>
+void upload(DBuf& b, const T* h, size_t count, hipStream_t st) {
+  b.alloc(count * sizeof(T));
+  if (count) {
+    HIPCHK(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct btpm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  btpm_set_dev* d_sets = nullptr;
+  btpm_status_dev* d_status = nullptr;
+  btpm_status_dev* h_status = nullptr;  // pinned mirror
+  std::vector<int> free_slots;
+  std::mutex mu;                        // one scheduler at a time
+};
+
+struct btpm_shape {
+  btpm_ctx* ctx = nullptr;
+  btpm_params p{};
+  int nf = 0, nb = 0, nnz = 0;
+  int has_crater = 0;
+  DBuf F;          // [nb][nf][BLK] total facet illumination
+  DBuf albedo;     // [nf]
+  DBuf one_minus_a;
+  DBuf numrevs;    // [nf]
+  DBuf rowptr, colidx, vf;
+  DBuf cvf32;      // [100][100] crater VF (fp32, vf[i*100+j] = VF(j->i))
+  DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
+  DBuf cillum;     // [nb][nf*100][BLK] crater illumination (after scattering)
+  DBuf drf;        // [nnz][100]
+  std::vector<float> h_albedo, h_numrevs;
+};
+
+namespace {
+
+struct SetHost {
+  btpm_set_dev d{};
+  int slot = -1;
+  DBuf src, T, prof, r, m1, frozen, dmax, qext, emis, tile_active, tile_work;
+  int ntiles = 0;
+};
+
+}  // namespace
+
+struct btpm_solve {
+  btpm_ctx* ctx = nullptr;
+  btpm_shape* shape = nullptr;
+  float ti = 0.0f;
+  std::vector<float> ti_facet;
+  int with_craters = 0;
+  // 0 created, 1 smooth running, 2 crater running, 3 done
+  int phase = 0;
+  std::unique_ptr<SetHost> smooth, crater;
+  btpm_solve_info info{};
+  std::chrono::steady_clock::time_point t_submit;
+  std::mutex mu;
+  std::condition_variable cv;
+};
+
+// ---------------------------------------------------------------------------
+namespace {
+
+void fill_set_common(btpm_set_dev& d, const btpm_params& p) {
+  d.ntau = p.ntau;
+  d.nb = nblocks(p.ntau);
+  d.max_rev = p.max_rev;
+  d.check_after = p.check_after;
+  d.es = p.eps * p.sigma;
+  d.esia = p.ir_abs * p.eps * p.sigma;
+  d.tacc = p.tacc;
+  d.newton_tol = p.newton_tol;
+  d.newton_cap = p.newton_cap;
+}
+
+// per-group coefficients r = dtau/dz^2 and m1 = Gamma sqrt(2 pi n / P)/dz
+void coefficients(const btpm_shape* sh, const btpm_solve* s, std::vector<float>& r,
+                  std::vector<float>& m1, int* inst) {
+  const int nf = sh->nf;
+  const double dz = 8.0 / BTPM_NX;
+  r.resize(nf);
+  m1.resize(nf);
+  int all_zero = 1;
+  for (int f = 0; f < nf; ++f) {
+    const double n = sh->h_numrevs[f];
+    const double ti = s->ti_facet.empty() ? s->ti : s->ti_facet[f];
+    if (ti != 0.0) all_zero = 0;
+    r[f] = (float)((2.0 * kPi * n / sh->p.ntau) / (dz * dz));
+    m1[f] = (float)(ti * std::sqrt(2.0 * kPi * n / sh->p.rot_period) / dz);
+  }
+  *inst = all_zero;
+}
+
+// allocate and initialise the smooth column set of a solve
+void build_smooth(btpm_solve* s, hipStream_t st) {
+  btpm_shape* sh = s->shape;
+  auto S = std::make_unique<SetHost>();
+  btpm_set_dev& d = S->d;
+  fill_set_common(d, sh->p);
+  d.kind = BTPM_SET_SMOOTH;
+  d.ncols = sh->nf;
+  d.ngroups = sh->nf;
+  const size_t nser = (size_t)d.nb * d.ncols * BTPM_BLK;
+  std::vector<float> r, m1;
+  coefficients(sh, s, r, m1, &d.inst);
+  S->src.alloc(nser * sizeof(float));
+  S->T.alloc(nser * sizeof(float));
+  S->prof.alloc((size_t)BTPM_PSTRIDE * d.ncols * sizeof(float));
+  upload(S->r, r.data(), r.size(), st);
+  upload(S->m1, m1.data(), m1.size(), st);
+  S->frozen.alloc(d.ncols);
+  S->dmax.alloc(d.ncols * sizeof(float));
+  if (sh->nnz > 0) {
+    S->qext.alloc(nser * sizeof(float));
+    S->emis.alloc(nser * sizeof(float));
+  }
+  S->ntiles = (d.ncols + BTPM_SMOOTH_WG - 1) / BTPM_SMOOTH_WG;
+  S->tile_active.alloc(S->ntiles * sizeof(int));
+  S->tile_work.alloc(S->ntiles * sizeof(int));
+  d.src = S->src.as<float>();
+  d.T = S->T.as<float>();
+  d.prof = S->prof.as<float>();
+  d.r = S->r.as<float>();
+  d.m1 = S->m1.as<float>();
+  d.frozen = S->frozen.as<uint8_t>();
+  d.dmax = S->dmax.as<float>();
+  d.qext = S->qext.as<float>();
+  d.emis = S->emis.as<float>();
+  d.vfA = nullptr;
+  d.tile_active = S->tile_active.as<int>();
+  d.tile_work = S->tile_work.as<int>();
+  d.ntiles = S->ntiles;
+  d.tile_cols = BTPM_SMOOTH_WG;
+  // src = (1-A) F
+  hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st, S->src.as<float>(),
+                     sh->F.as<float>(), sh->one_minus_a.as<float>(), d.ncols, nser);
+  hipLaunchKernelGGL(btpm_init_smooth, dim3(grid1(d.ncols, 256)), dim3(256), 0, st, d);
+  HIPCHK(hipGetLastError());
+  s->smooth = std::move(S);
+}
+
+// RGH from the converged smooth temperatures, crater source and initial state
+void build_crater(btpm_solve* s, hipStream_t st) {
+  btpm_shape* sh = s->shape;
+  auto S = std::make_unique<SetHost>();
+  btpm_set_dev& d = S->d;
+  fill_set_common(d, sh->p);
+  d.kind = BTPM_SET_CRATER;
+  d.ngroups = sh->nf;
+  d.ncols = sh->nf * BTPM_NS;
+  const size_t nser = (size_t)d.nb * d.ncols * BTPM_BLK;
+  std::vector<float> r, m1;
+  coefficients(sh, s, r, m1, &d.inst);
+  S->src.alloc(nser * sizeof(float));
+  S->T.alloc(nser * sizeof(float));
+  S->prof.alloc((size_t)BTPM_PSTRIDE * d.ncols * sizeof(float));
+  upload(S->r, r.data(), r.size(), st);
+  upload(S->m1, m1.data(), m1.size(), st);
+  S->frozen.alloc(d.ngroups);
+  S->dmax.alloc(d.ngroups * sizeof(float));
+  S->ntiles = (d.ngroups + BTPM_CRATERS_WG - 1) / BTPM_CRATERS_WG;
+  S->tile_active.alloc(S->ntiles * sizeof(int));
+  S->tile_work.alloc(S->ntiles * sizeof(int));
+  d.src = S->src.as<float>();
+  d.T = S->T.as<float>();
+  d.prof = S->prof.as<float>();
+  d.r = S->r.as<float>();
+  d.m1 = S->m1.as<float>();
+  d.frozen = S->frozen.as<uint8_t>();
+  d.dmax = S->dmax.as<float>();
+  d.qext = nullptr;
+  d.emis = nullptr;
+  d.vfA = reinterpret_cast<const float*>(sh->cvfA.p);
+  d.tile_active = S->tile_active.as<int>();
+  d.tile_work = S->tile_work.as<int>();
+  d.ntiles = S->ntiles;
+  d.tile_cols = BTPM_CRATERS_WG;
+  // src = (1-A) illum + RGH (RGH from the converged smooth T)
+  if (sh->nnz > 0 && sh->drf.p) {
+    dim3 g(grid1((size_t)d.nb * BTPM_BLK, 64), sh->nf);
+    hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
+                       sh->colidx.as<int>(), sh->drf.as<float>(), s->smooth->T.as<float>(), 1,
+                       sh->p.eps * sh->p.sigma, sh->nf, sh->nf, sh->p.ntau, d.nb,
+                       sh->cillum.as<float>(), sh->one_minus_a.as<float>(), S->src.as<float>());
+  } else {
+    // (1-A_f) per crater column: scale with a per-column expansion
+    std::vector<float> oma((size_t)d.ncols);
+    for (int f = 0; f < sh->nf; ++f)
+      for (int j = 0; j < BTPM_NS; ++j) oma[(size_t)f * BTPM_NS + j] = 1.0f - sh->h_albedo[f];
+    DBuf tmp;
+    upload(tmp, oma.data(), oma.size(), st);
+    hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st,
+                       S->src.as<float>(), sh->cillum.as<float>(), tmp.as<float>(), d.ncols, nser);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  hipLaunchKernelGGL(btpm_init_crater, dim3(d.ngroups), dim3(128), 0, st, d,
+                     sh->cvf32.as<float>());
+  HIPCHK(hipGetLastError());
+  s->crater = std::move(S);
+}
+
+// smooth set no longer needed for stepping: keep T (results) and prof
+void shrink_smooth(SetHost* S) {
+  S->src.release();
+  S->qext.release();
+  S->emis.release();
+}
+
+size_t solve_bytes(const btpm_shape* sh, int with_craters) {
+  const size_t nb = nblocks(sh->p.ntau);
+  size_t b = (size_t)sh->nf * (nb * BTPM_BLK * (sh->nnz ? 4 : 2) + BTPM_NX + 1) * 4;
+  if (with_craters) b += (size_t)sh->nf * BTPM_NS * (nb * BTPM_BLK * 2 + BTPM_NX + 1) * 4;
+  return b + (1 << 20);
+}
+
+// ---------------------------------------------------------------------------
+// scheduler
+// ---------------------------------------------------------------------------
+struct Sched {
+  btpm_ctx* ctx;
+  btpm_batch_opts opt;
+  std::vector<btpm_solve*> active;
+  std::deque<btpm_solve*> pending;
+  // per-slot owner (set host) for status readback
+  std::map<int, std::pair<btpm_solve*, SetHost*>> slots;
+  bool dirty = true;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  DBuf d_tiles_s, d_tiles_c, d_ids;
+  int n_tiles_s = 0, n_tiles_c = 0, n_ids = 0;
+  btpm_batch_stats stats{};
+  size_t budget = 0, in_use = 0;
+  int craters_active = 0;
+  std::chrono::steady_clock::time_point t0;
+
+  explicit Sched(btpm_ctx* c, const btpm_batch_opts& o) : ctx(c), opt(o) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    budget = (size_t)(fr * (opt.mem_fraction > 0 ? opt.mem_fraction : 0.8));
+    t0 = std::chrono::steady_clock::now();
+  }
+  ~Sched() { drop_graph(); }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  int take_slot() {
+    if (ctx->free_slots.empty()) throw HipError{fail(BTPM_ERR_STATE, "too many resident sets")};
+    int s = ctx->free_slots.back();
+    ctx->free_slots.pop_back();
+    return s;
+  }
+
+  void add_set(btpm_solve* s, SetHost* S) {
+    S->slot = take_slot();
+    static const btpm_status_dev z{};
+    HIPCHK(hipMemcpyAsync(ctx->d_sets + S->slot, &S->d, sizeof(btpm_set_dev),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_status + S->slot, &z, sizeof z, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    slots[S->slot] = {s, S};
+    dirty = true;
+  }
+
+  void remove_set(SetHost* S) {
+    if (S->slot < 0) return;
+    slots.erase(S->slot);
+    ctx->free_slots.push_back(S->slot);
+    S->slot = -1;
+    dirty = true;
+  }
+
+  bool can_admit(btpm_solve* s) {
+    if (s->with_craters && opt.max_active_craters > 0 && craters_active >= opt.max_active_craters)
+      return false;
+    const size_t need = solve_bytes(s->shape, s->with_craters);
+    if (active.empty()) return true;  // always let one in
+    return in_use + need <= budget;
+  }
+
+  void admit() {
+    while (!pending.empty() && can_admit(pending.front())) {
+      btpm_solve* s = pending.front();
+      pending.pop_front();
+      in_use += solve_bytes(s->shape, s->with_craters);
+      if (s->with_craters) craters_active++;
+      build_smooth(s, ctx->stream);
+      s->phase = 1;
+      add_set(s, s->smooth.get());
+      active.push_back(s);
+    }
+  }
+
+  void rebuild_lists() {
+    std::vector<btpm_tile> ts, tc;
+    std::vector<int> ids;
+    for (auto& kv : slots) {
+      SetHost* S = kv.second.second;
+      ids.push_back(S->slot);
+      auto& v = S->d.kind == BTPM_SET_SMOOTH ? ts : tc;
+      for (int t = 0; t < S->ntiles; ++t) v.push_back(btpm_tile{S->slot, t});
+    }
+    n_tiles_s = (int)ts.size();
+    n_tiles_c = (int)tc.size();
+    n_ids = (int)ids.size();
+    upload(d_tiles_s, ts.data(), ts.size(), ctx->stream);
+    upload(d_tiles_c, tc.data(), tc.size(), ctx->stream);
+    upload(d_ids, ids.data(), ids.size(), ctx->stream);
+  }
+
+  // enqueue one rotation of every resident set
+  void enqueue_rotation(hipStream_t st) {
+    for (auto& kv : slots) {
+      SetHost* S = kv.second.second;
+      btpm_solve* s = kv.second.first;
+      if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext) {
+        btpm_shape* sh = s->shape;
+        const int bpt = 16;
+        dim3 g(grid1(sh->nf, 256), grid1(S->d.nb, bpt));
+        // qext = src + VF * emis: the total absorbed flux of this rotation
+        hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
+                           sh->colidx.as<int>(), sh->vf.as<float>(), S->emis.as<float>(),
+                           (const float*)S->src.as<float>(), (const float*)nullptr,
+                           S->qext.as<float>(), sh->nf, S->d.nb, bpt, ctx->d_status + S->slot);
+      }
+    }
+    if (n_tiles_s)
+      hipLaunchKernelGGL(btpm_conduct_smooth, dim3(n_tiles_s), dim3(BTPM_SMOOTH_WG), 0, st,
+                         d_tiles_s.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
+    if (n_tiles_c)
+      hipLaunchKernelGGL(btpm_conduct_crater, dim3(n_tiles_c), dim3(BTPM_CRATER_THREADS), 0, st,
+                         d_tiles_c.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
+    if (n_ids)
+      hipLaunchKernelGGL(btpm_status_kernel, dim3(n_ids), dim3(256), 0, st, d_ids.as<int>(),
+                         ctx->d_sets, ctx->d_status);
+  }
+
+  void run_chunk() {
+    const int R = std::max(1, opt.rotations_per_sync);
+    if (dirty) {
+      drop_graph();
+      rebuild_lists();
+      if (opt.use_graph) {
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream);
+        HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
+        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      }
+      dirty = false;
+    }
+    if (opt.use_graph) {
+      HIPCHK(hipGraphLaunch(gexec, ctx->stream));
+      stats.graph_launches++;
+    } else {
+      for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->h_status, ctx->d_status, sizeof(btpm_status_dev) * kMaxSets,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    stats.syncs++;
+  }
+
+  void finish_set(btpm_solve* s, SetHost* S) {
+    const btpm_status_dev& st = ctx->h_status[S->slot];
+    if (S->d.kind == BTPM_SET_SMOOTH) {
+      s->info.rotations = st.rev;
+      s->info.converged = st.converged;
+      s->info.column_steps = st.colsteps;
+    } else {
+      s->info.crater_rotations = st.rev;
+      s->info.crater_converged = st.converged;
+      s->info.crater_column_steps = st.colsteps;
+    }
+    stats.column_steps += st.colsteps;
+    stats.column_steps_all += (long long)st.rev * S->d.ncols * S->d.ntau;
+    remove_set(S);
+  }
+
+  void complete(btpm_solve* s) {
+    s->info.latency_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t_submit).count();
+    in_use -= std::min(in_use, solve_bytes(s->shape, s->with_craters));
+    if (s->with_craters) craters_active--;
+    stats.solves++;
+    if (s->info.converged && (!s->with_craters || s->info.crater_converged)) stats.converged++;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->phase = 3;
+    }
+    s->cv.notify_all();
+  }
+
+  // inspect statuses, advance phases, retire finished solves
+  void advance() {
+    std::vector<btpm_solve*> keep;
+    for (btpm_solve* s : active) {
+      bool alive = true;
+      if (s->phase == 1) {
+        SetHost* S = s->smooth.get();
+        if (ctx->h_status[S->slot].done) {
+          finish_set(s, S);
+          shrink_smooth(S);
+          if (s->with_craters) {
+            build_crater(s, ctx->stream);
+            s->phase = 2;
+            add_set(s, s->crater.get());
+          } else {
+            complete(s);
+            alive = false;
+          }
+        }
+      } else if (s->phase == 2) {
+        SetHost* S = s->crater.get();
+        if (ctx->h_status[S->slot].done) {
+          finish_set(s, S);
+          S->src.release();
+          complete(s);
+          alive = false;
+        }
+      }
+      if (alive) keep.push_back(s);
+    }
+    active.swap(keep);
+  }
+
+  void step() {
+    admit();
+    if (active.empty()) return;
+    run_chunk();
+    advance();
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int btpm_version(void) { return 100; }
+
+const char* btpm_last_error(void) { return g_err.c_str(); }
+
+void btpm_params_default(btpm_params* p, int ntau, float rot_period, float tacc) {
+  p->ntau = ntau;
+  p->nx = BTPM_NX;
+  p->rot_period = rot_period;
+  p->eps = 0.95f;
+  p->sigma = 5.67e-8f;
+  p->ir_abs = 1.0f;
+  p->tacc = tacc;
+  p->newton_tol = tacc / 10.0f;
+  p->max_rev = 301;
+  p->check_after = 5;
+  p->newton_cap = 200;
+}
+
+void btpm_batch_opts_default(btpm_batch_opts* o) {
+  o->rotations_per_sync = 4;
+  o->use_graph = 1;
+  o->max_active_craters = 0;
+  o->mem_fraction = 0.8;
+}
+
+int btpm_ctx_create(int device, btpm_ctx** out) {
+  if (!out) return fail(BTPM_ERR_ARG, "null out");
+  API_BEGIN
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(BTPM_ERR_ARG, "device %d of %d", device, n);
+  HIPCHK(hipSetDevice(device));
+  auto c = std::make_unique<btpm_ctx>();
+  c->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipMalloc(&c->d_sets, sizeof(btpm_set_dev) * kMaxSets));
+  HIPCHK(hipMalloc(&c->d_status, sizeof(btpm_status_dev) * kMaxSets));
+  HIPCHK(hipMemset(c->d_status, 0, sizeof(btpm_status_dev) * kMaxSets));
+  HIPCHK(hipHostMalloc(&c->h_status, sizeof(btpm_status_dev) * kMaxSets, hipHostMallocDefault));
+  for (int i = kMaxSets - 1; i >= 0; --i) c->free_slots.push_back(i);
+  *out = c.release();
+  API_END
+}
+
+int btpm_ctx_destroy(btpm_ctx* c) {
+  if (!c) return BTPM_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_sets);
+  (void)hipFree(c->d_status);
+  (void)hipHostFree(c->h_status);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BTPM_OK;
+}
+
+int btpm_ctx_mem_info(btpm_ctx* c, size_t* fr, size_t* tot) {
+  if (!c) return fail(BTPM_ERR_ARG, "null ctx");
+  API_BEGIN
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemGetInfo(fr, tot));
+  API_END
+}
+
+int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body,
+                      const btpm_crater* crater, btpm_shape** out) {
+  if (!ctx || !p || !body || !out) return fail(BTPM_ERR_ARG, "null argument");
+  if (p->nx != BTPM_NX) return fail(BTPM_ERR_ARG, "nx must be %d (got %d)", BTPM_NX, p->nx);
+  if (p->ntau < 1 || body->nf < 1 || !body->illum || !body->albedo)
+    return fail(BTPM_ERR_ARG, "empty body or ntau");
+  if ((body->nnz > 0) != (body->rowptr != nullptr))
+    return fail(BTPM_ERR_ARG, "nnz and rowptr disagree");
+  if (crater && !crater->vf) return fail(BTPM_ERR_ARG, "crater without view factors");
+  if (crater && !crater->illum && (!crater->shadow || !crater->cosang))
+    return fail(BTPM_ERR_ARG, "crater needs illum or shadow+cosang");
+  if (p->max_rev < 1 || p->newton_cap < 1 || !(p->tacc > 0) || !(p->newton_tol > 0))
+    return fail(BTPM_ERR_ARG, "bad convergence parameters");
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  auto sh = std::make_unique<btpm_shape>();
+  sh->ctx = ctx;
+  sh->p = *p;
+  const int nf = sh->nf = body->nf;
+  const int nt = p->ntau;
+  const int nb = sh->nb = nblocks(nt);
+  sh->nnz = body->nnz;
+  sh->h_albedo.assign(body->albedo, body->albedo + nf);
+  sh->h_numrevs.assign(nf, 1.0f);
+  if (body->numrevs) sh->h_numrevs.assign(body->numrevs, body->numrevs + nf);
+  std::vector<float> oma(nf);
+  for (int f = 0; f < nf; ++f) oma[f] = 1.0f - sh->h_albedo[f];
+  upload(sh->albedo, sh->h_albedo.data(), nf, st);
+  upload(sh->one_minus_a, oma.data(), nf, st);
+  upload(sh->numrevs, sh->h_numrevs.data(), nf, st);
+  if (body->nnz > 0) {
+    for (int f = 0; f < nf; ++f)
+      if (body->rowptr[f] > body->rowptr[f + 1]) return fail(BTPM_ERR_ARG, "rowptr not monotone");
+    if (body->rowptr[0] != 0 || body->rowptr[nf] != body->nnz)
+      return fail(BTPM_ERR_ARG, "rowptr does not span nnz");
+    for (int k = 0; k < body->nnz; ++k)
+      if (body->colidx[k] < 0 || body->colidx[k] >= nf)
+        return fail(BTPM_ERR_ARG, "colidx[%d]=%d out of range", k, body->colidx[k]);
+    upload(sh->rowptr, body->rowptr, nf + 1, st);
+    upload(sh->colidx, body->colidx, body->nnz, st);
+    upload(sh->vf, body->vf, body->nnz, st);
+  }
+  // facet illumination -> blocked
+  const size_t nser = (size_t)nb * nf * BTPM_BLK;
+  {
+    DBuf raw;
+    upload(raw, body->illum, (size_t)nf * nt, st);
+    sh->F.alloc(nser * sizeof(float));
+    hipLaunchKernelGGL(btpm_pack_blocked, dim3(grid1((size_t)nf * nb, 256)), dim3(256), 0, st,
+                       raw.as<float>(), sh->F.as<float>(), (const float*)nullptr, nf, nt, nb);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (body->scatter && body->nnz > 0) {
+    // singlescattering: F = D + VF (A D); multiplescattering: p = A(D + VF p)
+    DBuf D, AD, p, y, part;
+    D.alloc(nser * 4);
+    AD.alloc(nser * 4);
+    p.alloc(nser * 4);
+    y.alloc(nser * 4);
+    const unsigned nblk = grid1(nser, 256);
+    part.alloc((size_t)nblk * 4);
+    HIPCHK(hipMemcpyAsync(D.p, sh->F.p, nser * 4, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(btpm_scale_cols, dim3(nblk), dim3(256), 0, st, AD.as<float>(),
+                       D.as<float>(), sh->albedo.as<float>(), nf, nser);
+    const int bpt = 16;
+    dim3 g(grid1(nf, 256), grid1(nb, bpt));
+    hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
+                       sh->colidx.as<int>(), sh->vf.as<float>(), AD.as<float>(), D.as<float>(),
+                       (const float*)nullptr, sh->F.as<float>(), nf, nb, bpt,
+                       (const btpm_status_dev*)nullptr);
+    if (body->scatter >= 2) {
+      const float acc = body->scatter_acc > 0 ? body->scatter_acc : 1e-4f;
+      hipLaunchKernelGGL(btpm_scale_cols, dim3(nblk), dim3(256), 0, st, p.as<float>(),
+                         sh->F.as<float>(), sh->albedo.as<float>(), nf, nser);
+      std::vector<float> hpart(nblk);
+      for (int it = 0; it < 1000; ++it) {
+        hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
+                           sh->colidx.as<int>(), sh->vf.as<float>(), p.as<float>(), D.as<float>(),
+                           sh->albedo.as<float>(), y.as<float>(), nf, nb, bpt,
+                           (const btpm_status_dev*)nullptr);
+        hipLaunchKernelGGL(btpm_delta_max, dim3(nblk), dim3(256), 0, st, p.as<float>(),
+                           y.as<float>(), part.as<float>(), nser);
+        HIPCHK(hipMemcpyAsync(hpart.data(), part.p, nblk * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        float m = 0.0f;
+        for (float v : hpart) m = std::max(m, v);
+        if (!(m > acc)) break;
+      }
+      hipLaunchKernelGGL(btpm_finish_scatter, dim3(nblk), dim3(256), 0, st, sh->F.as<float>(),
+                         p.as<float>(), D.as<float>(), sh->albedo.as<float>(), nf, nser);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (crater) {
+    sh->has_crater = 1;
+    upload(sh->cvf32, crater->vf, (size_t)BTPM_NS * BTPM_NS, st);
+    // VF^T as fp16 hi/lo tiles [2][112][112]: A[m=j][k=i] = vf[i*100+j]
+    std::vector<_Float16> a(2 * 112 * 112, (_Float16)0.0f);
+    for (int jj = 0; jj < BTPM_NS; ++jj)
+      for (int ii = 0; ii < BTPM_NS; ++ii) {
+        const float v = crater->vf[ii * BTPM_NS + jj];
+        const _Float16 hi = (_Float16)v;
+        const _Float16 lo = (_Float16)(v - (float)hi);
+        a[jj * 112 + ii] = hi;
+        a[112 * 112 + jj * 112 + ii] = lo;
+      }
+    upload(sh->cvfA, a.data(), a.size(), st);
+    const size_t ncser = (size_t)nb * nf * BTPM_NS * BTPM_BLK;
+    sh->cillum.alloc(ncser * sizeof(float));
+    if (crater->drf && body->nnz > 0) upload(sh->drf, crater->drf, (size_t)body->nnz * BTPM_NS, st);
+    if (crater->illum) {
+      DBuf raw;
+      upload(raw, crater->illum, (size_t)nf * nt * BTPM_NS, st);
+      hipLaunchKernelGGL(btpm_pack_crater, dim3(grid1((size_t)nf * BTPM_NS * nb, 256)), dim3(256),
+                         0, st, raw.as<float>(), sh->cillum.as<float>(), nf, nt, nb);
+      HIPCHK(hipStreamSynchronize(st));
+    } else {
+      DBuf dsh, dcos, rgi_blk, rgi;
+      upload(dsh, crater->shadow, (size_t)nf * nt * BTPM_NS, st);
+      upload(dcos, crater->cosang, (size_t)nf * nt * BTPM_NS, st);
+      const float* rgip = nullptr;
+      if (sh->drf.p) {
+        // RGI = sum_g drf * A_g F_g(s): blocked over crater columns, then
+        // unpacked to the reference layout the scattering kernel reads
+        DBuf AF;
+        AF.alloc(nser * 4);
+        hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st,
+                           AF.as<float>(), sh->F.as<float>(), sh->albedo.as<float>(), nf, nser);
+        rgi_blk.alloc(ncser * 4);
+        dim3 g(grid1((size_t)nb * BTPM_BLK, 64), nf);
+        hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
+                           sh->colidx.as<int>(), sh->drf.as<float>(), AF.as<float>(), 0, 0.0f, nf,
+                           nf, nt, nb, (const float*)nullptr, (const float*)nullptr,
+                           rgi_blk.as<float>());
+        rgi.alloc((size_t)nf * nt * BTPM_NS * 4);
+        hipLaunchKernelGGL(btpm_unpack_crater, dim3(grid1((size_t)nf * BTPM_NS * nb, 256)),
+                           dim3(256), 0, st, rgi_blk.as<float>(), rgi.as<float>(), nf, nt, nb);
+        HIPCHK(hipStreamSynchronize(st));
+        rgi_blk.release();
+        rgip = rgi.as<float>();
+      }
+      const float acc = crater->scatter_acc > 0 ? crater->scatter_acc : 1e-4f;
+      hipLaunchKernelGGL(btpm_crater_scatter, dim3(nb, nf), dim3(128), 0, st,
+                         sh->cvf32.as<float>(), sh->albedo.as<float>(), dsh.as<float>(),
+                         dcos.as<float>(), rgip, crater->solar, crater->multiple, acc, 10000, nf,
+                         nt, nb, sh->cillum.as<float>());
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  *out = sh.release();
+  API_END
+}
+
+int btpm_shape_destroy(btpm_shape* sh) {
+  if (!sh) return BTPM_OK;
+  (void)hipSetDevice(sh->ctx->device);
+  (void)hipStreamSynchronize(sh->ctx->stream);
+  delete sh;
+  return BTPM_OK;
+}
+
+int btpm_shape_facet_illum(btpm_shape* sh, float* out) {
+  if (!sh || !out) return fail(BTPM_ERR_ARG, "null argument");
+  API_BEGIN
+  btpm_ctx* c = sh->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  DBuf raw;
+  raw.alloc((size_t)sh->nf * sh->p.ntau * 4);
+  hipLaunchKernelGGL(btpm_unpack_blocked, dim3(grid1((size_t)sh->nf * sh->nb, 256)), dim3(256), 0,
+                     c->stream, sh->F.as<float>(), raw.as<float>(), sh->nf, sh->p.ntau, sh->nb);
+  HIPCHK(hipMemcpyAsync(out, raw.p, raw.n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+int btpm_shape_crater_illum(btpm_shape* sh, float* out) {
+  if (!sh || !out) return fail(BTPM_ERR_ARG, "null argument");
+  if (!sh->has_crater) return fail(BTPM_ERR_STATE, "shape has no crater model");
+  API_BEGIN
+  btpm_ctx* c = sh->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  DBuf raw;
+  raw.alloc((size_t)sh->nf * sh->p.ntau * BTPM_NS * 4);
+  hipLaunchKernelGGL(btpm_unpack_crater, dim3(grid1((size_t)sh->nf * BTPM_NS * sh->nb, 256)),
+                     dim3(256), 0, c->stream, sh->cillum.as<float>(), raw.as<float>(), sh->nf,
+                     sh->p.ntau, sh->nb);
+  HIPCHK(hipMemcpyAsync(out, raw.p, raw.n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+int btpm_solve_create(btpm_ctx* ctx, btpm_shape* shape, float ti, const float* ti_per_facet,
+                      int with_craters, btpm_solve** out) {
+  if (!ctx || !shape || !out) return fail(BTPM_ERR_ARG, "null argument");
+  if (with_craters && !shape->has_crater) return fail(BTPM_ERR_ARG, "shape has no crater model");
+  if (!(ti >= 0.0f)) return fail(BTPM_ERR_ARG, "negative thermal inertia");
+  auto s = std::make_unique<btpm_solve>();
+  s->ctx = ctx;
+  s->shape = shape;
+  s->ti = ti;
+  if (ti_per_facet) s->ti_facet.assign(ti_per_facet, ti_per_facet + shape->nf);
+  s->with_craters = with_craters;
+  *out = s.release();
+  return BTPM_OK;
+}
+
+int btpm_solve_destroy(btpm_solve* s) {
+  if (!s) return BTPM_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return BTPM_OK;
+}
+
+int btpm_batch_run(btpm_ctx* ctx, btpm_solve** solves, int n, const btpm_batch_opts* opts,
+                   btpm_batch_stats* stats) {
+  if (!ctx || (n > 0 && !solves)) return fail(BTPM_ERR_ARG, "null argument");
+  for (int i = 0; i < n; ++i)
+    if (!solves[i] || solves[i]->phase != 0) return fail(BTPM_ERR_STATE, "solve %d not fresh", i);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  btpm_batch_opts o;
+  btpm_batch_opts_default(&o);
+  if (opts) o = *opts;
+  Sched sc(ctx, o);
+  for (int i = 0; i < n; ++i) {
+    solves[i]->t_submit = sc.t0;
+    sc.pending.push_back(solves[i]);
+  }
+  while (!sc.pending.empty() || !sc.active.empty()) sc.step();
+  sc.stats.wall_s =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - sc.t0).count();
+  if (stats) *stats = sc.stats;
+  API_END
+}
+
+int btpm_solve_run(btpm_solve* s) {
+  if (!s) return fail(BTPM_ERR_ARG, "null solve");
+  btpm_batch_opts o;
+  btpm_batch_opts_default(&o);
+  o.rotations_per_sync = 1;
+  o.use_graph = 0;
+  return btpm_batch_run(s->ctx, &s, 1, &o, nullptr);
+}
+
+int btpm_solve_get_info(btpm_solve* s, btpm_solve_info* info) {
+  if (!s || !info) return fail(BTPM_ERR_ARG, "null argument");
+  *info = s->info;
+  return BTPM_OK;
+}
+
+static int fetch_series(btpm_solve* s, SetHost* S, float* out, int crater) {
+  btpm_ctx* c = s->ctx;
+  API_BEGIN
+  HIPCHK(hipSetDevice(c->device));
+  const int nt = s->shape->p.ntau, nb = S->d.nb, nf = s->shape->nf;
+  DBuf raw;
+  raw.alloc((size_t)S->d.ncols * nt * 4);
+  if (crater)
+    hipLaunchKernelGGL(btpm_unpack_crater, dim3(grid1((size_t)S->d.ncols * nb, 256)), dim3(256),
+                       0, c->stream, S->T.as<float>(), raw.as<float>(), nf, nt, nb);
+  else
+    hipLaunchKernelGGL(btpm_unpack_blocked, dim3(grid1((size_t)S->d.ncols * nb, 256)), dim3(256),
+                       0, c->stream, S->T.as<float>(), raw.as<float>(), S->d.ncols, nt, nb);
+  HIPCHK(hipMemcpyAsync(out, raw.p, raw.n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  API_END
+}
+
+int btpm_solve_surface_temps(btpm_solve* s, float* out) {
+  if (!s || !out) return fail(BTPM_ERR_ARG, "null argument");
+  if (s->phase != 3 || !s->smooth) return fail(BTPM_ERR_STATE, "solve not finished");
+  return fetch_series(s, s->smooth.get(), out, 0);
+}
+
+int btpm_solve_crater_temps(btpm_solve* s, float* out) {
+  if (!s || !out) return fail(BTPM_ERR_ARG, "null argument");
+  if (s->phase != 3 || !s->crater) return fail(BTPM_ERR_STATE, "no finished crater solve");
+  return fetch_series(s, s->crater.get(), out, 1);
+}
+
+static int fetch_profile(btpm_solve* s, SetHost* S, float* out, int crater) {
+  btpm_ctx* c = s->ctx;
+  API_BEGIN
+  HIPCHK(hipSetDevice(c->device));
+  const int nc = S->d.ncols, np = BTPM_NX + 1;
+  std::vector<float> h((size_t)BTPM_PSTRIDE * nc);
+  HIPCHK(hipMemcpyAsync(h.data(), S->prof.p, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // device [i][c] -> smooth [f][i], crater [f][i][j]
+  if (!crater) {
+    for (int f = 0; f < nc; ++f)
+      for (int i = 0; i < np; ++i) out[(size_t)f * np + i] = h[(size_t)f * BTPM_PSTRIDE + i];
+  } else {
+    const int nf = nc / BTPM_NS;
+    for (int f = 0; f < nf; ++f)
+      for (int i = 0; i < np; ++i)
+        for (int j = 0; j < BTPM_NS; ++j)
+          out[((size_t)f * np + i) * BTPM_NS + j] = h[((size_t)f * BTPM_NS + j) * BTPM_PSTRIDE + i];
+  }
+  API_END
+}
+
+int btpm_solve_profile(btpm_solve* s, float* out) {
+  if (!s || !out) return fail(BTPM_ERR_ARG, "null argument");
+  if (s->phase != 3 || !s->smooth) return fail(BTPM_ERR_STATE, "solve not finished");
+  return fetch_profile(s, s->smooth.get(), out, 0);
+}
+
+int btpm_solve_crater_profile(btpm_solve* s, float* out) {
+  if (!s || !out) return fail(BTPM_ERR_ARG, "null argument");
+  if (s->phase != 3 || !s->crater) return fail(BTPM_ERR_STATE, "no finished crater solve");
+  return fetch_profile(s, s->crater.get(), out, 1);
+}
+
+int btpm_crater_contract(btpm_ctx* ctx, const float* vf, const float* e, int ncol, float* q) {
+  if (!ctx || !vf || !e || !q || ncol < 1) return fail(BTPM_ERR_ARG, "bad argument");
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  std::vector<_Float16> a(2 * 112 * 112, (_Float16)0.0f);
+  for (int jj = 0; jj < BTPM_NS; ++jj)
+    for (int ii = 0; ii < BTPM_NS; ++ii) {
+      const float v = vf[ii * BTPM_NS + jj];
+      const _Float16 hi = (_Float16)v;
+      a[jj * 112 + ii] = hi;
+      a[112 * 112 + jj * 112 + ii] = (_Float16)(v - (float)hi);
+    }
+  DBuf dA, dE, dQ;
+  upload(dA, a.data(), a.size(), st);
+  upload(dE, e, (size_t)ncol * BTPM_NS, st);
+  dQ.alloc((size_t)ncol * BTPM_NS * 4);
+  const int per = BTPM_CRATERS_WG * BTPM_BLK;  // product columns per workgroup
+  hipLaunchKernelGGL(btpm_contract_kernel, dim3((ncol + per - 1) / per), dim3(BTPM_CRATER_THREADS),
+                     0, st, dA.as<_Float16>(), dE.as<float>(), dQ.as<float>(), ncol);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(q, dQ.p, dQ.n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  API_END
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// server: continuous batching on a host thread
+// ===========================================================================
+struct btpm_server {
+  btpm_ctx* ctx = nullptr;
+  btpm_batch_opts opt{};
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<btpm_solve*> inbox;
+  bool stop = false;
+  std::atomic<int> err{BTPM_OK};
+  std::string errmsg;
+  btpm_batch_stats stats{};
+};
+
+static void server_loop(btpm_server* srv) {
+  std::lock_guard<std::mutex> glk(srv->ctx->mu);
+  try {
+    HIPCHK(hipSetDevice(srv->ctx->device));
+    Sched sc(srv->ctx, srv->opt);
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        if (sc.active.empty() && sc.pending.empty())
+          srv->cv.wait(lk, [&] { return srv->stop || !srv->inbox.empty(); });
+        while (!srv->inbox.empty()) {
+          btpm_solve* s = srv->inbox.front();
+          srv->inbox.pop_front();
+          sc.pending.push_back(s);
+        }
+        if (srv->stop && sc.active.empty() && sc.pending.empty()) break;
+      }
+      sc.step();
+    }
+    sc.stats.wall_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - sc.t0).count();
+    srv->stats = sc.stats;
+  } catch (const HipError& e) {
+    srv->err = e.code;
+    srv->errmsg = g_err;
+  }
+}
+
+extern "C" {
+
+int btpm_server_start(btpm_ctx* ctx, const btpm_batch_opts* opts, btpm_server** out) {
+  if (!ctx || !out) return fail(BTPM_ERR_ARG, "null argument");
+  auto srv = std::make_unique<btpm_server>();
+  srv->ctx = ctx;
+  btpm_batch_opts_default(&srv->opt);
+  if (opts) srv->opt = *opts;
+  srv->th = std::thread(server_loop, srv.get());
+  *out = srv.release();
+  return BTPM_OK;
+}
+
+int btpm_server_submit(btpm_server* srv, btpm_solve* s) {
+  if (!srv || !s) return fail(BTPM_ERR_ARG, "null argument");
+  if (s->phase != 0) return fail(BTPM_ERR_STATE, "solve already used");
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->stop) return fail(BTPM_ERR_STATE, "server stopping");
+    s->t_submit = std::chrono::steady_clock::now();
+    srv->inbox.push_back(s);
+  }
+  srv->cv.notify_all();
+  return BTPM_OK;
+}
+
+int btpm_server_wait(btpm_server* srv, btpm_solve* s, double timeout_s) {
+  if (!srv || !s) return fail(BTPM_ERR_ARG, "null argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  const bool ok = s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                                 [&] { return s->phase == 3 || srv->err != BTPM_OK; });
+  if (srv->err != BTPM_OK) return fail(srv->err, "server failed: %s", srv->errmsg.c_str());
+  if (!ok) return fail(BTPM_ERR_STATE, "timeout");
+  return BTPM_OK;
+}
+
+int btpm_server_stop(btpm_server* srv, btpm_batch_stats* stats) {
+  if (!srv) return BTPM_OK;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stop = true;
+  }
+  srv->cv.notify_all();
+  if (srv->th.joinable()) srv->th.join();
+  const int err = srv->err;
+  if (stats) *stats = srv->stats;
+  std::string msg = srv->errmsg;
+  delete srv;
+  if (err != BTPM_OK) return fail(err, "server failed: %s", msg.c_str());
+  return BTPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,274 @@
+"""Python front end of libbtpm: the Thermal stage's solve objects.
+
+Mirrors the reference's thermal-stage operations (single_model's per-TI
+solve: initial temperatures, rotations to convergence with per-facet
+freezing, then the crater columns driven by the shape->crater heating;
+175706/Run-04/thermal.cpp:3465-3818) over the C ABI of include/btpm.h.
+Arrays use the reference layouts: facet series [nf][ntau], crater series
+[nf][ntau][100], crater view factors vf[i*100+j] = VF(j->i).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, Params, SolveInfo, check, lib)
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+def _f32(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != tuple(shape):
+        a = a.reshape(shape)
+    return a
+
+
+def _ptr(a, t=_fp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def params(ntau: int, rot_period: float, tacc: float = 0.01, **kw) -> Params:
+    """Model constants (reference globals, thermal.cpp:14-83); keyword
+    arguments override eps, sigma, ir_abs, newton_tol, max_rev, check_after,
+    newton_cap."""
+    p = Params()
+    lib().btpm_params_default(C.byref(p), int(ntau), float(rot_period), float(tacc))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One GPU and its stream (set_cuda_device, thermal_GPU.cu:698-702)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().btpm_ctx_create(int(device), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def mem_info(self):
+        fr, tot = C.c_size_t(), C.c_size_t()
+        check(lib().btpm_ctx_mem_info(self._h, C.byref(fr), C.byref(tot)))
+        return fr.value, tot.value
+
+    def close(self):
+        if self._h:
+            lib().btpm_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+@dataclass
+class CraterModel:
+    """Crater (roughness) inputs; see btpm_crater in include/btpm.h."""
+    vf: np.ndarray                        # [100][100], vf[i*100+j] = VF(j->i)
+    illum: Optional[np.ndarray] = None    # [nf][ntau][100] total illumination
+    shadow: Optional[np.ndarray] = None   # [nf][ntau][100] sunlit fraction
+    cosang: Optional[np.ndarray] = None   # [nf][ntau][100] cos incidence
+    solar: float = 0.0
+    multiple: bool = True
+    scatter_acc: float = 1e-4
+    drf: Optional[np.ndarray] = None      # [nnz][100] shape->crater view factors
+
+
+class Shape:
+    """Device-resident inputs shared by solves (facet illumination, view
+    factors, crater model and crater illumination after scattering)."""
+
+    def __init__(self, ctx: Context, p: Params, illum, albedo, numrevs=None, csr=None,
+                 scatter: int = 0, scatter_acc: float = 1e-4,
+                 crater: Optional[CraterModel] = None):
+        illum = _f32(illum)
+        nf, nt = illum.shape
+        if nt != p.ntau:
+            raise ValueError(f"illum has {nt} steps, params.ntau = {p.ntau}")
+        self.nf, self.ntau = nf, nt
+        self.params = p
+        keep = [illum]
+        albedo = _f32(np.broadcast_to(np.asarray(albedo, dtype=np.float32), (nf,)))
+        keep.append(albedo)
+        nr = None if numrevs is None else _f32(np.broadcast_to(np.asarray(numrevs, np.float32), (nf,)))
+        keep.append(nr)
+        b = Body()
+        b.nf = nf
+        b.illum, b.albedo, b.numrevs = _ptr(illum), _ptr(albedo), _ptr(nr)
+        b.scatter, b.scatter_acc = int(scatter), float(scatter_acc)
+        self.nnz = 0
+        if csr is not None:
+            rp = np.ascontiguousarray(csr[0], dtype=np.int32)
+            ci = np.ascontiguousarray(csr[1], dtype=np.int32)
+            vv = _f32(csr[2])
+            keep += [rp, ci, vv]
+            b.nnz = int(rp[-1])
+            b.rowptr, b.colidx, b.vf = _ptr(rp, _ip), _ptr(ci, _ip), _ptr(vv)
+            self.nnz = b.nnz
+        cr_p = None
+        self.has_crater = crater is not None
+        if crater is not None:
+            cr = Crater()
+            vf = _f32(crater.vf, (NS * NS,))
+            keep.append(vf)
+            cr.vf = _ptr(vf)
+            for name in ("illum", "shadow", "cosang"):
+                arr = getattr(crater, name)
+                if arr is not None:
+                    arr = _f32(arr, (nf, nt, NS))
+                    keep.append(arr)
+                    setattr(cr, name, _ptr(arr))
+            if crater.drf is not None:
+                drf = _f32(crater.drf, (self.nnz, NS))
+                keep.append(drf)
+                cr.drf = _ptr(drf)
+            cr.solar = float(crater.solar)
+            cr.multiple = int(bool(crater.multiple))
+            cr.scatter_acc = float(crater.scatter_acc)
+            cr_p = C.byref(cr)
+        h = C.c_void_p()
+        check(lib().btpm_shape_create(ctx.handle, C.byref(p), C.byref(b), cr_p, C.byref(h)))
+        del keep
+        self._h = h
+        self.ctx = ctx
+
+    @property
+    def handle(self):
+        return self._h
+
+    def facet_illum(self) -> np.ndarray:
+        out = np.empty((self.nf, self.ntau), dtype=np.float32)
+        check(lib().btpm_shape_facet_illum(self._h, _ptr(out)))
+        return out
+
+    def crater_illum(self) -> np.ndarray:
+        out = np.empty((self.nf, self.ntau, NS), dtype=np.float32)
+        check(lib().btpm_shape_crater_illum(self._h, _ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().btpm_shape_destroy(self._h)
+            self._h = None
+
+
+class Solve:
+    """One thermal-inertia solve of a shape (smooth facets, then craters)."""
+
+    def __init__(self, ctx: Context, shape: Shape, ti: float, ti_per_facet=None,
+                 with_craters: bool = False):
+        tpf = None if ti_per_facet is None else _f32(ti_per_facet, (shape.nf,))
+        h = C.c_void_p()
+        check(lib().btpm_solve_create(ctx.handle, shape.handle, float(ti), _ptr(tpf),
+                                      int(bool(with_craters)), C.byref(h)))
+        self._h = h
+        self.shape = shape
+        self.ti = float(ti)
+        self.with_craters = bool(with_craters)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def run(self):
+        """Run alone with a host convergence check every rotation."""
+        check(lib().btpm_solve_run(self._h))
+        return self
+
+    def info(self) -> SolveInfo:
+        inf = SolveInfo()
+        check(lib().btpm_solve_get_info(self._h, C.byref(inf)))
+        return inf
+
+    def surface_temps(self) -> np.ndarray:
+        out = np.empty((self.shape.nf, self.shape.ntau), dtype=np.float32)
+        check(lib().btpm_solve_surface_temps(self._h, _ptr(out)))
+        return out
+
+    def crater_temps(self) -> np.ndarray:
+        out = np.empty((self.shape.nf, self.shape.ntau, NS), dtype=np.float32)
+        check(lib().btpm_solve_crater_temps(self._h, _ptr(out)))
+        return out
+
+    def profile(self) -> np.ndarray:
+        out = np.empty((self.shape.nf, NX + 1), dtype=np.float32)
+        check(lib().btpm_solve_profile(self._h, _ptr(out)))
+        return out
+
+    def crater_profile(self) -> np.ndarray:
+        out = np.empty((self.shape.nf, NX + 1, NS), dtype=np.float32)
+        check(lib().btpm_solve_crater_profile(self._h, _ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().btpm_solve_destroy(self._h)
+            self._h = None
+
+
+def batch_opts(rotations_per_sync=4, use_graph=True, max_active_craters=0,
+               mem_fraction=0.8) -> BatchOpts:
+    o = BatchOpts()
+    lib().btpm_batch_opts_default(C.byref(o))
+    o.rotations_per_sync = int(rotations_per_sync)
+    o.use_graph = int(bool(use_graph))
+    o.max_active_craters = int(max_active_craters)
+    o.mem_fraction = float(mem_fraction)
+    return o
+
+
+def batch_run(ctx: Context, solves: Sequence[Solve], opts: Optional[BatchOpts] = None
+              ) -> BatchStats:
+    arr = (C.c_void_p * len(solves))(*[s.handle for s in solves])
+    st = BatchStats()
+    o = opts if opts is not None else batch_opts()
+    check(lib().btpm_batch_run(ctx.handle, arr, len(solves), C.byref(o), C.byref(st)))
+    return st
+
+
+class Server:
+    """Continuous-batching request loop (btpm_server_*)."""
+
+    def __init__(self, ctx: Context, opts: Optional[BatchOpts] = None):
+        h = C.c_void_p()
+        o = opts if opts is not None else batch_opts()
+        check(lib().btpm_server_start(ctx.handle, C.byref(o), C.byref(h)))
+        self._h = h
+
+    def submit(self, s: Solve):
+        check(lib().btpm_server_submit(self._h, s.handle))
+
+    def wait(self, s: Solve, timeout_s: float = 600.0):
+        check(lib().btpm_server_wait(self._h, s.handle, float(timeout_s)))
+
+    def stop(self) -> BatchStats:
+        st = BatchStats()
+        h, self._h = self._h, None
+        if h:
+            check(lib().btpm_server_stop(h, C.byref(st)))
+        return st
+
+
+def crater_contract(ctx: Context, vf, e) -> np.ndarray:
+    """q[n][j] = sum_i VF(j->i) e[n][i] through the MFMA split product."""
+    vf = _f32(vf, (NS * NS,))
+    e = _f32(e)
+    n = e.shape[0]
+    q = np.empty((n, NS), dtype=np.float32)
+    check(lib().btpm_crater_contract(ctx.handle, _ptr(vf), _ptr(e), n, _ptr(q)))
+    return q

@@ -1,0 +1,208 @@
+/*
+ * btpm.h -- C ABI of libbtpm, the MI355X (gfx950) Thermal stage of the
+ * binary-asteroid ATPM: per-column 1-D explicit conduction with a Newton
+ * surface energy balance, facet self-heating, crater sub-facet scattering
+ * and re-emission, rotation-to-rotation convergence with per-column freezing.
+ *
+ * Plain pointers and sizes only.  Host arrays are read during the call that
+ * receives them; the library owns every device buffer.  All functions return
+ * BTPM_OK (0) or a negative BTPM_ERR_* code; btpm_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   - the in-process smooth path of `single_model`:
+ *     initialisetemperatures / inertiadifference / instequilibrium /
+ *     shape_converged, 175706/Run-04/thermal.cpp:1861-2033, 2422-2454,
+ *     3590-3607 (no ABI in the reference; these entry points add one);
+ *   - the crater path behind the extern "C" CUDA host API,
+ *     175706/Run-04/thermal_GPU.cu:698-1279 (also available symbol for
+ *     symbol through btpm_compat.h);
+ *   - the host loops load_GPU_roughness_globalillumination /
+ *     _globalheating, 175706/Run-04/thermal.cpp:2693-2803;
+ *   - singlescattering / multiplescattering, thermal.cpp:1745-1855.
+ */
+#ifndef BTPM_H
+#define BTPM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BTPM_OK 0
+#define BTPM_ERR_ARG (-1)     /* invalid argument or size                 */
+#define BTPM_ERR_HIP (-2)     /* HIP runtime failure                      */
+#define BTPM_ERR_NOMEM (-3)   /* device memory exhausted                  */
+#define BTPM_ERR_STATE (-4)   /* call not valid in the object's state     */
+
+#define BTPM_NS 100           /* crater sub-facets per parent facet       */
+#define BTPM_NX 56            /* depth nodes below the surface            */
+
+typedef struct btpm_ctx btpm_ctx;
+typedef struct btpm_shape btpm_shape;
+typedef struct btpm_solve btpm_solve;
+typedef struct btpm_server btpm_server;
+
+/* Model constants.  Field meanings follow the reference globals of
+ * 175706/Run-04/thermal.cpp:14-83 (NTAU, NX, rotation_period, EPS, SIGMA,
+ * IR_ABS, TACC); newton_tol is TACC/10 in the reference (:1936), max_rev the
+ * 301-rotation cap (:3592) and check_after the `rev > 5` gate (:3600).
+ * newton_cap bounds the Newton loop, which the reference leaves unbounded.  */
+typedef struct {
+    int ntau;
+    int nx;              /* must be BTPM_NX */
+    float rot_period;    /* s */
+    float eps;
+    float sigma;
+    float ir_abs;
+    float tacc;          /* K */
+    float newton_tol;    /* K */
+    int max_rev;
+    int check_after;
+    int newton_cap;
+} btpm_params;
+
+/* A body: its facets' illumination and the sparse facet view factors.
+ * illum is [nf][ntau] (facet-major, the reference's totalillumination).
+ * If scatter != 0, illum holds the DIRECT illumination solar*cos(i)*shadow
+ * and the library applies singlescattering (+ multiplescattering when
+ * scatter == 2) on the GPU with scatter_acc (W m^-2).
+ * CSR rows are receiving facets: row f lists (colidx[k], vf[k]) = (g, VF_fg)
+ * as in <shape>_selfheating_list.txt (T3905/main.cpp:418-434).              */
+typedef struct {
+    int nf;
+    const float *illum;
+    const float *albedo;     /* [nf] */
+    const float *numrevs;    /* [nf] revolutions per cycle, NULL = 1 */
+    int nnz;
+    const int *rowptr;       /* [nf+1] or NULL */
+    const int *colidx;       /* [nnz] */
+    const float *vf;         /* [nnz] */
+    int scatter;             /* 0: illum is total; 1: single; 2: multiple */
+    float scatter_acc;
+} btpm_body;
+
+/* The crater (roughness) model attached to every facet of a body.
+ * vf[i*100+j] = VF(j->i), the transposed storage that
+ * initialise_GPU_constants receives (T3905/main.cpp:1994-2000).
+ * Crater illumination is either given as `illum` [nf][ntau][100] (total,
+ * after scattering) or computed on the GPU from shadow/cosang
+ * [nf][ntau][100], `solar` and the shape->crater global illumination
+ * (roughness_single/multiple_scattering, thermal_GPU.cu:241-341).
+ * drf [nnz][100] (optional) are the shape->crater view factors per CSR entry
+ * of the body (DepRoughFacet::viewfactor(1..100), thermal.cpp:318-334); they
+ * drive RGI and RGH.                                                         */
+typedef struct {
+    const float *vf;
+    const float *illum;
+    const float *shadow;
+    const float *cosang;
+    float solar;
+    int multiple;            /* multiple scattering */
+    float scatter_acc;
+    const float *drf;
+} btpm_crater;
+
+typedef struct {
+    int rotations;               /* smooth rotations executed            */
+    int converged;               /* smooth: every facet froze            */
+    long long column_steps;      /* smooth: integrated column-steps      */
+    int crater_rotations;
+    int crater_converged;
+    long long crater_column_steps;
+    double latency_s;            /* admission to completion (batch/server) */
+} btpm_solve_info;
+
+typedef struct {
+    int rotations_per_sync;      /* rotations enqueued between host checks */
+    int use_graph;               /* capture each rotation set in a hipGraph */
+    int max_active_craters;      /* crater solves resident at once (0: by memory) */
+    double mem_fraction;         /* share of free HBM the batch may hold   */
+} btpm_batch_opts;
+
+typedef struct {
+    double wall_s;
+    int solves;
+    int converged;
+    long long column_steps;      /* integrated (executed) column-steps     */
+    long long column_steps_all;  /* including frozen columns              */
+    int graph_launches;
+    int syncs;
+} btpm_batch_stats;
+
+/* ---- library ------------------------------------------------------------ */
+int btpm_version(void);
+const char *btpm_last_error(void);
+void btpm_params_default(btpm_params *p, int ntau, float rot_period, float tacc);
+void btpm_batch_opts_default(btpm_batch_opts *o);
+
+/* set_cuda_device (thermal_GPU.cu:698-702) + module state: one context per
+ * device, each with its own stream.                                         */
+int btpm_ctx_create(int device, btpm_ctx **out);
+int btpm_ctx_destroy(btpm_ctx *ctx);
+int btpm_ctx_mem_info(btpm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+
+/* ---- shapes (inputs shared by many solves) ---------------------------------
+ * Uploads the body (and crater model), runs facet scattering, RGI and crater
+ * scattering on the GPU.  Replaces initialise_GPU_constants,
+ * initialise_GPU_*_arrays, load_GPU_globalillumination,
+ * GPU_single_scattering / GPU_multiple_scattering
+ * (thermal_GPU.cu:708-1172) and their host loops.                            */
+int btpm_shape_create(btpm_ctx *ctx, const btpm_params *p, const btpm_body *body,
+                      const btpm_crater *crater, btpm_shape **out);
+int btpm_shape_destroy(btpm_shape *shape);
+/* total facet illumination F after scattering, [nf][ntau] */
+int btpm_shape_facet_illum(btpm_shape *shape, float *out);
+/* total crater illumination after scattering, [nf][ntau][100]
+ * (obtain_GPU_roughness_illumination, thermal_GPU.cu:1122-1135)            */
+int btpm_shape_crater_illum(btpm_shape *shape, float *out);
+
+/* ---- solves -------------------------------------------------------------
+ * One thermal-inertia solve of a shape: the smooth facets to convergence
+ * and, with with_craters, then RGH from the converged facets and the crater
+ * columns to convergence (single_model's per-TI body,
+ * thermal.cpp:3676-3760).  ti_per_facet (optional, [nf]) overrides ti.
+ * Every solve starts from the reference initial state.                      */
+int btpm_solve_create(btpm_ctx *ctx, btpm_shape *shape, float ti, const float *ti_per_facet,
+                      int with_craters, btpm_solve **out);
+int btpm_solve_destroy(btpm_solve *s);
+/* Runs one solve alone, checking convergence on the host after every
+ * rotation (the reference's GPU control flow; the "plain" mode).            */
+int btpm_solve_run(btpm_solve *s);
+int btpm_solve_get_info(btpm_solve *s, btpm_solve_info *info);
+/* final surface temperatures: smooth [nf][ntau] (writeglobaltemperature's
+ * values), crater [nf][ntau][100] (obtain_GPU_roughness_temperatures'
+ * sfc_stemp1), depth profiles [nf][nx+1] / [nf][nx+1][100]                   */
+int btpm_solve_surface_temps(btpm_solve *s, float *out);
+int btpm_solve_crater_temps(btpm_solve *s, float *out);
+int btpm_solve_profile(btpm_solve *s, float *out);
+int btpm_solve_crater_profile(btpm_solve *s, float *out);
+
+/* ---- batches ---------------------------------------------------------------
+ * Runs n solves to convergence together: solves are admitted while HBM
+ * allows, every rotation advances all resident sets in one launch per kernel
+ * kind (captured in a hipGraph), convergence and phase changes are checked
+ * on the host every rotations_per_sync rotations, and finished solves make
+ * room for waiting ones.                                                    */
+int btpm_batch_run(btpm_ctx *ctx, btpm_solve **solves, int n, const btpm_batch_opts *opts,
+                   btpm_batch_stats *stats);
+
+/* ---- server (continuous batching) -------------------------------------------
+ * A request loop on its own host thread: submitted solves are admitted as
+ * memory and slots free up, advanced together with the resident ones, and
+ * completed with their latency.                                              */
+int btpm_server_start(btpm_ctx *ctx, const btpm_batch_opts *opts, btpm_server **out);
+int btpm_server_submit(btpm_server *srv, btpm_solve *s);
+/* blocks until s completes (or timeout_s elapses: BTPM_ERR_STATE)          */
+int btpm_server_wait(btpm_server *srv, btpm_solve *s, double timeout_s);
+int btpm_server_stop(btpm_server *srv, btpm_batch_stats *stats);
+
+/* ---- kernel-level entry points (tests and tools) ----------------------------
+ * Crater self-heating product of the crater conduction kernel, standalone:
+ * q[n][j] = sum_i VF(j->i) e[n][i] for n < ncol, computed with the same
+ * fp16 hi/lo MFMA split; e is [ncol][100], q is [ncol][100].                */
+int btpm_crater_contract(btpm_ctx *ctx, const float *vf, const float *e, int ncol, float *q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,181 @@
+/*
+ * btpm_oracle.h -- FP64 CPU restatement of the Thermal-stage hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the parity checker for the HIP
+ * library under binarytpm_amd/; it is imported by tests/, by the smoke entry
+ * point and by the cpu_baseline leg of bench.py, never by the product path.
+ *
+ * Every function restates the reference algorithm in double precision with
+ * the intended (deterministic) lagged-coupling semantics: every cross-column
+ * term inside a rotation reads the PREVIOUS rotation's surface temperature at
+ * the same time step.  Reference files (abbreviations):
+ *   T175706/thermal.cpp      = 175706/Run-04/thermal.cpp
+ *   T175706/thermal_GPU.cu   = 175706/Run-04/thermal_GPU.cu
+ *   T3905/main.cpp           = 3905/Thermal/NominalPole/main.cpp
+ *
+ * Layouts (row-major, last index fastest):
+ *   smooth facets   F, stemp : [nf][ntau]        temp : [nf][nx+1]
+ *   crater columns  illum, rgh, stemp, shadow, cosang, rgi : [nfac][ntau][ns]
+ *                   temp : [nfac][nx+1][ns]
+ *   crater view factors vf[i*ns + j] = VF(j -> i): the coefficient of
+ *   sub-facet i's emission in sub-facet j's incident flux (the transposed
+ *   storage of T3905/main.cpp:1994-2000).
+ *   sparse facet view factors: CSR rowptr[nf+1], colidx[nnz], vf[nnz];
+ *   row f lists the facets g that illuminate f with VF_fg.
+ *   shape->crater view factors drf[nnz][ns]: per CSR entry (f,g), the view
+ *   factor of g seen from each of f's crater sub-facets.
+ *
+ * Pinning status of each function is stated in DESIGN.md's oracle table and
+ * in tests/test_oracle.py.
+ */
+#ifndef BTPM_ORACLE_H
+#define BTPM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int ntau;            /* time steps per cycle                                  */
+    int nx;              /* depth nodes below the surface (reference NX = 56)     */
+    double rot_period;   /* rotation period of the body, s                        */
+    double eps;          /* IR emissivity                                         */
+    double sigma;        /* Stefan-Boltzmann constant                             */
+    double ir_abs;       /* IR absorptivity (multiplies self-heating)             */
+    double tacc;         /* rotation-to-rotation freeze tolerance, K              */
+    double newton_tol;   /* Newton stop tolerance, K (reference: tacc/10)         */
+    int max_rev;         /* rotation cap (reference 301)                          */
+    int check_after;     /* freeze test runs when rev > check_after (reference 5) */
+    int newton_cap;      /* safety cap on Newton iterations (reference: none)     */
+} orc_params;
+
+typedef struct {
+    int rotations;       /* rotations executed                                    */
+    int converged;       /* 1 if every column froze before the cap                */
+    long newton_iters;   /* total Newton iterations                               */
+    long newton_capped;  /* Newton solves stopped by newton_cap                   */
+    long column_steps;   /* executed (unfrozen) column-steps                      */
+} orc_stats;
+
+/* Defaults of the reference (TACC from the config, Newton at TACC/10). */
+void orc_default_params(orc_params *p, int ntau, double rot_period, double tacc);
+
+/* ---- smooth facets ---------------------------------------------------- */
+
+/* initialisetemperatures, T175706/thermal.cpp:1861-1889:
+ * T = ((1-A) mean_s F / (eps sigma))^(1/4) at every depth and step.          */
+void orc_smooth_init(const orc_params *p, int nf, const double *F,
+                     const double *albedo, double *temp, double *stemp);
+
+/* One rotation of inertiadifference, T175706/thermal.cpp:1895-1979, for the
+ * columns with frozen[f]==0.  stemp holds the previous rotation on entry and
+ * the new one on exit; dmax[f] receives max_s |T_new - T_prev|.               */
+void orc_smooth_rotation(const orc_params *p, int nf, const double *F,
+                         const double *albedo, const double *ti,
+                         const double *numrevs, const int *rowptr,
+                         const int *colidx, const double *vf,
+                         const unsigned char *frozen, double *temp,
+                         double *stemp, double *dmax, orc_stats *st);
+
+/* One rotation of instequilibrium (TI = 0), T175706/thermal.cpp:1985-2033.  */
+void orc_inst_rotation(const orc_params *p, int nf, const double *F,
+                       const double *albedo, const int *rowptr,
+                       const int *colidx, const double *vf, double *stemp,
+                       double *dmax);
+
+/* The rotation loop of single_model, T175706/thermal.cpp:3590-3607:
+ * rotate until every facet is frozen or rev reaches max_rev; the freeze test
+ * (shape_converged, :2422-2454) runs only when rev > check_after.
+ * ti[f] == 0 for every facet selects instantaneous equilibrium.
+ * frozen_rev[f] (may be NULL) receives the rotation index at which f froze
+ * (-1 if never).                                                             */
+int orc_smooth_solve(const orc_params *p, int nf, const double *F,
+                     const double *albedo, const double *ti,
+                     const double *numrevs, const int *rowptr,
+                     const int *colidx, const double *vf, double *temp,
+                     double *stemp, int *frozen_rev, orc_stats *st);
+
+/* singlescattering, T175706/thermal.cpp:1745-1766:
+ * F_f = D_f + sum_g VF_fg A_g D_g, with D = solar cos(i) shadow.             */
+void orc_facet_scatter_single(int nf, int ntau, const double *direct,
+                              const double *albedo, const int *rowptr,
+                              const int *colidx, const double *vf, double *F);
+
+/* multiplescattering, T175706/thermal.cpp:1772-1855: fixed point
+ * p_f = A_f (D_f + sum_g VF_fg p_g), started from p = A F_single, iterated
+ * until every |p_new - p_old| <= acc, then F = p / A (Jacobi sweeps; the
+ * reference sweeps Gauss-Seidel, same fixed point).                          */
+void orc_facet_scatter_multiple(int nf, int ntau, const double *direct,
+                                const double *albedo, const int *rowptr,
+                                const int *colidx, const double *vf,
+                                double acc, double *F);
+
+/* ---- crater (roughness) columns --------------------------------------- */
+
+/* load_GPU_roughness_globalillumination / _globalheating,
+ * T175706/thermal.cpp:2693-2725, 2769-2803:
+ *   out[f][s][j] = sum_{g in row f} drf[k][j] * w_g(s)
+ * with w_g(s) = F_g(s) A_g (RGI) or eps sigma T_g(s)^4 (RGH).               */
+void orc_rgi(int nfac, int ntau, int ns, const int *rowptr, const int *colidx,
+             const double *drf, const double *F, const double *albedo,
+             double *rgi);
+void orc_rgh(int nfac, int ntau, int ns, const int *rowptr, const int *colidx,
+             const double *drf, const double *stemp_smooth, double eps_sigma,
+             double *rgh);
+
+/* roughness_single_scattering, T175706/thermal_GPU.cu:241-276.              */
+void orc_crater_scatter_single(int nfac, int ntau, int ns, const double *vf,
+                               const double *albedo, const double *shadow,
+                               const double *cosang, const double *rgi,
+                               double solar, double *illum);
+
+/* roughness_multiple_scattering, T175706/thermal_GPU.cu:283-341 (Jacobi,
+ * start from A * single-scattered illumination, stop when every
+ * |p_new - p_old| <= acc).  illum: in = single result, out = final.         */
+void orc_crater_scatter_multiple(int nfac, int ntau, int ns, const double *vf,
+                                 const double *albedo, const double *shadow,
+                                 const double *cosang, const double *rgi,
+                                 double solar, double acc, double *illum);
+
+/* roughness_initialise_temperatures, T175706/thermal_GPU.cu:374-423.        */
+void orc_crater_init(const orc_params *p, int nfac, int ns, const double *vf,
+                     const double *albedo, const double *illum,
+                     const double *rgh, double *temp, double *stemp);
+
+/* One rotation of finite_difference + boundary_condition,
+ * T175706/thermal_GPU.cu:430-574, for parent facets with frozen[f]==0;
+ * dmax[f] = max over steps and sub-facets of |T_new - T_prev|.              */
+void orc_crater_rotation(const orc_params *p, int nfac, int ns,
+                         const double *vf, const double *albedo,
+                         const double *ti, const double *numrevs,
+                         const double *illum, const double *rgh,
+                         const unsigned char *frozen, double *temp,
+                         double *stemp, double *dmax, orc_stats *st);
+
+/* One rotation of roughness_inst_equilibrium (TI = 0),
+ * T175706/thermal_GPU.cu:581-641.                                           */
+void orc_crater_inst_rotation(const orc_params *p, int nfac, int ns,
+                              const double *vf, const double *albedo,
+                              const double *illum, const double *rgh,
+                              const unsigned char *frozen, double *stemp,
+                              double *dmax);
+
+/* Crater rotation loop of single_model, T175706/thermal.cpp:3631-3649 with
+ * the per-facet freeze of the converged kernel, T175706/thermal_GPU.cu:648-692.
+ * ti[f] == 0 for every facet selects instantaneous equilibrium.             */
+int orc_crater_solve(const orc_params *p, int nfac, int ns, const double *vf,
+                     const double *albedo, const double *ti,
+                     const double *numrevs, const double *illum,
+                     const double *rgh, double *temp, double *stemp,
+                     int *frozen_rev, orc_stats *st);
+
+/* Newton root of the surface energy balance (Appendix-B form of
+ * T175706/thermal.cpp:1933-1942), exposed for the closed-form tests:
+ *   g(T) = src + m1 (t1 - T) - es T^4 = 0, started at t0.                   */
+double orc_newton_sbc(double src, double m1, double t1, double es, double t0,
+                      double tol, int cap, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,312 @@
+"""ctypes front end of the CPU checker libraries (test infrastructure only).
+
+Imported by tests/, by ``__graft_entry__.smoke()`` and by the cpu_baseline leg
+of bench.py.  The product path (``binarytpm_amd``) never imports this module.
+
+``liboracle.so`` is the FP64 restatement (btpm_oracle.c); ``librefcpu.so`` is
+the float OpenMP restatement of the reference CPU loop shape (refcpu.c).
+Both are built by ``make -C oracle``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC = None
+_RC = None
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_ubyte)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("ntau", C.c_int), ("nx", C.c_int), ("rot_period", C.c_double),
+        ("eps", C.c_double), ("sigma", C.c_double), ("ir_abs", C.c_double),
+        ("tacc", C.c_double), ("newton_tol", C.c_double), ("max_rev", C.c_int),
+        ("check_after", C.c_int), ("newton_cap", C.c_int),
+    ]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [
+        ("rotations", C.c_int), ("converged", C.c_int), ("newton_iters", C.c_long),
+        ("newton_capped", C.c_long), ("column_steps", C.c_long),
+    ]
+
+
+def _load(name):
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run `make -C {_HERE}`")
+    return C.CDLL(path)
+
+
+def lib():
+    global _ORC
+    if _ORC is None:
+        _ORC = _load("liboracle.so")
+    return _ORC
+
+
+def rclib():
+    global _RC
+    if _RC is None:
+        _RC = _load("librefcpu.so")
+        _RC.rc_max_threads.restype = C.c_int
+    return _RC
+
+
+def _d(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def params(ntau, rot_period, tacc=0.01, eps=0.95, sigma=5.67e-8, ir_abs=1.0,
+           nx=56, max_rev=301, check_after=5, newton_tol=None, newton_cap=200):
+    p = OrcParams()
+    p.ntau, p.nx, p.rot_period = ntau, nx, rot_period
+    p.eps, p.sigma, p.ir_abs = eps, sigma, ir_abs
+    p.tacc = tacc
+    p.newton_tol = tacc / 10.0 if newton_tol is None else newton_tol
+    p.max_rev, p.check_after, p.newton_cap = max_rev, check_after, newton_cap
+    return p
+
+
+@dataclass
+class SolveResult:
+    stemp: np.ndarray
+    temp: np.ndarray
+    rotations: int
+    converged: bool
+    frozen_rev: np.ndarray
+    newton_iters: int
+    newton_capped: int
+    column_steps: int
+
+
+def _csr(csr):
+    if csr is None:
+        return None, None, None
+    rowptr, colidx, vals = csr
+    return _i(rowptr), _i(colidx), _d(vals)
+
+
+def smooth_init(p, F, albedo):
+    F = _d(F)
+    nf = F.shape[0]
+    temp = np.zeros((nf, p.nx + 1))
+    stemp = np.zeros((nf, p.ntau))
+    lib().orc_smooth_init(C.byref(p), nf, _ptr(F, _dp), _ptr(_d(albedo), _dp),
+                          _ptr(temp, _dp), _ptr(stemp, _dp))
+    return temp, stemp
+
+
+def smooth_solve(p, F, albedo, ti, numrevs=None, csr=None, temp=None, stemp=None):
+    """Full smooth solve; cold start from smooth_init unless temp/stemp given."""
+    F = _d(F)
+    nf = F.shape[0]
+    albedo = _d(np.broadcast_to(albedo, (nf,)))
+    ti = _d(np.broadcast_to(ti, (nf,)))
+    numrevs = _d(np.ones(nf) if numrevs is None else np.broadcast_to(numrevs, (nf,)))
+    if temp is None:
+        temp, stemp = smooth_init(p, F, albedo)
+    temp, stemp = _d(temp).copy(), _d(stemp).copy()
+    rp, ci, vv = _csr(csr)
+    frozen_rev = np.zeros(nf, dtype=np.int32)
+    st = OrcStats()
+    lib().orc_smooth_solve(C.byref(p), nf, _ptr(F, _dp), _ptr(albedo, _dp), _ptr(ti, _dp),
+                           _ptr(numrevs, _dp), _ptr(rp, _ip), _ptr(ci, _ip), _ptr(vv, _dp),
+                           _ptr(temp, _dp), _ptr(stemp, _dp), _ptr(frozen_rev, _ip),
+                           C.byref(st))
+    return SolveResult(stemp, temp, st.rotations, bool(st.converged), frozen_rev,
+                       st.newton_iters, st.newton_capped, st.column_steps)
+
+
+def smooth_rotation(p, F, albedo, ti, numrevs, temp, stemp, csr=None, frozen=None):
+    F = _d(F)
+    nf = F.shape[0]
+    temp, stemp = _d(temp).copy(), _d(stemp).copy()
+    dmax = np.zeros(nf)
+    rp, ci, vv = _csr(csr)
+    fz = None if frozen is None else np.ascontiguousarray(frozen, dtype=np.uint8)
+    lib().orc_smooth_rotation(C.byref(p), nf, _ptr(F, _dp), _ptr(_d(albedo), _dp),
+                              _ptr(_d(ti), _dp), _ptr(_d(numrevs), _dp), _ptr(rp, _ip),
+                              _ptr(ci, _ip), _ptr(vv, _dp), _ptr(fz, _up), _ptr(temp, _dp),
+                              _ptr(stemp, _dp), _ptr(dmax, _dp), None)
+    return temp, stemp, dmax
+
+
+def newton_sbc(src, m1, t1, es, t0, tol, cap=200):
+    it = C.c_int(0)
+    f = lib().orc_newton_sbc
+    f.restype = C.c_double
+    f.argtypes = [C.c_double] * 6 + [C.c_int, C.POINTER(C.c_int)]
+    x = f(src, m1, t1, es, t0, tol, cap, C.byref(it))
+    return x, it.value
+
+
+def facet_scatter(direct, albedo, csr, multiple=True, acc=1e-4):
+    direct = _d(direct)
+    nf, nt = direct.shape
+    albedo = _d(np.broadcast_to(albedo, (nf,)))
+    rp, ci, vv = _csr(csr)
+    F = np.zeros_like(direct)
+    lib().orc_facet_scatter_single(nf, nt, _ptr(direct, _dp), _ptr(albedo, _dp), _ptr(rp, _ip),
+                                   _ptr(ci, _ip), _ptr(vv, _dp), _ptr(F, _dp))
+    if multiple:
+        fn = lib().orc_facet_scatter_multiple
+        fn.argtypes = [C.c_int, C.c_int, _dp, _dp, _ip, _ip, _dp, C.c_double, _dp]
+        fn(nf, nt, _ptr(direct, _dp), _ptr(albedo, _dp), _ptr(rp, _ip), _ptr(ci, _ip),
+           _ptr(vv, _dp), acc, _ptr(F, _dp))
+    return F
+
+
+def rgi(csr, drf, F, albedo, nfac):
+    rp, ci, _ = _csr(csr)
+    drf, F = _d(drf), _d(F)
+    ns = drf.shape[1]
+    nt = F.shape[1]
+    out = np.zeros((nfac, nt, ns))
+    lib().orc_rgi(nfac, nt, ns, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(drf, _dp), _ptr(F, _dp),
+                  _ptr(_d(np.broadcast_to(albedo, (F.shape[0],))), _dp), _ptr(out, _dp))
+    return out
+
+
+def rgh(csr, drf, stemp_smooth, eps_sigma, nfac):
+    rp, ci, _ = _csr(csr)
+    drf, st = _d(drf), _d(stemp_smooth)
+    ns = drf.shape[1]
+    nt = st.shape[1]
+    out = np.zeros((nfac, nt, ns))
+    fn = lib().orc_rgh
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_double, _dp]
+    fn(nfac, nt, ns, _ptr(rp, _ip), _ptr(ci, _ip), _ptr(drf, _dp), _ptr(st, _dp), eps_sigma,
+       _ptr(out, _dp))
+    return out
+
+
+def crater_scatter(vf, albedo, shadow, cosang, rgi_, solar, multiple=True, acc=1e-4):
+    vf, shadow, cosang, rgi_ = _d(vf), _d(shadow), _d(cosang), _d(rgi_)
+    nfac, nt, ns = shadow.shape
+    albedo = _d(np.broadcast_to(albedo, (nfac,)))
+    out = np.zeros((nfac, nt, ns))
+    fn = lib().orc_crater_scatter_single
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp]
+    fn(nfac, nt, ns, _ptr(vf, _dp), _ptr(albedo, _dp), _ptr(shadow, _dp), _ptr(cosang, _dp),
+       _ptr(rgi_, _dp), solar, _ptr(out, _dp))
+    if multiple:
+        fn = lib().orc_crater_scatter_multiple
+        fn.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                       C.c_double, _dp]
+        fn(nfac, nt, ns, _ptr(vf, _dp), _ptr(albedo, _dp), _ptr(shadow, _dp),
+           _ptr(cosang, _dp), _ptr(rgi_, _dp), solar, acc, _ptr(out, _dp))
+    return out
+
+
+def crater_init(p, vf, albedo, illum, rgh_):
+    vf, illum, rgh_ = _d(vf), _d(illum), _d(rgh_)
+    nfac, nt, ns = illum.shape
+    albedo = _d(np.broadcast_to(albedo, (nfac,)))
+    temp = np.zeros((nfac, p.nx + 1, ns))
+    stemp = np.zeros((nfac, nt, ns))
+    lib().orc_crater_init(C.byref(p), nfac, ns, _ptr(vf, _dp), _ptr(albedo, _dp),
+                          _ptr(illum, _dp), _ptr(rgh_, _dp), _ptr(temp, _dp), _ptr(stemp, _dp))
+    return temp, stemp
+
+
+def crater_solve(p, vf, albedo, ti, illum, rgh_, numrevs=None, temp=None, stemp=None):
+    vf, illum, rgh_ = _d(vf), _d(illum), _d(rgh_)
+    nfac, nt, ns = illum.shape
+    albedo = _d(np.broadcast_to(albedo, (nfac,)))
+    ti = _d(np.broadcast_to(ti, (nfac,)))
+    numrevs = _d(np.ones(nfac) if numrevs is None else np.broadcast_to(numrevs, (nfac,)))
+    if temp is None:
+        temp, stemp = crater_init(p, vf, albedo, illum, rgh_)
+    temp, stemp = _d(temp).copy(), _d(stemp).copy()
+    frozen_rev = np.zeros(nfac, dtype=np.int32)
+    st = OrcStats()
+    lib().orc_crater_solve(C.byref(p), nfac, ns, _ptr(vf, _dp), _ptr(albedo, _dp),
+                           _ptr(ti, _dp), _ptr(numrevs, _dp), _ptr(illum, _dp),
+                           _ptr(rgh_, _dp), _ptr(temp, _dp), _ptr(stemp, _dp),
+                           _ptr(frozen_rev, _ip), C.byref(st))
+    return SolveResult(stemp, temp, st.rotations, bool(st.converged), frozen_rev,
+                       st.newton_iters, st.newton_capped, st.column_steps)
+
+
+# ---- refcpu (float OpenMP restatement, the timed CPU baseline) -------------
+
+def refcpu_threads():
+    return rclib().rc_max_threads()
+
+
+def refcpu_smooth_solve(F, albedo, ti, ntau, rot_period, tacc=0.01, numrevs=None, csr=None,
+                        eps=0.95, sigma=5.67e-8, ir_abs=1.0, nx=56, max_rev=301,
+                        check_after=5, nthreads=0):
+    F = _f(F)
+    nf = F.shape[0]
+    albedo = _f(np.broadcast_to(albedo, (nf,)))
+    ti = _f(np.broadcast_to(ti, (nf,)))
+    numrevs = _f(np.ones(nf) if numrevs is None else np.broadcast_to(numrevs, (nf,)))
+    if csr is not None:
+        rp, ci, vv = _i(csr[0]), _i(csr[1]), _f(csr[2])
+    else:
+        rp = ci = vv = None
+    out = np.zeros((nf, ntau), dtype=np.float32)
+    steps = C.c_long(0)
+    fn = rclib().rc_smooth_solve
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                   C.c_float, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _ip, _ip, _fp, _fp,
+                   C.c_int, C.POINTER(C.c_long)]
+    fn.restype = C.c_int
+    rev = fn(nf, ntau, nx, rot_period, eps, sigma, ir_abs, tacc, max_rev, check_after,
+             _ptr(F, _fp), _ptr(albedo, _fp), _ptr(ti, _fp), _ptr(numrevs, _fp),
+             _ptr(rp, _ip), _ptr(ci, _ip), _ptr(vv, _fp), _ptr(out, _fp), nthreads,
+             C.byref(steps))
+    return out, rev, steps.value
+
+
+def refcpu_crater_solve(vf, albedo, ti, illum, rgh_, rot_period, tacc=0.01, numrevs=None,
+                        eps=0.95, sigma=5.67e-8, ir_abs=1.0, nx=56, max_rev=301,
+                        check_after=5, nthreads=0):
+    illum, rgh_, vf = _f(illum), _f(rgh_), _f(vf)
+    nfac, nt, ns = illum.shape
+    albedo = _f(np.broadcast_to(albedo, (nfac,)))
+    ti = _f(np.broadcast_to(ti, (nfac,)))
+    numrevs = _f(np.ones(nfac) if numrevs is None else np.broadcast_to(numrevs, (nfac,)))
+    out = np.zeros((nfac, nt, ns), dtype=np.float32)
+    steps = C.c_long(0)
+    fn = rclib().rc_crater_solve
+    fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                   C.c_float, C.c_float, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp, _fp, _fp,
+                   C.c_int, C.POINTER(C.c_long)]
+    fn.restype = C.c_int
+    rev = fn(nfac, ns, nt, nx, rot_period, eps, sigma, ir_abs, tacc, max_rev, check_after,
+             _ptr(vf, _fp), _ptr(albedo, _fp), _ptr(ti, _fp), _ptr(numrevs, _fp),
+             _ptr(illum, _fp), _ptr(rgh_, _fp), _ptr(out, _fp), nthreads, C.byref(steps))
+    return out, rev, steps.value

@@ -1,0 +1,52 @@
+"""Writes the small data fixtures the CPU tests use, from data files that ship
+in the reference tree (data only: meshes, view-factor lists, the mesh
+statistics file).  Run in a checkout that has the reference at REF:
+
+    python tests/golden/make_fixtures.py [REF]
+
+Outputs (committed): tests/golden/new_crater2.npz,
+tests/golden/doppler_binary.npz, tests/golden/doppler_binary_properties.json
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from binarytpm_amd import formats  # noqa: E402
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+D = os.path.join(REF, "3905", "Thermal", "NominalPole")
+
+
+def main():
+    V, F = formats.read_shape(os.path.join(D, "new_crater2.txt"), 61, 100)
+    csr = formats.read_selfheating_list(os.path.join(D, "new_crater2_selfheating_list.txt"), 100)
+    np.savez_compressed(os.path.join(HERE, "new_crater2.npz"), V=V, F=F, rowptr=csr[0],
+                        colidx=csr[1], vf=csr[2])
+    V, F = formats.read_shape(os.path.join(D, "doppler_binary.txt"), 3204, 6400)
+    np.savez_compressed(os.path.join(HERE, "doppler_binary.npz"), V=V, F=F.astype(np.int32))
+    props = {}
+    with open(os.path.join(D, "doppler_binary_properties.txt")) as fh:
+        for line in fh:
+            # "Key:\tv [Key2:\tv2]" -- split every "name:" token pair
+            parts = line.replace(":", ":\t").split("\t")
+            key = None
+            for tok in parts:
+                tok = tok.strip()
+                if not tok:
+                    continue
+                if tok.endswith(":"):
+                    key = tok[:-1]
+                    props[key] = []
+                else:
+                    props[key].append(float(tok))
+    with open(os.path.join(HERE, "doppler_binary_properties.json"), "w") as fh:
+        json.dump(props, fh, indent=1)
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+"""GPU parity: the HIP library (through the C ABI) against the FP64 oracle on
+the same seeded inputs.  Bar: max |dT| <= 0.05 K (BASELINE tolerance), RMS
+reported; integer/flag outputs (rotation counts) within one rotation."""
+import numpy as np
+import pytest
+
+from binarytpm_amd import inputs
+from binarytpm_amd import thermal as th
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL_K = 0.05
+ES = 0.95 * 5.67e-8
+
+
+def _report(name, a, b):
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    print(f"[parity] {name}: max|dT| = {d.max():.5f} K, rms = {np.sqrt((d ** 2).mean()):.5f} K",
+          flush=True)
+    return d.max()
+
+
+def test_mfma_contraction_layout(ctx):
+    """The crater product of the conduction kernel (fp16 hi/lo MFMA split)
+    against an FP64 GEMM, with the cap's and a random non-symmetric VF."""
+    rng = np.random.default_rng(11)
+    cg = inputs.crater_cap(45.0)
+    for vf in (cg.vf, rng.uniform(0, 0.01, 10000).astype(np.float32)):
+        e = rng.uniform(0.3, 1500.0, (333, 100)).astype(np.float32)
+        q = th.crater_contract(ctx, vf, e)
+        V = vf.astype(np.float64).reshape(100, 100)
+        ref = e.astype(np.float64) @ V            # q[n][j] = sum_i e[n][i] V[i][j]
+        rel = np.abs(q - ref).max() / np.abs(ref).max()
+        print(f"[parity] mfma contraction rel err {rel:.2e}", flush=True)
+        assert rel < 2e-6
+
+
+@pytest.mark.parametrize("ti", [10.0, 200.0, 2500.0])
+def test_smooth_parity(ctx, ti):
+    case = inputs.config0(subdiv=2)                  # 320 facets, 650 steps
+    p = th.params(650, case.rot_period, tacc=0.01)
+    shape = th.Shape(ctx, p, case.illum, 0.1)
+    s = th.Solve(ctx, shape, ti).run()
+    gpu = s.surface_temps()
+    info = s.info()
+    ref = O.smooth_solve(O.params(650, case.rot_period, tacc=0.01), case.illum, 0.1, ti)
+    assert _report(f"smooth TI={ti}", gpu, ref.stemp) <= TOL_K
+    assert info.converged and abs(info.rotations - ref.rotations) <= 1
+    prof = s.profile()
+    assert np.abs(prof - ref.temp).max() <= TOL_K
+
+
+def test_smooth_inst_parity(ctx):
+    case = inputs.config0(subdiv=1)
+    p = th.params(650, case.rot_period, tacc=0.01)
+    shape = th.Shape(ctx, p, case.illum, 0.1)
+    s = th.Solve(ctx, shape, 0.0).run()
+    ref = O.smooth_solve(O.params(650, case.rot_period, tacc=0.01), case.illum, 0.1, 0.0)
+    assert _report("smooth TI=0", s.surface_temps(), ref.stemp) <= TOL_K
+
+
+def test_self_heating_super_cycle_parity(ctx):
+    """Binary-like body: 2:1 super-cycle (numrevs 2 and 1 on ntau = 1300),
+    sparse facet view factors, facet multiple scattering."""
+    nt = 1300
+    Vp, Fp = inputs.ellipsoid(1, (0.6, 0.5, 0.45))
+    Vs, Fs = inputs.ellipsoid(0, (0.3, 0.3, 0.3))
+    bi = inputs.binary_illumination(Vp, Fp, (0.6, 0.5, 0.45), Vs, Fs, 0.3, 3.0, nt, 2, 1,
+                                    inputs.solar_at(1.1))
+    D = np.concatenate([bi.D_primary, bi.D_secondary])
+    nf = D.shape[0]
+    numrevs = np.concatenate([np.full(len(Fp), 2.0), np.full(len(Fs), 1.0)]).astype(np.float32)
+    csr, _ = inputs.sparse_viewfactors(nf, 0.1, 0.2, seed=4)
+    P = inputs.P_PRIMARY * 2
+    p = th.params(nt, P, tacc=0.01)
+    shape = th.Shape(ctx, p, D, 0.1, numrevs=numrevs, csr=csr, scatter=2, scatter_acc=1e-4)
+    F_ref = O.facet_scatter(D, 0.1, csr, multiple=True, acc=1e-4)
+    assert np.abs(shape.facet_illum() - F_ref).max() < 0.05
+    s = th.Solve(ctx, shape, 150.0).run()
+    ref = O.smooth_solve(O.params(nt, P, tacc=0.01), F_ref, 0.1, 150.0, numrevs=numrevs, csr=csr)
+    assert _report("binary self-heating", s.surface_temps(), ref.stemp) <= TOL_K
+
+
+@pytest.fixture(scope="module")
+def rough80():
+    return inputs.rough_primary(subdiv=1)          # 80 facets x 100 sub-facets
+
+
+def _oracle_rough(case, crater, ti, tacc=0.01):
+    p = O.params(650, case.rot_period, tacc=tacc)
+    nf = case.illum.shape[0]
+    F = O.facet_scatter(case.illum, 0.1, case.csr, multiple=True, acc=1e-4)
+    sm = O.smooth_solve(p, F, 0.1, ti, csr=case.csr)
+    rgi = O.rgi(case.csr, crater["drf"], F, np.full(nf, 0.1), nf)
+    il = O.crater_scatter(crater["vf"], 0.1, crater["shadow"], crater["cosang"], rgi,
+                          crater["solar"], multiple=True, acc=1e-4)
+    rgh = O.rgh(case.csr, crater["drf"], sm.stemp, ES, nf)
+    cr = O.crater_solve(p, crater["vf"], 0.1, ti, il, rgh)
+    return F, sm, il, cr
+
+
+@pytest.mark.parametrize("ti", [50.0, 1000.0])
+def test_crater_parity(ctx, rough80, ti):
+    case, crater = rough80
+    p = th.params(650, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
+                        solar=crater["solar"], multiple=True, scatter_acc=1e-4,
+                        drf=crater["drf"])
+    shape = th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2, crater=cm)
+    s = th.Solve(ctx, shape, ti, with_craters=True).run()
+    F, sm, il, cr = _oracle_rough(case, crater, ti)
+    assert np.abs(shape.crater_illum() - il).max() < 0.05
+    assert _report(f"rough smooth TI={ti}", s.surface_temps(), sm.stemp) <= TOL_K
+    info = s.info()
+    assert info.converged and info.crater_converged
+    assert _report(f"crater TI={ti}", s.crater_temps(), cr.stemp) <= TOL_K
+    assert abs(info.crater_rotations - cr.rotations) <= 1
+
+
+def test_batch_equals_alone(ctx, rough80):
+    """Batched (graph-captured, several solves resident) results are bitwise
+    the results of each solve run alone."""
+    case, crater = rough80
+    p = th.params(650, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
+                        solar=crater["solar"], drf=crater["drf"])
+    shape = th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2, crater=cm)
+    plan = [(30.0, True), (300.0, False), (2000.0, True), (80.0, False), (600.0, True)]
+    alone = []
+    for ti, cr in plan:
+        s = th.Solve(ctx, shape, ti, with_craters=cr).run()
+        alone.append((s.surface_temps(), s.crater_temps() if cr else None))
+    batch = [th.Solve(ctx, shape, ti, with_craters=cr) for ti, cr in plan]
+    st = th.batch_run(ctx, batch, th.batch_opts(rotations_per_sync=3, use_graph=True))
+    assert st.solves == len(plan)
+    for (a_s, a_c), b in zip(alone, batch):
+        np.testing.assert_array_equal(a_s, b.surface_temps())
+        if a_c is not None:
+            np.testing.assert_array_equal(a_c, b.crater_temps())
+
+
+def test_server_results_match_alone(ctx):
+    """Continuous batching: solves submitted while others run complete with
+    the same temperatures as alone."""
+    import time
+    case = inputs.config0(subdiv=2)
+    p = th.params(650, case.rot_period, tacc=0.01)
+    shape = th.Shape(ctx, p, case.illum, 0.1)
+    tis = [20.0, 700.0, 150.0, 2400.0]
+    alone = [th.Solve(ctx, shape, t).run().surface_temps() for t in tis]
+    srv = th.Server(ctx, th.batch_opts(rotations_per_sync=2))
+    sol = []
+    for t in tis:
+        s = th.Solve(ctx, shape, t)
+        srv.submit(s)
+        sol.append(s)
+        time.sleep(0.005)
+    for s in sol:
+        srv.wait(s, 120.0)
+    stats = srv.stop()
+    assert stats.solves == len(tis)
+    for a, s in zip(alone, sol):
+        np.testing.assert_array_equal(a, s.surface_temps())
+        assert s.info().latency_s > 0
+
+
+def test_full_size_config0_parity(ctx):
+    """Full config-0 size (5120 facets, 650 steps, TI 200) against the
+    oracle, plus the periodic energy balance of the GPU result."""
+    case = inputs.config0()
+    p = th.params(650, case.rot_period, tacc=0.01)
+    shape = th.Shape(ctx, p, case.illum, 0.1)
+    s = th.Solve(ctx, shape, 200.0).run()
+    gpu = s.surface_temps()
+    ref = O.smooth_solve(O.params(650, case.rot_period, tacc=0.01), case.illum, 0.1, 200.0)
+    assert _report("config0 full size", gpu, ref.stemp) <= TOL_K
+    absorbed = 0.9 * case.illum.astype(np.float64).sum(1)
+    emitted = (ES * gpu.astype(np.float64) ** 4).sum(1)
+    assert (np.abs(absorbed - emitted) / absorbed).max() < 1e-2
