@@ -37,7 +37,7 @@ struct btpm_set_dev {
   float tacc;         // freeze tolerance (K)
   float newton_tol;   // Newton stop tolerance (K)
   int newton_cap;     // Newton iteration cap
-  int pad_;
+  float vf_unscale;   // crater: 2^-s undoing the power-of-two scale of vfA
   // arrays (device pointers)
   const float* src;   // [nb][ncols][BLK] absorbed flux (1-A)F (+RGH)
   float* T;           // [nb][ncols][BLK] surface T, last rotation (in place)

@@ -254,7 +254,9 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
 // with M = 100 (padded 112), K = 100 (padded 112), N = 80, using
 // v_mfma_f32_16x16x16_f16 and the three-product split
 //     VF E ~ VFhi Ehi + VFhi Elo + VFlo Ehi   (FP32 accumulation),
-// where x_hi = fp16(x) and x_lo = fp16(x - x_hi).
+// where x_hi = fp16(x) and x_lo = fp16(x - x_hi).  VF is stored pre-scaled by
+// 2^s (max entry in [0.5, 1)) so its low halves stay normal fp16 numbers;
+// the FP32 accumulator is multiplied by 2^-s, which is exact.
 // ---------------------------------------------------------------------------
 namespace {
 constexpr int kMP = 112;           // padded M (receiving sub-facet j)
@@ -273,7 +275,7 @@ constexpr int kKT = kKP / 16;      // 7 K chunks
 // receives D[m = 4(l/16) + i][n = l%16], i = 0..3.
 __device__ __forceinline__ void crater_product(const _Float16 (*lds_a)[kMP * kLdsK],
                                                const _Float16 (*lds_b)[kNP * kLdsK],
-                                               float* lds_q, int wave, int lane) {
+                                               float* lds_q, int wave, int lane, float unscale) {
   for (int pt = wave; pt < kMT * kNT; pt += BTPM_CRATER_THREADS / 64) {
     const int mt = pt / kNT, nt = pt - (pt / kNT) * kNT;
     float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -292,7 +294,8 @@ __device__ __forceinline__ void crater_product(const _Float16 (*lds_a)[kMP * kLd
       acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, bhi, acc, 0, 0, 0);
     }
     float* q = &lds_q[bn * kLdsQ + mt * 16 + kq];
-    *reinterpret_cast<float4*>(q) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    *reinterpret_cast<float4*>(q) =
+        make_float4(acc[0] * unscale, acc[1] * unscale, acc[2] * unscale, acc[3] * unscale);
   }
 }
 
@@ -405,7 +408,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
       const size_t on = (size_t)(b + 1) * bstride + coff;
       if (more) load8(Tin + on, nprev);
       __syncthreads();
-      crater_product(lds_a, lds_b, lds_q, wave, lane);
+      crater_product(lds_a, lds_b, lds_q, wave, lane, S.vf_unscale);
       __syncthreads();
 
       const int kend = min(BTPM_BLK, ntau - b * BTPM_BLK);
@@ -464,7 +467,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
 //   q[n][j] = sum_i VF(j->i) e[n][i],  n < ncol, with the kernel's own split.
 extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_contract_kernel(
     const _Float16* __restrict__ vfA, const float* __restrict__ e, float* __restrict__ q,
-    int ncol) {
+    int ncol, float unscale) {
   __shared__ _Float16 lds_a[2][kMP * kLdsK];
   __shared__ _Float16 lds_b[2][kNP * kLdsK];
   __shared__ float lds_q[kNP * kLdsQ];
@@ -480,7 +483,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_contract_
     lds_b[1][x] = (_Float16)(v - (float)hi);
   }
   __syncthreads();
-  crater_product(lds_a, lds_b, lds_q, tid >> 6, tid & 63);
+  crater_product(lds_a, lds_b, lds_q, tid >> 6, tid & 63, unscale);
   __syncthreads();
   for (int x = tid; x < kNP * BTPM_NS; x += BTPM_CRATER_THREADS) {
     const int n = x / BTPM_NS, jj = x - n * BTPM_NS;

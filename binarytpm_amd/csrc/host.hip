@@ -28,7 +28,7 @@ extern "C" __global__ void btpm_conduct_smooth(const btpm_tile*, const btpm_set_
                                                const btpm_status_dev*);
 extern "C" __global__ void btpm_conduct_crater(const btpm_tile*, const btpm_set_dev*,
                                                const btpm_status_dev*);
-extern "C" __global__ void btpm_contract_kernel(const _Float16*, const float*, float*, int);
+extern "C" __global__ void btpm_contract_kernel(const _Float16*, const float*, float*, int, float);
 extern "C" __global__ void btpm_pack_blocked(const float*, float*, const float*, int, int, int);
 extern "C" __global__ void btpm_unpack_blocked(const float*, float*, int, int, int);
 extern "C" __global__ void btpm_pack_crater(const float*, float*, int, int, int);
@@ -93,6 +93,30 @@ constexpr int kMaxSets = 1024;
 const double kPi = 3.14159265358979323846;
 
 inline int nblocks(int ntau) { return (ntau + BTPM_BLK - 1) / BTPM_BLK; }
+
+// VF^T as fp16 hi/lo tiles [2][112][112] (A[m=j][k=i] = vf[i*100+j]), zero
+// padded, scaled by 2^s so the largest entry lies in [0.5, 1): the low
+// halves then stay normal fp16 numbers.  Returns 2^-s.
+float vf_tiles(const float* vf, std::vector<_Float16>& a) {
+  float vmax = 0.0f;
+  for (int e = 0; e < BTPM_NS * BTPM_NS; ++e) vmax = std::max(vmax, std::fabs(vf[e]));
+  int s = 0;
+  if (vmax > 0.0f) {
+    int ex = 0;
+    std::frexp(vmax, &ex);  // vmax = m 2^ex, m in [0.5, 1)
+    s = std::max(-14, std::min(24, -ex));
+  }
+  const float sc = std::ldexp(1.0f, s);
+  a.assign(2 * 112 * 112, (_Float16)0.0f);
+  for (int jj = 0; jj < BTPM_NS; ++jj)
+    for (int ii = 0; ii < BTPM_NS; ++ii) {
+      const float v = vf[ii * BTPM_NS + jj] * sc;
+      const _Float16 hi = (_Float16)v;
+      a[jj * 112 + ii] = hi;
+      a[112 * 112 + jj * 112 + ii] = (_Float16)(v - (float)hi);
+    }
+  return std::ldexp(1.0f, -s);
+}
 inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 // device buffer owned by RAII
@@ -156,6 +180,7 @@ struct btpm_shape {
   DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
   DBuf cillum;     // [nb][nf*100][BLK] crater illumination (after scattering)
   DBuf drf;        // [nnz][100]
+  float vf_unscale = 1.0f;
   std::vector<float> h_albedo, h_numrevs;
 };
 
@@ -298,6 +323,7 @@ void build_crater(btpm_solve* s, hipStream_t st) {
   d.qext = nullptr;
   d.emis = nullptr;
   d.vfA = reinterpret_cast<const float*>(sh->cvfA.p);
+  d.vf_unscale = sh->vf_unscale;
   d.tile_active = S->tile_active.as<int>();
   d.tile_work = S->tile_work.as<int>();
   d.ntiles = S->ntiles;
@@ -732,16 +758,8 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
   if (crater) {
     sh->has_crater = 1;
     upload(sh->cvf32, crater->vf, (size_t)BTPM_NS * BTPM_NS, st);
-    // VF^T as fp16 hi/lo tiles [2][112][112]: A[m=j][k=i] = vf[i*100+j]
-    std::vector<_Float16> a(2 * 112 * 112, (_Float16)0.0f);
-    for (int jj = 0; jj < BTPM_NS; ++jj)
-      for (int ii = 0; ii < BTPM_NS; ++ii) {
-        const float v = crater->vf[ii * BTPM_NS + jj];
-        const _Float16 hi = (_Float16)v;
-        const _Float16 lo = (_Float16)(v - (float)hi);
-        a[jj * 112 + ii] = hi;
-        a[112 * 112 + jj * 112 + ii] = lo;
-      }
+    std::vector<_Float16> a;
+    sh->vf_unscale = vf_tiles(crater->vf, a);
     upload(sh->cvfA, a.data(), a.size(), st);
     const size_t ncser = (size_t)nb * nf * BTPM_NS * BTPM_BLK;
     sh->cillum.alloc(ncser * sizeof(float));
@@ -959,21 +977,15 @@ int btpm_crater_contract(btpm_ctx* ctx, const float* vf, const float* e, int nco
   API_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  std::vector<_Float16> a(2 * 112 * 112, (_Float16)0.0f);
-  for (int jj = 0; jj < BTPM_NS; ++jj)
-    for (int ii = 0; ii < BTPM_NS; ++ii) {
-      const float v = vf[ii * BTPM_NS + jj];
-      const _Float16 hi = (_Float16)v;
-      a[jj * 112 + ii] = hi;
-      a[112 * 112 + jj * 112 + ii] = (_Float16)(v - (float)hi);
-    }
+  std::vector<_Float16> a;
+  const float unscale = vf_tiles(vf, a);
   DBuf dA, dE, dQ;
   upload(dA, a.data(), a.size(), st);
   upload(dE, e, (size_t)ncol * BTPM_NS, st);
   dQ.alloc((size_t)ncol * BTPM_NS * 4);
   const int per = BTPM_CRATERS_WG * BTPM_BLK;  // product columns per workgroup
   hipLaunchKernelGGL(btpm_contract_kernel, dim3((ncol + per - 1) / per), dim3(BTPM_CRATER_THREADS),
-                     0, st, dA.as<_Float16>(), dE.as<float>(), dQ.as<float>(), ncol);
+                     0, st, dA.as<_Float16>(), dE.as<float>(), dQ.as<float>(), ncol, unscale);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(q, dQ.p, dQ.n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

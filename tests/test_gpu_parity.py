@@ -32,7 +32,7 @@ def test_mfma_contraction_layout(ctx):
         ref = e.astype(np.float64) @ V            # q[n][j] = sum_i e[n][i] V[i][j]
         rel = np.abs(q - ref).max() / np.abs(ref).max()
         print(f"[parity] mfma contraction rel err {rel:.2e}", flush=True)
-        assert rel < 2e-6
+        assert rel < 3e-6
 
 
 @pytest.mark.parametrize("ti", [10.0, 200.0, 2500.0])
