@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Benchmark: the 128-solve thermal-inertia x roughness sweep on one MI355X.
+
+Workload (one "step"): 32 thermal inertias log-spaced in 10..2500 SI x
+roughness fractions {0, 0.25, 0.5, 1.0} = 128 solves over the rough primary:
+a 5120-facet icosphere-ellipsoid (1.2 x 1.0 x 0.9 km, P = 4.3 h, 650 steps,
+r = 1.1 AU, A = 0.1) with 3 %-dense facet view factors and a 45-degree
+spherical-cap crater of 100 sub-facets on every facet.  f = 0 solves are the
+smooth facets only (5120 columns); f > 0 solves are smooth facets, then RGH
+and 512,000 crater columns, each to convergence (TACC = 0.01 K, Newton
+TACC/10, freeze after rev > 5, cap 301).  All 128 run through the batched,
+graph-captured scheduler; inputs are resident in HBM before timing.
+
+Prints ONE JSON line.  value = converged solves per hour (whole job).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NX = 56
+BYTES_PER_COLSTEP = 12.0   # read absorbed flux + previous-rotation T, write T
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(subdiv, gamma):
+    from binarytpm_amd import inputs
+    t0 = time.time()
+    case, crater = inputs.rough_primary(subdiv=subdiv, gamma_deg=gamma)
+    log(f"inputs generated in {time.time() - t0:.1f} s: nf={case.illum.shape[0]}, "
+        f"nnz={len(case.csr[1])}")
+    return case, crater
+
+
+def make_shape(ctx, case, crater):
+    from binarytpm_amd import thermal as th
+    p = th.params(case.ntau, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
+                        solar=crater["solar"], multiple=True, scatter_acc=1e-4,
+                        drf=crater["drf"])
+    return th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2, scatter_acc=1e-4, crater=cm)
+
+
+def sweep_plan(n_ti=32):
+    from binarytpm_amd import inputs
+    tis = inputs.sweep_inertias(n_ti)
+    return [(float(t), f) for t in tis for f in (0.0, 0.25, 0.5, 1.0)]
+
+
+def run_sweep(ctx, shape, plan, opts):
+    from binarytpm_amd import thermal as th
+    solves = [th.Solve(ctx, shape, ti, with_craters=f > 0) for ti, f in plan]
+    t0 = time.perf_counter()
+    stats = th.batch_run(ctx, solves, opts)
+    wall = time.perf_counter() - t0
+    infos = [s.info() for s in solves]
+    return solves, stats, wall, infos
+
+
+def cpu_baseline(ctx, shape, case, crater, plan, solves, n_craters=16, check_ti_idx=(6, 25)):
+    """Times the float OpenMP restatement of the reference loop shape on a
+    bounded sample and checks the GPU results of the same solves against the
+    FP64 oracle (smooth facets in full, `n_craters` crater parents)."""
+    from binarytpm_amd import inputs
+    from oracle import oracle as O
+    nf = case.illum.shape[0]
+    es = 0.95 * 5.67e-8
+    threads = len(os.sched_getaffinity(0))
+    tis = inputs.sweep_inertias(32)
+    t0 = time.time()
+    F = O.facet_scatter(case.illum, 0.1, case.csr, multiple=True, acc=1e-4)
+    log(f"oracle facet scattering {time.time() - t0:.1f} s")
+    sel = np.linspace(0, nf - 1, n_craters).astype(int)
+    rp, ci, vv = case.csr
+    # CSR restricted to the sampled crater parents (rows), columns unchanged
+    sub_rp = np.concatenate([[0], np.cumsum(rp[sel + 1] - rp[sel])]).astype(np.int32)
+    rows = np.concatenate([np.arange(rp[f], rp[f + 1]) for f in sel])
+    sub_csr = (sub_rp, ci[rows], vv[rows])
+    sub_drf = crater["drf"][rows]
+    rgi = O.rgi(sub_csr, sub_drf, F, np.full(nf, 0.1), n_craters)
+    il = O.crater_scatter(crater["vf"], 0.1, crater["shadow"][sel], crater["cosang"][sel], rgi,
+                          crater["solar"], multiple=True, acc=1e-4)
+    t_smooth, t_crater, n_cs_smooth, n_cs_crater = [], [], [], []
+    dmax, sq, cnt = 0.0, 0.0, 0
+    for k in check_ti_idx:
+        ti = float(tis[k])
+        # CPU baseline timing (refcpu: float, OpenMP, reference loop shape)
+        t0 = time.time()
+        st_rc, rev_rc, cs = O.refcpu_smooth_solve(F, 0.1, ti, case.ntau, case.rot_period,
+                                                  tacc=0.01, csr=case.csr, nthreads=threads)
+        t_smooth.append(time.time() - t0)
+        n_cs_smooth.append(cs)
+        rgh_rc = O.rgh(sub_csr, sub_drf, st_rc, es, n_craters)
+        t0 = time.time()
+        _, rev_c, cs = O.refcpu_crater_solve(crater["vf"], 0.1, ti, il, rgh_rc, case.rot_period,
+                                             tacc=0.01, nthreads=threads)
+        t_crater.append(time.time() - t0)
+        n_cs_crater.append(cs)
+        log(f"refcpu TI={ti:.1f}: smooth {t_smooth[-1]:.1f} s ({rev_rc} rot), "
+            f"{n_craters}-crater sample {t_crater[-1]:.1f} s ({rev_c} rot)")
+        # FP64 oracle check of the GPU results for this TI (f = 0.5 solve)
+        p = O.params(case.ntau, case.rot_period, tacc=0.01)
+        sm = O.smooth_solve(p, F, 0.1, ti, csr=case.csr)
+        rgh = O.rgh(sub_csr, sub_drf, sm.stemp, es, n_craters)
+        cr = O.crater_solve(p, crater["vf"], 0.1, ti, il, rgh)
+        idx = plan.index((ti, 0.5))
+        g_s = solves[idx].surface_temps().astype(np.float64)
+        g_c = solves[idx].crater_temps()[sel].astype(np.float64)
+        for a, b in ((g_s, sm.stemp), (g_c, cr.stemp)):
+            d = np.abs(a - b)
+            dmax = max(dmax, float(d.max()))
+            sq += float((d ** 2).sum())
+            cnt += d.size
+        log(f"oracle check TI={ti:.1f}: running max|dT| {dmax:.4f} K")
+    # extrapolate: f = 0 solve = smooth; f > 0 solve = smooth + all craters
+    t_s = float(np.mean(t_smooth))
+    t_c_full = float(np.mean(t_crater)) * nf / n_craters
+    n0 = sum(1 for _, f in plan if f == 0)
+    n1 = len(plan) - n0
+    cpu_sweep_s = n0 * t_s + n1 * (t_s + t_c_full)
+    sample_s = sum(t_smooth) + sum(t_crater)
+    colsteps = sum(n_cs_smooth) + sum(n_cs_crater)
+    return {
+        "value": 3600.0 * len(plan) / cpu_sweep_s,
+        "unit": "converged solves/hour",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"refcpu (float OpenMP restatement of the reference loop shape) on TI "
+                   f"{[round(float(tis[k]), 1) for k in check_ti_idx]}: full 5120-facet smooth "
+                   f"solve + {n_craters} of 5120 crater parents, extrapolated to the "
+                   f"128-solve sweep; sample {sample_s:.1f} s of CPU wall time"),
+        "facet_layer_steps_per_s": colsteps * NX / sample_s,
+        "check_max_dT_K": dmax,
+        "check_rms_dT_K": float(np.sqrt(sq / max(cnt, 1))),
+    }
+
+
+def plain_reference(ctx, shape, plan, idx):
+    """The same solves run one at a time with a host convergence check per
+    rotation (no batching, no graphs): the plain single-solve GPU path."""
+    from binarytpm_amd import thermal as th
+    t = []
+    for i in idx:
+        ti, f = plan[i]
+        s = th.Solve(ctx, shape, ti, with_craters=f > 0)
+        t0 = time.perf_counter()
+        s.run()
+        t.append((f > 0, time.perf_counter() - t0))
+        s.close()
+    ts = np.mean([x for c, x in t if not c]) if any(not c for c, _ in t) else 0.0
+    tc = np.mean([x for c, x in t if c]) if any(c for c, _ in t) else 0.0
+    n0 = sum(1 for _, f in plan if f == 0)
+    total = n0 * ts + (len(plan) - n0) * tc
+    return 3600.0 * len(plan) / total, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--subdiv", type=int, default=4)
+    ap.add_argument("--ti", type=int, default=32, help="thermal inertias in the sweep")
+    ap.add_argument("--gamma", type=float, default=45.0)
+    ap.add_argument("--rps", type=int, default=4, help="rotations per host sync")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (solves are "
+              "independent; the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from binarytpm_amd import thermal as th
+    case, crater = build_workload(args.subdiv, args.gamma)
+    plan = sweep_plan(args.ti)
+    with th.Context(0) as ctx:
+        t0 = time.time()
+        shape = make_shape(ctx, case, crater)
+        log(f"shape resident (upload + facet/crater scattering) in {time.time() - t0:.1f} s")
+        opts = th.batch_opts(rotations_per_sync=args.rps, use_graph=True, profile=True)
+        for w in range(args.warmup):
+            solves, st, wall, _ = run_sweep(ctx, shape, plan, opts)
+            log(f"warmup {w}: {wall:.2f} s")
+            for s in solves:
+                s.close()
+        walls, stats, last = [], [], None
+        for k in range(args.steps):
+            if last is not None:
+                for s in last:
+                    s.close()
+            solves, st, wall, infos = run_sweep(ctx, shape, plan, opts)
+            walls.append(wall)
+            stats.append(st)
+            last = solves
+            log(f"step {k}: {wall:.2f} s, {st.solves} solves, {st.converged} converged, "
+                f"{st.syncs} syncs")
+        T = float(np.sum(walls))
+        nsol = len(plan) * args.steps
+        conv = sum(s.converged for s in stats)
+        colsteps = sum(s.column_steps for s in stats)
+        colsteps_all = sum(s.column_steps_all for s in stats)
+        ck_s = sum(s.crater_kernel_s for s in stats)
+        ck_n = sum(s.crater_launches for s in stats)
+        ck_cs = sum(s.crater_column_steps for s in stats)
+        sk_s = sum(s.smooth_kernel_s for s in stats)
+        sk_n = sum(s.smooth_launches for s in stats)
+        sk_cs = sum(s.smooth_column_steps for s in stats)
+        lat = np.array([i.latency_s for i in infos])
+        achieved = BYTES_PER_COLSTEP * ck_cs / ck_s / 1e9 if ck_s > 0 else 0.0
+        line = {
+            "metric": "converged solves/hour (128-solve TI x roughness sweep)",
+            "value": 3600.0 * conv / T,
+            "unit": "converged solves/hour",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": 1000.0 * T / args.steps,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32 state, f64 Newton, fp16x3 MFMA crater product",
+            "data": "synthetic (seeded shapes, illumination, view factors, crater model)",
+            "config": {
+                "workload": "128-solve sweep: 32 TI x f{0,.25,.5,1}, 5120-facet primary, "
+                            "45-deg cap crater x100 sub-facets per facet, 650 steps, TACC 0.01 K",
+                "solves": len(plan), "facets": int(case.illum.shape[0]),
+                "crater_columns_per_rough_solve": int(case.illum.shape[0]) * 100,
+                "ntau": case.ntau, "nx": NX, "parallelism": "1 GPU, batched solves",
+            },
+            "facet_layer_steps_per_s": colsteps * NX / T,
+            "facet_layer_steps_per_s_incl_frozen": colsteps_all * NX / T,
+            "solves_converged": int(conv),
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "roofline": {
+                "kernel": "btpm_conduct_crater",
+                "bound": "hbm",
+                "achieved": achieved,
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None,
+                "avg_launch_ms": 1000.0 * ck_s / max(ck_n, 1),
+                "launches": ck_n,
+                "bytes_per_column_step": BYTES_PER_COLSTEP,
+                "smooth_kernel_gbs": BYTES_PER_COLSTEP * sk_cs / sk_s / 1e9 if sk_s > 0 else 0.0,
+                "smooth_avg_launch_ms": 1000.0 * sk_s / max(sk_n, 1),
+            },
+        }
+        if not args.no_plain:
+            idx = [plan.index(x) for x in plan if x[0] == plan[4 * 12][0]][:2]
+            v, t = plain_reference(ctx, shape, plan, idx)
+            line["plain_solves_per_hour"] = v
+            line["batched_over_plain"] = line["value"] / v
+            log(f"plain single-solve path: {t}")
+        if not args.no_cpu:
+            cb = cpu_baseline(ctx, shape, case, crater, plan, last)
+            line["cpu_baseline"] = cb
+            line["vs_cpu"] = line["value"] / cb["value"]
+            line["max_dT_K"] = cb["check_max_dT_K"]
+            line["rms_dT_K"] = cb["check_rms_dT_K"]
+        for s in last:
+            s.close()
+        shape.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

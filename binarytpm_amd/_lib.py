@@ -64,7 +64,7 @@ class SolveInfo(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [
         ("rotations_per_sync", C.c_int), ("use_graph", C.c_int),
-        ("max_active_craters", C.c_int), ("mem_fraction", C.c_double),
+        ("max_active_craters", C.c_int), ("mem_fraction", C.c_double), ("profile", C.c_int),
     ]
 
 
@@ -73,6 +73,9 @@ class BatchStats(C.Structure):
         ("wall_s", C.c_double), ("solves", C.c_int), ("converged", C.c_int),
         ("column_steps", C.c_longlong), ("column_steps_all", C.c_longlong),
         ("graph_launches", C.c_int), ("syncs", C.c_int),
+        ("smooth_kernel_s", C.c_double), ("crater_kernel_s", C.c_double),
+        ("smooth_launches", C.c_int), ("crater_launches", C.c_int),
+        ("smooth_column_steps", C.c_longlong), ("crater_column_steps", C.c_longlong),
     ]
 
 

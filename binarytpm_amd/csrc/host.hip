@@ -382,6 +382,8 @@ struct Sched {
   DBuf d_tiles_s, d_tiles_c, d_ids;
   int n_tiles_s = 0, n_tiles_c = 0, n_ids = 0;
   btpm_batch_stats stats{};
+  // launch timing (opts.profile): events around each conduction launch
+  std::vector<hipEvent_t> ev_s, ev_c;  // pairs (start, end) per rotation
   size_t budget = 0, in_use = 0;
   int craters_active = 0;
   std::chrono::steady_clock::time_point t0;
@@ -392,7 +394,23 @@ struct Sched {
     budget = (size_t)(fr * (opt.mem_fraction > 0 ? opt.mem_fraction : 0.8));
     t0 = std::chrono::steady_clock::now();
   }
-  ~Sched() { drop_graph(); }
+  ~Sched() {
+    drop_graph();
+    for (hipEvent_t e : ev_s) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_c) (void)hipEventDestroy(e);
+  }
+
+  void make_events(int R) {
+    auto grow = [](std::vector<hipEvent_t>& v, size_t n) {
+      while (v.size() < n) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        v.push_back(e);
+      }
+    };
+    grow(ev_s, 2 * (size_t)R);
+    grow(ev_c, 2 * (size_t)R);
+  }
 
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -466,8 +484,8 @@ struct Sched {
     upload(d_ids, ids.data(), ids.size(), ctx->stream);
   }
 
-  // enqueue one rotation of every resident set
-  void enqueue_rotation(hipStream_t st) {
+  // enqueue one rotation of every resident set (rotation i of the chunk)
+  void enqueue_rotation(hipStream_t st, int i) {
     for (auto& kv : slots) {
       SetHost* S = kv.second.second;
       btpm_solve* s = kv.second.first;
@@ -482,12 +500,19 @@ struct Sched {
                            S->qext.as<float>(), sh->nf, S->d.nb, bpt, ctx->d_status + S->slot);
       }
     }
-    if (n_tiles_s)
+    const bool prof = opt.profile != 0;
+    if (n_tiles_s) {
+      if (prof) HIPCHK(hipEventRecord(ev_s[2 * i], st));
       hipLaunchKernelGGL(btpm_conduct_smooth, dim3(n_tiles_s), dim3(BTPM_SMOOTH_WG), 0, st,
                          d_tiles_s.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
-    if (n_tiles_c)
+      if (prof) HIPCHK(hipEventRecord(ev_s[2 * i + 1], st));
+    }
+    if (n_tiles_c) {
+      if (prof) HIPCHK(hipEventRecord(ev_c[2 * i], st));
       hipLaunchKernelGGL(btpm_conduct_crater, dim3(n_tiles_c), dim3(BTPM_CRATER_THREADS), 0, st,
                          d_tiles_c.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
+      if (prof) HIPCHK(hipEventRecord(ev_c[2 * i + 1], st));
+    }
     if (n_ids)
       hipLaunchKernelGGL(btpm_status_kernel, dim3(n_ids), dim3(256), 0, st, d_ids.as<int>(),
                          ctx->d_sets, ctx->d_status);
@@ -495,12 +520,13 @@ struct Sched {
 
   void run_chunk() {
     const int R = std::max(1, opt.rotations_per_sync);
+    if (opt.profile) make_events(R);
     if (dirty) {
       drop_graph();
       rebuild_lists();
       if (opt.use_graph) {
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream);
+        for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream, i);
         HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
         HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
       }
@@ -510,13 +536,28 @@ struct Sched {
       HIPCHK(hipGraphLaunch(gexec, ctx->stream));
       stats.graph_launches++;
     } else {
-      for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream);
+      for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream, i);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_status, ctx->d_status, sizeof(btpm_status_dev) * kMaxSets,
                           hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     stats.syncs++;
+    if (opt.profile) {
+      for (int i = 0; i < R; ++i) {
+        float ms = 0.0f;
+        if (n_tiles_s) {
+          HIPCHK(hipEventElapsedTime(&ms, ev_s[2 * i], ev_s[2 * i + 1]));
+          stats.smooth_kernel_s += ms * 1e-3;
+          stats.smooth_launches++;
+        }
+        if (n_tiles_c) {
+          HIPCHK(hipEventElapsedTime(&ms, ev_c[2 * i], ev_c[2 * i + 1]));
+          stats.crater_kernel_s += ms * 1e-3;
+          stats.crater_launches++;
+        }
+      }
+    }
   }
 
   void finish_set(btpm_solve* s, SetHost* S) {
@@ -531,6 +572,8 @@ struct Sched {
       s->info.crater_column_steps = st.colsteps;
     }
     stats.column_steps += st.colsteps;
+    if (S->d.kind == BTPM_SET_SMOOTH) stats.smooth_column_steps += st.colsteps;
+    else stats.crater_column_steps += st.colsteps;
     stats.column_steps_all += (long long)st.rev * S->d.ncols * S->d.ntau;
     remove_set(S);
   }
@@ -620,6 +663,7 @@ void btpm_batch_opts_default(btpm_batch_opts* o) {
   o->use_graph = 1;
   o->max_active_craters = 0;
   o->mem_fraction = 0.8;
+  o->profile = 0;
 }
 
 int btpm_ctx_create(int device, btpm_ctx** out) {

@@ -24,6 +24,8 @@
 #ifndef BTPM_H
 #define BTPM_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -117,6 +119,7 @@ typedef struct {
     int use_graph;               /* capture each rotation set in a hipGraph */
     int max_active_craters;      /* crater solves resident at once (0: by memory) */
     double mem_fraction;         /* share of free HBM the batch may hold   */
+    int profile;                 /* time every conduction launch with events */
 } btpm_batch_opts;
 
 typedef struct {
@@ -127,6 +130,14 @@ typedef struct {
     long long column_steps_all;  /* including frozen columns              */
     int graph_launches;
     int syncs;
+    /* with opts.profile: summed launch durations and launch counts of the
+     * two conduction kernels, and the column-steps each integrated        */
+    double smooth_kernel_s;
+    double crater_kernel_s;
+    int smooth_launches;
+    int crater_launches;
+    long long smooth_column_steps;
+    long long crater_column_steps;
 } btpm_batch_stats;
 
 /* ---- library ------------------------------------------------------------ */
