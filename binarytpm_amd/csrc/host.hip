@@ -379,6 +379,8 @@ struct Sched {
   bool dirty = true;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  hipGraphExec_t phase_exec[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraph_t phase_graph[4] = {nullptr, nullptr, nullptr, nullptr};
   DBuf d_tiles_s, d_tiles_c, d_ids;
   int n_tiles_s = 0, n_tiles_c = 0, n_ids = 0;
   btpm_batch_stats stats{};
@@ -417,6 +419,12 @@ struct Sched {
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
+    for (int ph = 0; ph < 4; ++ph) {
+      if (phase_exec[ph]) (void)hipGraphExecDestroy(phase_exec[ph]);
+      if (phase_graph[ph]) (void)hipGraphDestroy(phase_graph[ph]);
+      phase_exec[ph] = nullptr;
+      phase_graph[ph] = nullptr;
+    }
   }
 
   int take_slot() {
@@ -484,38 +492,81 @@ struct Sched {
     upload(d_ids, ids.data(), ids.size(), ctx->stream);
   }
 
-  // enqueue one rotation of every resident set (rotation i of the chunk)
-  void enqueue_rotation(hipStream_t st, int i) {
-    for (auto& kv : slots) {
-      SetHost* S = kv.second.second;
-      btpm_solve* s = kv.second.first;
-      if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext) {
-        btpm_shape* sh = s->shape;
-        const int bpt = 16;
-        dim3 g(grid1(sh->nf, 256), grid1(S->d.nb, bpt));
-        // qext = src + VF * emis: the total absorbed flux of this rotation
-        hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
-                           sh->colidx.as<int>(), sh->vf.as<float>(), S->emis.as<float>(),
-                           (const float*)S->src.as<float>(), (const float*)nullptr,
-                           S->qext.as<float>(), sh->nf, S->d.nb, bpt, ctx->d_status + S->slot);
+  // one rotation of every resident set, in four phases: facet self-heating
+  // SpMMs, smooth conduction, crater conduction, status
+  void enqueue_phase(hipStream_t st, int ph) {
+    if (ph == 0) {
+      for (auto& kv : slots) {
+        SetHost* S = kv.second.second;
+        btpm_solve* s = kv.second.first;
+        if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext) {
+          btpm_shape* sh = s->shape;
+          const int bpt = 16;
+          dim3 g(grid1(sh->nf, 256), grid1(S->d.nb, bpt));
+          // qext = src + VF * emis: the total absorbed flux of this rotation
+          hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
+                             sh->colidx.as<int>(), sh->vf.as<float>(), S->emis.as<float>(),
+                             (const float*)S->src.as<float>(), (const float*)nullptr,
+                             S->qext.as<float>(), sh->nf, S->d.nb, bpt, ctx->d_status + S->slot);
+        }
       }
+    } else if (ph == 1) {
+      if (n_tiles_s)
+        hipLaunchKernelGGL(btpm_conduct_smooth, dim3(n_tiles_s), dim3(BTPM_SMOOTH_WG), 0, st,
+                           d_tiles_s.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
+    } else if (ph == 2) {
+      if (n_tiles_c)
+        hipLaunchKernelGGL(btpm_conduct_crater, dim3(n_tiles_c), dim3(BTPM_CRATER_THREADS), 0,
+                           st, d_tiles_c.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
+    } else {
+      if (n_ids)
+        hipLaunchKernelGGL(btpm_status_kernel, dim3(n_ids), dim3(256), 0, st, d_ids.as<int>(),
+                           ctx->d_sets, ctx->d_status);
     }
-    const bool prof = opt.profile != 0;
-    if (n_tiles_s) {
-      if (prof) HIPCHK(hipEventRecord(ev_s[2 * i], st));
-      hipLaunchKernelGGL(btpm_conduct_smooth, dim3(n_tiles_s), dim3(BTPM_SMOOTH_WG), 0, st,
-                         d_tiles_s.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
-      if (prof) HIPCHK(hipEventRecord(ev_s[2 * i + 1], st));
+  }
+
+  void enqueue_rotation(hipStream_t st) {
+    for (int ph = 0; ph < 4; ++ph) enqueue_phase(st, ph);
+  }
+
+  bool phase_empty(int ph) const {
+    if (ph == 0) {
+      for (auto& kv : slots)
+        if (kv.second.second->d.kind == BTPM_SET_SMOOTH && kv.second.second->d.qext) return false;
+      return true;
     }
-    if (n_tiles_c) {
-      if (prof) HIPCHK(hipEventRecord(ev_c[2 * i], st));
-      hipLaunchKernelGGL(btpm_conduct_crater, dim3(n_tiles_c), dim3(BTPM_CRATER_THREADS), 0, st,
-                         d_tiles_c.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
-      if (prof) HIPCHK(hipEventRecord(ev_c[2 * i + 1], st));
+    if (ph == 1) return n_tiles_s == 0;
+    if (ph == 2) return n_tiles_c == 0;
+    return n_ids == 0;
+  }
+
+  // profiled rotation: each conduction phase bracketed by stream events
+  // (around a launch of the phase's own graph when graphs are in use)
+  void profiled_rotation(int i) {
+    hipStream_t st = ctx->stream;
+    for (int ph = 0; ph < 4; ++ph) {
+      hipEvent_t* ev = ph == 1 ? &ev_s[2 * i] : ph == 2 ? &ev_c[2 * i] : nullptr;
+      const bool empty = phase_empty(ph);
+      if (empty) continue;
+      if (ev) HIPCHK(hipEventRecord(ev[0], st));
+      if (opt.use_graph) {
+        HIPCHK(hipGraphLaunch(phase_exec[ph], st));
+        stats.graph_launches++;
+      } else {
+        enqueue_phase(st, ph);
+      }
+      if (ev) HIPCHK(hipEventRecord(ev[1], st));
     }
-    if (n_ids)
-      hipLaunchKernelGGL(btpm_status_kernel, dim3(n_ids), dim3(256), 0, st, d_ids.as<int>(),
-                         ctx->d_sets, ctx->d_status);
+  }
+
+  void capture(hipGraph_t* g, hipGraphExec_t* x, int R, int phase) {
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    if (phase < 0)
+      for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream);
+    else
+      enqueue_phase(ctx->stream, phase);
+    HIPCHK(hipStreamEndCapture(ctx->stream, g));
+    HIPCHK(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
   }
 
   void run_chunk() {
@@ -525,18 +576,22 @@ struct Sched {
       drop_graph();
       rebuild_lists();
       if (opt.use_graph) {
-        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream, i);
-        HIPCHK(hipStreamEndCapture(ctx->stream, &graph));
-        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        if (opt.profile) {
+          for (int ph = 0; ph < 4; ++ph)
+            if (!phase_empty(ph)) capture(&phase_graph[ph], &phase_exec[ph], 1, ph);
+        }
+        else
+          capture(&graph, &gexec, R, -1);
       }
       dirty = false;
     }
-    if (opt.use_graph) {
+    if (opt.profile) {
+      for (int i = 0; i < R; ++i) profiled_rotation(i);
+    } else if (opt.use_graph) {
       HIPCHK(hipGraphLaunch(gexec, ctx->stream));
       stats.graph_launches++;
     } else {
-      for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream, i);
+      for (int i = 0; i < R; ++i) enqueue_rotation(ctx->stream);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_status, ctx->d_status, sizeof(btpm_status_dev) * kMaxSets,
