@@ -207,6 +207,8 @@ def main():
             last = solves
             log(f"step {k}: {wall:.2f} s, {st.solves} solves, {st.converged} converged, "
                 f"{st.syncs} syncs")
+            log("rotations (smooth/crater) per solve: " + " ".join(
+                f"{i.rotations}/{i.crater_rotations}" for i in infos))
         T = float(np.sum(walls))
         nsol = len(plan) * args.steps
         conv = sum(s.converged for s in stats)
@@ -261,7 +263,8 @@ def main():
             },
         }
         if not args.no_plain:
-            idx = [plan.index(x) for x in plan if x[0] == plan[4 * 12][0]][:2]
+            mid_ti = plan[len(plan) // 2][0]
+            idx = [i for i, x in enumerate(plan) if x[0] == mid_ti][:2]
             v, t = plain_reference(ctx, shape, plan, idx)
             line["plain_solves_per_hour"] = v
             line["batched_over_plain"] = line["value"] / v

@@ -102,11 +102,9 @@ extern "C" __global__ void btpm_init_smooth(btpm_set_dev S) {
   for (int b = 0; b < S.nb; ++b) {
     float* p = S.T + ((size_t)b * S.ncols + c) * BTPM_BLK;
     for (int k = 0; k < BTPM_BLK; ++k) p[k] = t;
-    if (S.emis) {
-      float* q = S.emis + ((size_t)b * S.ncols + c) * BTPM_BLK;
-      for (int k = 0; k < BTPM_BLK; ++k) q[k] = e;
-    }
   }
+  if (S.emis)  // facet-major [c][nb*BLK]
+    for (int k = 0; k < S.nb * BTPM_BLK; ++k) S.emis[(size_t)c * S.nb * BTPM_BLK + k] = e;
   S.frozen[c] = 0;
   S.dmax[c] = 0.0f;
 }
@@ -345,6 +343,49 @@ extern "C" __global__ __launch_bounds__(256) void btpm_spmm_blocked(
         yp[1] = make_float4(r[4], r[5], r[6], r[7]);
       }
     }
+  }
+}
+
+// Facet self-heating flux of the next rotation from the previous rotation's
+// emission, written as the total absorbed flux the conduction kernel reads:
+//   y[b][f][k] = src[b][f][k] + sum_{(f,g)} vf * emis_t[g][8b+k]
+// emis_t is facet-major ([g][npad], npad = nb*BLK) so a wavefront whose lanes
+// are 64 consecutive steps reads one coalesced 256-byte run per view factor;
+// the CSR row is uniform across the wavefront (scalar loads).  blockIdx.x
+// runs over rows fastest, so at any time every XCD works on the same 64-step
+// chunk, whose emission slice (nrows x 256 B) stays in its L2.
+extern "C" __global__ __launch_bounds__(256) void btpm_selfheat_spmm(
+    const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ vals,
+    const float* __restrict__ emis_t, const float* __restrict__ src, float* __restrict__ y,
+    int nrows, int nb, const btpm_status_dev* __restrict__ st) {
+  if (st && st->done) return;
+  const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int npad = nb * BTPM_BLK;
+  const int s = blockIdx.y * 64 + (threadIdx.x & 63);
+  if (f >= nrows) return;
+  const int sc = s < npad ? s : npad - 1;
+  const int k0 = rowptr[f], k1 = rowptr[f + 1];
+  // eight independent gathers in flight per lane
+  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  int kk = k0;
+  for (; kk + 8 <= k1; kk += 8) {
+    int g[8];
+    float v[8], e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      g[q] = colidx[kk + q];
+      v[q] = vals[kk + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) e[q] = emis_t[(size_t)g[q] * npad + sc];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = fmaf(v[q], e[q], acc[q]);
+  }
+  for (; kk < k1; ++kk) acc[0] = fmaf(vals[kk], emis_t[(size_t)colidx[kk] * npad + sc], acc[0]);
+  const float tot = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  if (s < npad) {
+    const size_t o = ((size_t)(s / BTPM_BLK) * nrows + f) * BTPM_BLK + (s % BTPM_BLK);
+    y[o] = src[o] + tot;
   }
 }
 

@@ -47,7 +47,8 @@ struct btpm_set_dev {
   uint8_t* frozen;    // [ngroups]
   float* dmax;        // [ngroups] max |dT| of the last executed rotation
   const float* qext;  // [nb][ncols][BLK] facet self-heating flux (or null)
-  float* emis;        // [nb][ncols][BLK] esia T^4 for the next rotation (or null)
+  float* emis;        // [ncols][nb*BLK] esia T^4 for the next rotation (or null),
+                      // facet-major for the self-heating SpMM's coalesced reads
   const float* vfA;   // crater: fp16 hi/lo view-factor tiles (see crater kernel)
   // status, written by the kernels
   int* tile_active;   // [ntiles] active groups after this rotation's test

@@ -217,7 +217,7 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
             const float t = out[k];
             out[k] = esia * (t * t) * (t * t);
           }
-          store8(Eout + o, out);
+          store8(Eout + (size_t)cc * nb * BTPM_BLK + (size_t)b * BTPM_BLK, out);  // facet-major
         }
       }
     }

@@ -44,6 +44,9 @@ extern "C" __global__ void btpm_crater_scatter(const float*, const float*, const
 extern "C" __global__ void btpm_spmm_blocked(const int*, const int*, const float*, const float*,
                                              const float*, const float*, float*, int, int, int,
                                              const btpm_status_dev*);
+extern "C" __global__ void btpm_selfheat_spmm(const int*, const int*, const float*, const float*,
+                                              const float*, float*, int, int,
+                                              const btpm_status_dev*);
 extern "C" __global__ void btpm_scale_cols(float*, const float*, const float*, int, size_t);
 extern "C" __global__ void btpm_delta_max(float*, const float*, float*, size_t);
 extern "C" __global__ void btpm_finish_scatter(float*, const float*, const float*, const float*,
@@ -501,13 +504,12 @@ struct Sched {
         btpm_solve* s = kv.second.first;
         if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext) {
           btpm_shape* sh = s->shape;
-          const int bpt = 16;
-          dim3 g(grid1(sh->nf, 256), grid1(S->d.nb, bpt));
           // qext = src + VF * emis: the total absorbed flux of this rotation
-          hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
+          dim3 g(grid1(sh->nf, 4), grid1((size_t)S->d.nb * BTPM_BLK, 64));
+          hipLaunchKernelGGL(btpm_selfheat_spmm, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                              sh->colidx.as<int>(), sh->vf.as<float>(), S->emis.as<float>(),
-                             (const float*)S->src.as<float>(), (const float*)nullptr,
-                             S->qext.as<float>(), sh->nf, S->d.nb, bpt, ctx->d_status + S->slot);
+                             (const float*)S->src.as<float>(), S->qext.as<float>(), sh->nf,
+                             S->d.nb, ctx->d_status + S->slot);
         }
       }
     } else if (ph == 1) {
