@@ -22,6 +22,7 @@
 
 #include "../../include/btpm.h"
 #include "btpm_internal.h"
+#include "runtime.h"
 
 // ---- kernels (conduct.hip, aux.hip) ----------------------------------------
 extern "C" __global__ void btpm_conduct_smooth(const btpm_tile*, const btpm_set_dev*,
@@ -53,53 +54,14 @@ extern "C" __global__ void btpm_finish_scatter(float*, const float*, const float
                                                int, size_t);
 extern "C" __global__ void btpm_status_kernel(const int*, const btpm_set_dev*, btpm_status_dev*);
 
-namespace {
+using namespace btpm;
 
-thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
+namespace btpm {
+std::string& last_error() {
+  thread_local std::string e;
+  return e;
 }
 
-struct HipError {
-  int code;
-};
-
-#define HIPCHK(x)                                                                   \
-  do {                                                                              \
-    hipError_t e_ = (x);                                                            \
-    if (e_ != hipSuccess) {                                                         \
-      fail(e_ == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP, "%s: %s (%s:%d)", \
-           #x, hipGetErrorString(e_), __FILE__, __LINE__);                          \
-      throw HipError{e_ == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP};    \
-    }                                                                               \
-  } while (0)
-
-#define API_BEGIN try {
-#define API_END                                   \
-  }                                               \
-  catch (const HipError& e) {                     \
-    return e.code;                                \
-  }                                               \
-  catch (const std::bad_alloc&) {                 \
-    return fail(BTPM_ERR_NOMEM, "host allocation failed"); \
-  }                                               \
-  return BTPM_OK;
-
-constexpr int kMaxSets = 1024;
-const double kPi = 3.14159265358979323846;
-
-inline int nblocks(int ntau) { return (ntau + BTPM_BLK - 1) / BTPM_BLK; }
-
-// VF^T as fp16 hi/lo tiles [2][112][112] (A[m=j][k=i] = vf[i*100+j]), zero
-// padded, scaled by 2^s so the largest entry lies in [0.5, 1): the low
-// halves then stay normal fp16 numbers.  Returns 2^-s.
 float vf_tiles(const float* vf, std::vector<_Float16>& a) {
   float vmax = 0.0f;
   for (int e = 0; e < BTPM_NS * BTPM_NS; ++e) vmax = std::max(vmax, std::fabs(vf[e]));
@@ -120,54 +82,12 @@ float vf_tiles(const float* vf, std::vector<_Float16>& a) {
     }
   return std::ldexp(1.0f, -s);
 }
-inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+}  // namespace btpm
 
-// device buffer owned by RAII
-struct DBuf {
-  void* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void alloc(size_t bytes) {
-    release();
-    if (bytes == 0) return;
-    HIPCHK(hipMalloc(&p, bytes));
-    n = bytes;
-  }
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(p);
-  }
-};
-
-template <class T>
-void upload(DBuf& b, const T* h, size_t count, hipStream_t st) {
-  b.alloc(count * sizeof(T));
-  if (count) {
-    HIPCHK(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-  }
-}
-
+namespace {
+constexpr int kMaxSets = 1024;
+const double kPi = 3.14159265358979323846;
 }  // namespace
-
-// ---------------------------------------------------------------------------
-struct btpm_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  btpm_set_dev* d_sets = nullptr;
-  btpm_status_dev* d_status = nullptr;
-  btpm_status_dev* h_status = nullptr;  // pinned mirror
-  std::vector<int> free_slots;
-  std::mutex mu;                        // one scheduler at a time
-};
 
 struct btpm_shape {
   btpm_ctx* ctx = nullptr;
@@ -699,7 +619,7 @@ extern "C" {
 
 int btpm_version(void) { return 100; }
 
-const char* btpm_last_error(void) { return g_err.c_str(); }
+const char* btpm_last_error(void) { return last_error().c_str(); }
 
 void btpm_params_default(btpm_params* p, int ntau, float rot_period, float tacc) {
   p->ntau = ntau;
@@ -1135,7 +1055,7 @@ static void server_loop(btpm_server* srv) {
     srv->stats = sc.stats;
   } catch (const HipError& e) {
     srv->err = e.code;
-    srv->errmsg = g_err;
+    srv->errmsg = last_error();
   }
 }
 

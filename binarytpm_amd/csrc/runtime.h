@@ -1,0 +1,111 @@
+// runtime.h -- host-side helpers shared by libbtpm's translation units:
+// error reporting, RAII device buffers and uploads.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/btpm.h"
+#include "btpm_internal.h"
+
+namespace btpm {
+
+// message of the last failure on this thread (btpm_last_error)
+std::string& last_error();
+
+inline int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  last_error() = buf;
+  return code;
+}
+
+struct HipError {
+  int code;
+};
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      ::btpm::fail(e_ == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP,       \
+                   "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+      throw ::btpm::HipError{e_ == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP}; \
+    }                                                                               \
+  } while (0)
+
+#define API_BEGIN try {
+#define API_END                                              \
+  }                                                          \
+  catch (const ::btpm::HipError& e) {                        \
+    return e.code;                                           \
+  }                                                          \
+  catch (const std::bad_alloc&) {                            \
+    return ::btpm::fail(BTPM_ERR_NOMEM, "host allocation failed"); \
+  }                                                          \
+  return BTPM_OK;
+
+inline int nblocks(int ntau) { return (ntau + BTPM_BLK - 1) / BTPM_BLK; }
+inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// device buffer owned by RAII
+struct DBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return;
+    HIPCHK(hipMalloc(&p, bytes));
+    n = bytes;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+template <class T>
+void upload(DBuf& b, const T* h, size_t count, hipStream_t st) {
+  b.alloc(count * sizeof(T));
+  if (count) {
+    HIPCHK(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+}
+
+// VF^T as fp16 hi/lo tiles [2][112][112] (A[m=j][k=i] = vf[i*100+j]), zero
+// padded, scaled by 2^s so the largest entry lies in [0.5, 1): the low
+// halves then stay normal fp16 numbers.  Returns 2^-s.
+float vf_tiles(const float* vf, std::vector<_Float16>& a);
+
+}  // namespace btpm
+
+// one device: its stream and the resident-set tables of the scheduler
+struct btpm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  btpm_set_dev* d_sets = nullptr;
+  btpm_status_dev* d_status = nullptr;
+  btpm_status_dev* h_status = nullptr;  // pinned mirror
+  std::vector<int> free_slots;
+  std::mutex mu;                        // one scheduler at a time
+};
+
