@@ -2,6 +2,7 @@
 single_model's call order (175706/Run-04/thermal.cpp:3465-3818) through
 ctypes, against the FP64 oracle and the exact cap visibility."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -187,3 +188,28 @@ def test_compat_crater_loop_matches_oracle():
     assert np.abs(s1 - s0).max() < 0.01          # frozen: last two rotations agree
     L.freeup_GPU_temperature_arrays()
     L.freeup_GPU_globalheating_arrays()
+
+
+def test_cli_file_contract(tmp_path):
+    """btpm_thermal reads the reference's input files and writes its
+    temperature files; their contents match the FP64 oracle."""
+    from binarytpm_amd import cli, formats
+    case = inputs.config0(subdiv=1)
+    V, F = inputs.ellipsoid(1, (0.6, 0.5, 0.45))
+    base = str(tmp_path / "body")
+    with open(base + ".txt", "w") as fh:
+        for v in V:
+            fh.write(f"{v[0]:.9g}\t{v[1]:.9g}\t{v[2]:.9g}\n")
+        for f in F:
+            fh.write(f"{f[0]}\t{f[1]}\t{f[2]}\n")
+    formats.write_temperature_dat(base + "_illumination.dat", V, F, case.illum)
+    rc = cli.main(["--shape", base, "--nv", str(len(V)), "--nf", str(len(F)), "--ntau", "650",
+                   "--period", str(case.rot_period), "--ti", "50", "250", "--tacc", "0.01",
+                   "--out", str(tmp_path)])
+    assert rc == 0
+    for ti in (50.0, 250.0):
+        _, _, T = formats.read_temperature_dat(str(tmp_path / f"body_temperature({ti:g}).dat"),
+                                               len(V), len(F), 650)
+        ref = O.smooth_solve(O.params(650, case.rot_period, tacc=0.01), case.illum, 0.1, ti)
+        assert np.abs(T.astype(np.float64) - ref.stemp).max() <= 0.05
+        assert os.path.exists(str(tmp_path / f"body_temperature({ti:g}).txt"))
