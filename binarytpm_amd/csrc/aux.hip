@@ -357,12 +357,14 @@ extern "C" __global__ __launch_bounds__(256) void btpm_spmm_blocked(
 extern "C" __global__ __launch_bounds__(256) void btpm_selfheat_spmm(
     const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ vals,
     const float* __restrict__ emis_t, const float* __restrict__ src, float* __restrict__ y,
-    int nrows, int nb, const btpm_status_dev* __restrict__ st) {
+    int nrows, int nb, const btpm_status_dev* __restrict__ st,
+    const uint8_t* __restrict__ frozen) {
   if (st && st->done) return;
   const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int npad = nb * BTPM_BLK;
   const int s = blockIdx.y * 64 + (threadIdx.x & 63);
   if (f >= nrows) return;
+  if (frozen && frozen[f]) return;  // a frozen facet is not integrated again
   const int sc = s < npad ? s : npad - 1;
   const int k0 = rowptr[f], k1 = rowptr[f + 1];
   // eight independent gathers in flight per lane

@@ -38,18 +38,25 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
-// Newton root of the surface energy balance in FP64.  Returns the new
-// surface temperature; *iters receives the iteration count.
+// Newton root of the surface energy balance in FP64,
+//   g(T) = src + m1 (t1 - T) - es T^4 = 0,   T <- T + g / (m1 + 4 es T^3),
+// stopped when |dT| < tol.  g' is inverted with v_rcp_f64 plus one
+// Newton-Raphson refinement of the reciprocal (relative error ~1e-16, so the
+// iterates are the exact-division iterates to rounding); g is concave and
+// decreasing, so the iteration converges from any positive start.
 __device__ __forceinline__ double sbc_newton(double src, double m1, double t1, double es,
                                              double x, double tol, int cap) {
+  const double es4 = 4.0 * es;
   for (int it = 0; it < cap; ++it) {
-    const double x3 = x * x * x;
-    const double g = src + m1 * (t1 - x) - es * x3 * x;
-    const double gp = m1 + 4.0 * es * x3;
-    const double xn = x + g / gp;
-    const double d = fabs(xn - x);
-    x = xn;
-    if (d < tol) break;
+    const double x2 = x * x;
+    const double x3 = x2 * x;
+    const double g = fma(-es * x3, x, fma(m1, t1 - x, src));
+    const double gp = fma(es4, x3, m1);
+    double rg = __builtin_amdgcn_rcp(gp);
+    rg = fma(rg, fma(-gp, rg, 1.0), rg);
+    const double d = g * rg;
+    x += d;
+    if (fabs(d) < tol) break;
   }
   return x;
 }
@@ -81,11 +88,16 @@ __device__ __forceinline__ void load4(const gfloat* p, float (&v)[BTPM_BLK], int
 }
 
 // FTCS + bottom boundary + FP64 Newton SBC for one step; returns T_0.
+// The Newton start x0 is a predictor: the previous rotation's surface
+// temperature at this step shifted by the current offset between the two
+// rotations at the previous step.  Any positive start converges to the same
+// root (to within the stop tolerance squared); a close one saves iterations.
 template <int NX>
 __device__ __forceinline__ float conduct_step(float (&u)[NX + 1], float r, double src, double m1,
-                                              double es, double Tref, double tol, int cap) {
+                                              double es, double Tref, double tol, int cap,
+                                              double x0) {
   ftcs<NX>(u, r);
-  const double x = sbc_newton(src, m1, Tref + (double)u[1], es, Tref + (double)u[0], tol, cap);
+  const double x = sbc_newton(src, m1, Tref + (double)u[1], es, fmax(x0, 1.0), tol, cap);
   u[0] = (float)(x - Tref);
   return (float)x;
 }
@@ -182,12 +194,15 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
   load8(Tin + coff, nprev);
 
   float dm = 0.0f;
+  float tpl = Tref + u[0];  // previous rotation's T at the previous step
   // the TI = 0 loop is a separate instantiation so the main loop stays lean
   auto rotation = [&](auto inst_tag) {
     constexpr bool INST = decltype(inst_tag)::value;
     for (int b = 0; b < nb; ++b) {
 #pragma unroll
       for (int k = 0; k < BTPM_BLK; ++k) lds_p[k][tid] = nprev[k];
+      // compiler barrier: keep the parked values in LDS, not in VGPRs
+      asm volatile("" ::: "memory");
       const bool more = b + 1 < nb;
       const size_t o = (size_t)b * bstride + coff;
       const size_t on = (size_t)(b + 1) * bstride + coff;
@@ -202,8 +217,10 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
           if constexpr (INST)  // instantaneous equilibrium (TI = 0), thermal.cpp:1985-2033
             t = (float)sqrt(sqrt((double)nsrc[k] / es));
           else
-            t = conduct_step<NX>(u, r, (double)nsrc[k], m1, es, Trefd, tol, cap);
+            t = conduct_step<NX>(u, r, (double)nsrc[k], m1, es, Trefd, tol, cap,
+                                 Trefd + (double)u[0] + (double)(tp - tpl));
           dm = fmaxf(dm, fabsf(t - tp));
+          tpl = tp;
         }
         out[k] = t;
         if (k == BTPM_BLK / 2 - 1 && more) load4(srcp + on, nsrc, 0);
@@ -384,6 +401,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   load8(Tin + coff, nprev);
 
   float dm = 0.0f;
+  float tpl = Tref + u[0];  // previous rotation's T at the previous step
   auto rotation = [&](auto inst_tag) {
     constexpr bool INST = decltype(inst_tag)::value;
     for (int b = 0; b < nb; ++b) {
@@ -422,8 +440,10 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
           if constexpr (INST)  // roughness_inst_equilibrium, thermal_GPU.cu:581-641
             t = (float)sqrt(sqrt(tot / es));
           else
-            t = conduct_step<NX>(u, r, tot, m1, es, Trefd, tol, cap);
+            t = conduct_step<NX>(u, r, tot, m1, es, Trefd, tol, cap,
+                                 Trefd + (double)u[0] + (double)(tp - tpl));
           dm = fmaxf(dm, fabsf(t - tp));
+          tpl = tp;
         }
         out[k] = t;
         if (k == BTPM_BLK / 2 - 1 && more) load4(srcp + on, nsrc, 0);

@@ -47,7 +47,7 @@ extern "C" __global__ void btpm_spmm_blocked(const int*, const int*, const float
                                              const btpm_status_dev*);
 extern "C" __global__ void btpm_selfheat_spmm(const int*, const int*, const float*, const float*,
                                               const float*, float*, int, int,
-                                              const btpm_status_dev*);
+                                              const btpm_status_dev*, const uint8_t*);
 extern "C" __global__ void btpm_scale_cols(float*, const float*, const float*, int, size_t);
 extern "C" __global__ void btpm_delta_max(float*, const float*, float*, size_t);
 extern "C" __global__ void btpm_finish_scatter(float*, const float*, const float*, const float*,
@@ -429,7 +429,8 @@ struct Sched {
           hipLaunchKernelGGL(btpm_selfheat_spmm, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                              sh->colidx.as<int>(), sh->vf.as<float>(), S->emis.as<float>(),
                              (const float*)S->src.as<float>(), S->qext.as<float>(), sh->nf,
-                             S->d.nb, ctx->d_status + S->slot);
+                             S->d.nb, ctx->d_status + S->slot,
+                             S->d.inst ? (const uint8_t*)nullptr : S->frozen.as<uint8_t>());
         }
       }
     } else if (ph == 1) {
