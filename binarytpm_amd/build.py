@@ -15,7 +15,8 @@ LIB = os.path.join(HERE, "libbtpm.so")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # packed FP32 ops stay off on this target
+    # no packed FP32 (v_pk_*_f32) code generation: some of its forms have given
+    # sporadic wrong results on this target, so every kernel is built without it
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]

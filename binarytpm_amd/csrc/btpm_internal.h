@@ -18,6 +18,11 @@
 #define BTPM_CRATERS_WG 10  // craters per crater-conduction workgroup (1000 columns)
 #define BTPM_CRATER_THREADS 1024
 #define BTPM_PSTRIDE 64     // floats per column in the profile array (57 used)
+// largest accepted r = dtau/dz^2 (FTCS is stable for r <= 1/2): the
+// conduction kernels integrate the depth profile in a variable scaled by
+// (1 - 2r)^-k over 8-step blocks, which must stay inside the FP32 range
+// ((1 - 2r)^-8 * 1e4 K < 3e38 needs 1 - 2r > 5.6e-5)
+#define BTPM_R_MAX 0.49995
 
 // kind of a column set
 enum { BTPM_SET_SMOOTH = 0, BTPM_SET_CRATER = 1 };

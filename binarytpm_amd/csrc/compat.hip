@@ -128,6 +128,8 @@ struct Compat {
   int configured = 0;
   int err = BTPM_OK;
   int nf = 0, ntau = 0, nb = 0;
+  int r_bad = -1;         // first facet with dtau/dz^2 outside the stable range
+  float r_bad_val = 0.0f;
   btpm_ctx* ctx = nullptr;
   float rot_period = 1.0f;
   std::vector<float> vf;  // [100][100], vf[i*100+j] = VF(j->i)
@@ -190,9 +192,14 @@ void coefficients() {
   Compat& c = C();
   std::vector<float> r(c.nf), m1(c.nf), oma(c.nf);
   const double dz = 8.0 / BTPM_NX;
+  c.r_bad = -1;
   for (int f = 0; f < c.nf; ++f) {
     const double n = c.h_numrevs[f];
     r[f] = (float)((2.0 * 3.14159265358979323846 * n / c.ntau) / (dz * dz));
+    if (!(r[f] >= 0.0f && r[f] <= (float)BTPM_R_MAX) && c.r_bad < 0) {
+      c.r_bad = f;  // reported when conduction is asked for (shadows need no r)
+      c.r_bad_val = r[f];
+    }
     m1[f] = (float)(c.h_ti[f] * std::sqrt(2.0 * 3.14159265358979323846 * n / c.rot_period) / dz);
     oma[f] = 1.0f - c.h_albedo[f];
   }
@@ -249,6 +256,9 @@ void ensure_source() {
 void rotation(float EPS, float SIGMA, float IR_ABS, float tol, int inst) {
   Compat& c = C();
   hipStream_t st = c.ctx->stream;
+  if (!inst && c.r_bad >= 0)
+    throw HipError{fail(BTPM_ERR_ARG, "facet %d: dtau/dz^2 = %.6g outside [0, %g] (FTCS stability "
+                        "limit)", c.r_bad, (double)c.r_bad_val, BTPM_R_MAX)};
   ensure_source();
   const size_t n = blk();
   hipLaunchKernelGGL(compat_keep_previous, dim3(grid1(n, 256)), dim3(256), 0, st,

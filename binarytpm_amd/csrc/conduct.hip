@@ -61,18 +61,79 @@ __device__ __forceinline__ double sbc_newton(double src, double m1, double t1, d
   return x;
 }
 
-// FTCS on the offset profile with the zero-flux bottom.
+// FTCS in scaled form.  With c = 1 - 2r the step is u_i' = c u_i + r (u_{i-1}
+// + u_{i+1}); holding U = u / s with s = c^k after k steps turns it into
+//     U_i' = U_i + kr (U_{i-1} + U_{i+1}),   kr = r / c,
+// one add and one FMA accumulated in place per node instead of three
+// operations.  The per-step factor is made consistent with the rounded kr as
+// sig = 1 / (1 + 2 kr) (so constants stay exactly constant), tracked in FP64
+// with its inverse, and folded back into U once per 8-step block (U grows by
+// at most c^-8 < 1e32 in between for r <= BTPM_R_MAX).  The zero-flux bottom
+// keeps the copy node u[NX] = u[NX-1] of the reference.
 template <int NX>
-__device__ __forceinline__ void ftcs(float (&u)[NX + 1], float r) {
+__device__ __forceinline__ void ftcs_scaled(float (&u)[NX + 1], float kr) {
   float before = u[0];
 #pragma unroll
   for (int i = 1; i < NX; ++i) {
     const float cur = u[i];
-    const float lap = fmaf(-2.0f, cur, u[i + 1]) + before;
-    u[i] = fmaf(r, lap, cur);
+    u[i] = fmaf(kr, u[i + 1] + before, cur);
     before = cur;
   }
   u[NX] = u[NX - 1];
+}
+
+// Per-column scale state of the scaled recurrence: real offset = sd * U.
+struct Scale {
+  float kr;         // r / (1 - 2r), FP32
+  double sig, isig; // per-step factor 1 / (1 + 2 kr) and its inverse (exact)
+  double sd, isd;   // current scale and its inverse
+};
+
+__device__ __forceinline__ float scaled_kr(float r) { return r / (1.0f - 2.0f * r); }
+
+__device__ __forceinline__ Scale make_scale(float r) {
+  Scale sc;
+  sc.kr = scaled_kr(r);
+  sc.isig = fma(2.0, (double)sc.kr, 1.0);
+  sc.sig = 1.0 / sc.isig;
+  sc.sd = 1.0;
+  sc.isd = 1.0;
+  return sc;
+}
+
+// Scale of the 8 steps of a block for one r, shared by all columns with that
+// r: sk[k] = (sig^(k+1), sig^-(k+1)) in FP32 for the per-step conversions and
+// pk[k] = sig^(k+1) in FP64 for the rescale at the block end.
+__device__ __forceinline__ void scale_table(float r, float2* sk, double* pk) {
+  const double isig = fma(2.0, (double)scaled_kr(r), 1.0);
+  const double sig = 1.0 / isig;
+  double p = 1.0, ip = 1.0;
+  for (int k = 0; k < BTPM_BLK; ++k) {
+    p *= sig;
+    ip *= isig;
+    sk[k] = make_float2((float)p, (float)ip);
+    pk[k] = p;
+  }
+}
+
+// Per-crater constants of the crater kernel, shared by the crater's 100 lanes.
+struct CraterConst {
+  float2 sk[BTPM_BLK];  // scale_table
+  double pk[BTPM_BLK];
+  double r0[2];         // residual of the scale carried from the last rescale
+  double m1;            // Gamma sqrt(2 pi n / P) / dz
+};
+
+// Fold the accumulated scale into the profile: U *= f with f = fp32(sd); the
+// residual sd / f (within 6e-8 of 1) is carried, so the block-to-block gain
+// has no systematic rounding bias.  sd = 1 / isd to second order.
+template <int NX>
+__device__ __forceinline__ void rescale(float (&u)[NX + 1], Scale& sc) {
+  const float f = (float)sc.sd;
+#pragma unroll
+  for (int i = 0; i <= NX; ++i) u[i] *= f;
+  sc.isd *= (double)f;
+  sc.sd = 2.0 - sc.isd;
 }
 
 __device__ __forceinline__ void load8(const gfloat* p, float (&v)[BTPM_BLK]) {
@@ -88,18 +149,28 @@ __device__ __forceinline__ void load4(const gfloat* p, float (&v)[BTPM_BLK], int
 }
 
 // FTCS + bottom boundary + FP64 Newton SBC for one step; returns T_0.
+// (s, is) is the scale after this step and its inverse (FP32 or FP64): the
+// Newton solve sees the real T_1 = Tref + s U_1 and the root goes back into
+// the scaled profile as U_0 = (T_0 - Tref) / s.
 // The Newton start x0 is a predictor: the previous rotation's surface
-// temperature at this step shifted by the current offset between the two
-// rotations at the previous step.  Any positive start converges to the same
-// root (to within the stop tolerance squared); a close one saves iterations.
-template <int NX>
-__device__ __forceinline__ float conduct_step(float (&u)[NX + 1], float r, double src, double m1,
-                                              double es, double Tref, double tol, int cap,
-                                              double x0) {
-  ftcs<NX>(u, r);
-  const double x = sbc_newton(src, m1, Tref + (double)u[1], es, fmax(x0, 1.0), tol, cap);
-  u[0] = (float)(x - Tref);
-  return (float)x;
+// temperature at this step shifted by the difference between the two
+// rotations at the previous step (the caller's dl = t - t_prev).
+template <int NX, typename SC>
+__device__ __forceinline__ double conduct_step(float (&u)[NX + 1], float kr, SC s, SC is,
+                                               double src, double m1, double es, double Tref,
+                                               double tol, int cap, double x0) {
+  ftcs_scaled<NX>(u, kr);
+  double t1;
+  if constexpr (std::is_same<SC, double>::value)
+    t1 = fma(s, (double)u[1], Tref);
+  else
+    t1 = Tref + (double)(s * u[1]);
+  const double x = sbc_newton(src, m1, t1, es, fmax(x0, 1.0), tol, cap);
+  if constexpr (std::is_same<SC, double>::value)
+    u[0] = (float)((x - Tref) * is);
+  else
+    u[0] = (float)(x - Tref) * is;
+  return x;
 }
 
 // profile of one column: 57 nodes in a 64-float record (one base address,
@@ -164,7 +235,7 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
     return;
   }
 
-  const float r = S.r[cc];
+  Scale sc = make_scale(S.r[cc]);
   const double m1 = (double)S.m1[cc];
   const double es = (double)S.es;
   const double tol = (double)S.newton_tol;
@@ -194,7 +265,7 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
   load8(Tin + coff, nprev);
 
   float dm = 0.0f;
-  float tpl = Tref + u[0];  // previous rotation's T at the previous step
+  float dl = 0.0f;  // T - T_prev at the previous step (predictor shift)
   // the TI = 0 loop is a separate instantiation so the main loop stays lean
   auto rotation = [&](auto inst_tag) {
     constexpr bool INST = decltype(inst_tag)::value;
@@ -208,25 +279,32 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
       const size_t on = (size_t)(b + 1) * bstride + coff;
       if (more) load8(Tin + on, nprev);
       const int kend = min(BTPM_BLK, ntau - b * BTPM_BLK);
-      float out[BTPM_BLK];
 #pragma unroll
       for (int k = 0; k < BTPM_BLK; ++k) {
         const float tp = lds_p[k][tid];
         float t = tp;
         if (k < kend) {
-          if constexpr (INST)  // instantaneous equilibrium (TI = 0), thermal.cpp:1985-2033
+          if constexpr (INST) {  // instantaneous equilibrium (TI = 0), thermal.cpp:1985-2033
             t = (float)sqrt(sqrt((double)nsrc[k] / es));
-          else
-            t = conduct_step<NX>(u, r, (double)nsrc[k], m1, es, Trefd, tol, cap,
-                                 Trefd + (double)u[0] + (double)(tp - tpl));
-          dm = fmaxf(dm, fabsf(t - tp));
-          tpl = tp;
+          } else {
+            sc.sd *= sc.sig;
+            sc.isd *= sc.isig;
+            t = (float)conduct_step<NX>(u, sc.kr, sc.sd, sc.isd, (double)nsrc[k], m1, es, Trefd,
+                                        tol, cap, (double)(tp + dl));
+          }
+          dl = t - tp;
+          dm = fmaxf(dm, fabsf(dl));
         }
-        out[k] = t;
+        lds_p[k][tid] = t;  // the slot now holds this rotation's T
         if (k == BTPM_BLK / 2 - 1 && more) load4(srcp + on, nsrc, 0);
         if (k == BTPM_BLK - 1 && more) load4(srcp + on, nsrc, 4);
       }
+      asm volatile("" ::: "memory");
+      if constexpr (!INST) rescale<NX>(u, sc);
       if (active) {
+        float out[BTPM_BLK];
+#pragma unroll
+        for (int k = 0; k < BTPM_BLK; ++k) out[k] = lds_p[k][tid];
         store8(Tout + o, out);
         if (has_e) {
 #pragma unroll
@@ -280,14 +358,15 @@ constexpr int kMP = 112;           // padded M (receiving sub-facet j)
 constexpr int kKP = 112;           // padded K (emitting sub-facet i)
 constexpr int kLdsK = 116;         // LDS row stride in halves (bank spread)
 constexpr int kNP = BTPM_CRATERS_WG * BTPM_BLK;  // 80 columns of the product
-constexpr int kLdsQ = 116;         // Q row stride in floats
+constexpr int kQCols = 1024;       // Q row length: 1000 columns + 24 idle lanes
 constexpr int kMT = kMP / 16;      // 7 M tiles
 constexpr int kNT = kNP / 16;      // 5 N tiles
 constexpr int kKT = kKP / 16;      // 7 K chunks
 
 // Q[n][m] = sum_k A[m][k] B[n][k] over the padded 112 x 112 x 80 tile held in
 // LDS (A = VF^T hi/lo, B = E hi/lo), 7 x 5 output tiles spread over the 16
-// waves.  Fragment layout of v_mfma_f32_16x16x16f16: lane l supplies
+// waves.  Q is written column-major for the conduction lanes: product column
+// n = crater * 8 + step goes to lds_q[step][crater * 100 + m], m < 100.  Fragment layout of v_mfma_f32_16x16x16f16: lane l supplies
 // A[m = l%16][k = 4(l/16) .. +3] and B[k = 4(l/16) .. +3][n = l%16] and
 // receives D[m = 4(l/16) + i][n = l%16], i = 0..3.
 __device__ __forceinline__ void crater_product(const _Float16 (*lds_a)[kMP * kLdsK],
@@ -310,9 +389,12 @@ __device__ __forceinline__ void crater_product(const _Float16 (*lds_a)[kMP * kLd
       acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, blo, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, bhi, acc, 0, 0, 0);
     }
-    float* q = &lds_q[bn * kLdsQ + mt * 16 + kq];
-    *reinterpret_cast<float4*>(q) =
-        make_float4(acc[0] * unscale, acc[1] * unscale, acc[2] * unscale, acc[3] * unscale);
+    const int m0 = mt * 16 + kq;  // 4 receiving sub-facets m0 .. m0 + 3
+    if (m0 < BTPM_NS) {
+      float* q = &lds_q[(bn & (BTPM_BLK - 1)) * kQCols + (bn / BTPM_BLK) * BTPM_NS + m0];
+      *reinterpret_cast<float4*>(q) =
+          make_float4(acc[0] * unscale, acc[1] * unscale, acc[2] * unscale, acc[3] * unscale);
+    }
   }
 }
 
@@ -336,8 +418,9 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   constexpr int P = BTPM_CRATERS_WG;
   __shared__ _Float16 lds_a[2][kMP * kLdsK];   // VF^T hi / lo  (52 KB)
   __shared__ _Float16 lds_b[2][kNP * kLdsK];   // E hi / lo     (37 KB)
-  __shared__ float lds_q[kNP * kLdsQ];         // product       (37 KB)
-  __shared__ float lds_p[BTPM_BLK][BTPM_CRATER_THREADS];  // T_prev (32 KB)
+  __shared__ float lds_q[BTPM_BLK * kQCols];   // product [step][column] (32 KB)
+  __shared__ float lds_p[BTPM_BLK][BTPM_CRATER_THREADS];  // T_prev, then T (32 KB)
+  __shared__ CraterConst lds_cc[P];            // per-crater constants
   __shared__ unsigned lds_cmax[P];
   __shared__ int lds_cnt;
 
@@ -353,7 +436,8 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   const int j = lc - lcr * NS;                     // sub-facet
   const int g = tl.tile * P + lcr;                 // global crater
   const bool valid = tid < P * NS && g < S.ngroups;
-  const int gg = valid ? g : 0;
+  // idle lanes (tid >= 1000) mirror column 999, so they stay well-behaved
+  const int gg = g < S.ngroups ? g : 0;
   const int c = gg * NS + j;                       // global column (clamped)
   const bool active = valid && !S.frozen[gg];
   const int nwork = __syncthreads_count(active);
@@ -370,12 +454,21 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
     stage_vf(reinterpret_cast<const _Float16*>(S.vfA), lds_a, tid);
     for (int e = tid; e < 2 * kNP * kLdsK; e += BTPM_CRATER_THREADS)
       (&lds_b[0][0])[e] = (_Float16)0.0f;
-    if (tid < P) lds_cmax[tid] = 0u;
+    if (tid < P) {
+      lds_cmax[tid] = 0u;
+      const int gt = tl.tile * P + tid < S.ngroups ? tl.tile * P + tid : 0;
+      scale_table(S.r[gt], lds_cc[tid].sk, lds_cc[tid].pk);
+      lds_cc[tid].r0[0] = 1.0;
+      lds_cc[tid].m1 = (double)S.m1[gt];
+    }
+    // the idle lanes' Q slots are never written by the product
+    if (tid >= P * NS)
+      for (int k = 0; k < BTPM_BLK; ++k) lds_q[k * kQCols + tid] = 0.0f;
   }
   __syncthreads();
 
-  const float r = S.r[gg];
-  const double m1 = (double)S.m1[gg];
+  const float kr = scaled_kr(S.r[gg]);
+  CraterConst& cc = lds_cc[lcr];
   const double es = (double)S.es;
   const double tol = (double)S.newton_tol;
   const int cap = S.newton_cap;
@@ -401,7 +494,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   load8(Tin + coff, nprev);
 
   float dm = 0.0f;
-  float tpl = Tref + u[0];  // previous rotation's T at the previous step
+  float dl = 0.0f;  // T - T_prev at the previous step (predictor shift)
   auto rotation = [&](auto inst_tag) {
     constexpr bool INST = decltype(inst_tag)::value;
     for (int b = 0; b < nb; ++b) {
@@ -430,26 +523,45 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
       __syncthreads();
 
       const int kend = min(BTPM_BLK, ntau - b * BTPM_BLK);
-      float out[BTPM_BLK];
 #pragma unroll
       for (int k = 0; k < BTPM_BLK; ++k) {
         const float tp = lds_p[k][tid];
         float t = tp;
         if (k < kend) {
-          const double tot = (double)(nsrc[k] + lds_q[(lcr * BTPM_BLK + k) * kLdsQ + j]);
-          if constexpr (INST)  // roughness_inst_equilibrium, thermal_GPU.cu:581-641
+          const double tot = (double)(nsrc[k] + lds_q[k * kQCols + tid]);
+          if constexpr (INST) {  // roughness_inst_equilibrium, thermal_GPU.cu:581-641
             t = (float)sqrt(sqrt(tot / es));
-          else
-            t = conduct_step<NX>(u, r, tot, m1, es, Trefd, tol, cap,
-                                 Trefd + (double)u[0] + (double)(tp - tpl));
-          dm = fmaxf(dm, fabsf(t - tp));
-          tpl = tp;
+          } else {
+            // the per-step conversions leave out the residual (within 6e-8 of
+            // 1): a bounded perturbation of T_1 and T_0, not a gain on the profile
+            const float2 sk = cc.sk[k];
+            t = (float)conduct_step<NX>(u, kr, sk.x, sk.y, tot, cc.m1, es, Trefd, tol, cap,
+                                        (double)(tp + dl));
+          }
+          dl = t - tp;
+          dm = fmaxf(dm, fabsf(dl));
         }
-        out[k] = t;
+        lds_p[k][tid] = t;  // the slot now holds this rotation's T
         if (k == BTPM_BLK / 2 - 1 && more) load4(srcp + on, nsrc, 0);
         if (k == BTPM_BLK - 1 && more) load4(srcp + on, nsrc, 4);
       }
-      if (active) store8(Tout + o, out);
+      asm volatile("" ::: "memory");
+      if constexpr (!INST) {
+        // U *= f = fp32(r0 sig^kend); the residual r0 sig^kend / f is carried
+        // (cc.r0 is double buffered by block parity: every lane reads one half,
+        // sub-facet 0 writes the other, and the block's barriers order them)
+        const double pb = cc.r0[b & 1] * cc.pk[kend - 1];
+        const float f = (float)pb;
+#pragma unroll
+        for (int i = 0; i <= NX; ++i) u[i] *= f;
+        if (j == 0) cc.r0[(b + 1) & 1] = fma(pb - (double)f, (double)__builtin_amdgcn_rcpf(f), 1.0);
+      }
+      if (active) {
+        float out[BTPM_BLK];
+#pragma unroll
+        for (int k = 0; k < BTPM_BLK; ++k) out[k] = lds_p[k][tid];
+        store8(Tout + o, out);
+      }
     }
   };
   if (S.inst) rotation(std::true_type{});
@@ -490,7 +602,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_contract_
     int ncol, float unscale) {
   __shared__ _Float16 lds_a[2][kMP * kLdsK];
   __shared__ _Float16 lds_b[2][kNP * kLdsK];
-  __shared__ float lds_q[kNP * kLdsQ];
+  __shared__ float lds_q[BTPM_BLK * kQCols];
   const int tid = (int)threadIdx.x;
   const int n0 = blockIdx.x * kNP;
   stage_vf(vfA, lds_a, tid);
@@ -507,6 +619,8 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_contract_
   __syncthreads();
   for (int x = tid; x < kNP * BTPM_NS; x += BTPM_CRATER_THREADS) {
     const int n = x / BTPM_NS, jj = x - n * BTPM_NS;
-    if (n0 + n < ncol) q[(size_t)(n0 + n) * BTPM_NS + jj] = lds_q[n * kLdsQ + jj];
+    if (n0 + n < ncol)
+      q[(size_t)(n0 + n) * BTPM_NS + jj] =
+          lds_q[(n % BTPM_BLK) * kQCols + (n / BTPM_BLK) * BTPM_NS + jj];
   }
 }

@@ -709,6 +709,13 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
   sh->h_albedo.assign(body->albedo, body->albedo + nf);
   sh->h_numrevs.assign(nf, 1.0f);
   if (body->numrevs) sh->h_numrevs.assign(body->numrevs, body->numrevs + nf);
+  for (int f = 0; f < nf; ++f) {  // explicit-scheme stability, as the kernels assume
+    const double dz = 8.0 / BTPM_NX;
+    const float r = (float)((2.0 * kPi * sh->h_numrevs[f] / nt) / (dz * dz));
+    if (!(r >= 0.0f && r <= (float)BTPM_R_MAX))
+      return fail(BTPM_ERR_ARG, "facet %d: dtau/dz^2 = %.6g outside [0, %g] (FTCS stability "
+                  "limit; raise ntau)", f, (double)r, BTPM_R_MAX);
+  }
   std::vector<float> oma(nf);
   for (int f = 0; f < nf; ++f) oma[f] = 1.0f - sh->h_albedo[f];
   upload(sh->albedo, sh->h_albedo.data(), nf, st);

@@ -4,7 +4,7 @@ reported; integer/flag outputs (rotation counts) within one rotation."""
 import numpy as np
 import pytest
 
-from binarytpm_amd import inputs
+from binarytpm_amd import _lib, inputs
 from binarytpm_amd import thermal as th
 from oracle import oracle as O
 
@@ -48,6 +48,28 @@ def test_smooth_parity(ctx, ti):
     assert info.converged and abs(info.rotations - ref.rotations) <= 1
     prof = s.profile()
     assert np.abs(prof - ref.temp).max() <= TOL_K
+
+
+@pytest.mark.parametrize("ntau", [616, 20000])
+def test_smooth_parity_dtau_range(ctx, ntau):
+    """Both ends of dtau/dz^2 for the kernels' scaled recurrence: ntau = 616
+    gives r = 0.49987 (1 - 2r = 2.6e-4, the largest growth between rescales),
+    ntau = 20000 gives r = 0.0154."""
+    case = inputs.config0(subdiv=1, ntau=ntau)
+    p = th.params(ntau, case.rot_period, tacc=0.01)
+    shape = th.Shape(ctx, p, case.illum, 0.1)
+    s = th.Solve(ctx, shape, 200.0).run()
+    ref = O.smooth_solve(O.params(ntau, case.rot_period, tacc=0.01), case.illum, 0.1, 200.0)
+    assert _report(f"smooth ntau={ntau}", s.surface_temps(), ref.stemp) <= TOL_K
+    assert abs(s.info().rotations - ref.rotations) <= 1
+    assert np.abs(s.profile() - ref.temp).max() <= TOL_K
+
+
+def test_unstable_dtau_rejected(ctx):
+    """dtau/dz^2 above the explicit-scheme limit is an argument error."""
+    case = inputs.config0(subdiv=1, ntau=600)             # r = 0.513
+    with pytest.raises(_lib.BtpmError, match="stability"):
+        th.Shape(ctx, th.params(600, case.rot_period, tacc=0.01), case.illum, 0.1)
 
 
 def test_smooth_inst_parity(ctx):
