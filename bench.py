@@ -166,6 +166,102 @@ def plain_reference(ctx, shape, plan, idx):
     return 3600.0 * len(plan) / total, t
 
 
+def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2):
+    """Serving workload: Poisson arrivals (fixed rate, seeded) of solve
+    requests over a pool of resident seeded icosphere-ellipsoid shapes of
+    1,280..81,920 facets, each request with TI ~ U[10, 2500], roughness
+    f ~ U[0, 1] (craters iff f > 0) and r ~ U[0.9, 3.0] AU (flux scale
+    1/r^2 on the 1 AU shape).  The crater sunlight is evaluated on the GPU
+    from per-facet Sun directions.  Latency = submit (at arrival) to done."""
+    import threading
+    from binarytpm_amd import inputs
+    from binarytpm_amd import thermal as th
+    t0 = time.time()
+    cg = inputs.crater_cap(45.0)
+    table = inputs.CraterTable(cg)
+    pool = {}
+    for sd in subdivs:
+        for k in range(seeds_per_size):
+            case, sun_local = inputs.serving_shape(sd, seed=1000 * sd + k)
+            p = th.params(case.ntau, case.rot_period, tacc=0.01)
+            cm = th.CraterModel(vf=cg.vf, sun_local=sun_local, vis_table=table.vis,
+                                normals=cg.normals, solar=inputs.solar_at(1.0), multiple=True,
+                                scatter_acc=1e-4)
+            pool[(sd, k)] = th.Shape(ctx, p, case.illum, 0.1, crater=cm)
+            del sun_local, case
+    log(f"serving pool: {len(pool)} shapes resident in {time.time() - t0:.1f} s")
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    arrive = np.cumsum(gaps) - gaps[0]
+    reqs = [(int(rng.choice(subdivs)), int(rng.integers(seeds_per_size)),
+             float(rng.uniform(10.0, 2500.0)), float(rng.uniform(0.0, 1.0)),
+             float(rng.uniform(0.9, 3.0))) for _ in range(n_req)]
+    srv = th.Server(ctx, th.batch_opts(rotations_per_sync=4, use_graph=True))
+    submitted, results, errors = [], [], []
+    lock = threading.Condition()
+
+    def waiter():
+        i = 0
+        while i < n_req:
+            with lock:
+                while len(submitted) <= i:
+                    lock.wait()
+                s, req = submitted[i]
+            try:
+                srv.wait(s, timeout_s=900.0)
+                inf = s.info()
+                results.append((req, inf.latency_s, inf.converged and
+                                (not s.with_craters or inf.crater_converged),
+                                inf.column_steps + inf.crater_column_steps, time.perf_counter()))
+            except Exception as e:  # noqa: BLE001 -- reported below
+                errors.append(repr(e))
+            s.close()
+            i += 1
+
+    th_w = threading.Thread(target=waiter, daemon=True)
+    th_w.start()
+    t_start = time.perf_counter()
+    for t_arr, req in zip(arrive, reqs):
+        dt = t_start + t_arr - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        sd, k, ti, f, r = req
+        s = th.Solve(ctx, pool[(sd, k)], ti, with_craters=f > 0, flux_scale=1.0 / (r * r))
+        srv.submit(s)
+        with lock:
+            submitted.append((s, req))
+            lock.notify_all()
+    th_w.join()
+    wall = max(x[4] for x in results) - t_start if results else float("nan")
+    st = srv.stop()
+    for sh in pool.values():
+        sh.close()
+    if errors:
+        raise RuntimeError(f"serving errors: {errors[:3]}")
+    lat = np.array([x[1] for x in results])
+    conv = sum(1 for x in results if x[2])
+    cols = sum(x[3] for x in results)
+    sizes = [20 * 4 ** x[0][0] for x in results]
+    log(f"serving: {n_req} requests at {rate}/s in {wall:.1f} s, p50 {np.percentile(lat, 50):.2f} s,"
+        f" p99 {np.percentile(lat, 99):.2f} s, {conv} converged")
+    return {
+        "workload": (f"{n_req} Poisson arrivals at {rate}/s (seed {seed}) over {len(pool)} "
+                     f"resident shapes of {min(sizes)}..{max(sizes)} facets, TI~U[10,2500], "
+                     "f~U[0,1], r~U[0.9,3.0] AU, craters (x100 sub-facets) iff f>0, "
+                     "650 steps, TACC 0.01 K"),
+        "requests": n_req,
+        "rate_per_s": rate,
+        "wall_s": wall,
+        "solves_per_hour": 3600.0 * conv / wall,
+        "converged": conv,
+        "facet_layer_steps_per_s": cols * NX / wall,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_max_s": float(lat.max()),
+        "graph_launches": int(st.graph_launches),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +273,9 @@ def main():
     ap.add_argument("--rps", type=int, default=4, help="rotations per host sync")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-n", type=int, default=40, help="serving requests")
+    ap.add_argument("--serve-rate", type=float, default=1.3, help="serving arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (solves are "
@@ -278,6 +377,8 @@ def main():
         for s in last:
             s.close()
         shape.close()
+        if not args.no_serve:
+            line["serving"] = serve_phase(ctx, args.serve_n, args.serve_rate)
     print(json.dumps(line), flush=True)
 
 

@@ -50,6 +50,8 @@ class Crater(C.Structure):
         ("shadow", C.POINTER(C.c_float)), ("cosang", C.POINTER(C.c_float)),
         ("solar", C.c_float), ("multiple", C.c_int), ("scatter_acc", C.c_float),
         ("drf", C.POINTER(C.c_float)),
+        ("sun_local", C.POINTER(C.c_float)), ("vis_table", C.POINTER(C.c_float)),
+        ("n_el", C.c_int), ("n_az", C.c_int), ("normals", C.POINTER(C.c_float)),
     ]
 
 
@@ -98,6 +100,7 @@ PROTOTYPES = [
     ("btpm_shape_crater_illum", C.c_int, [_vp, _fp]),
     ("btpm_solve_create", C.c_int, [_vp, _vp, C.c_float, _fp, C.c_int, C.POINTER(_vp)]),
     ("btpm_solve_destroy", C.c_int, [_vp]),
+    ("btpm_solve_set_flux_scale", C.c_int, [_vp, C.c_float]),
     ("btpm_solve_run", C.c_int, [_vp]),
     ("btpm_solve_get_info", C.c_int, [_vp, C.POINTER(SolveInfo)]),
     ("btpm_solve_surface_temps", C.c_int, [_vp, _fp]),

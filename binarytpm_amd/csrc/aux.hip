@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "btpm_internal.h"
+#include "btpm_kernels.h"
 
 // ---------------------------------------------------------------------------
 // layout: facet-major x[c][ntau]  <->  blocked x[b][c][BLK]
@@ -153,13 +154,14 @@ extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev 
 // w = weights in blocked layout over the smooth columns:
 //   RGI: w = A_g F_g(s);  RGH: w = es T_g(s)^4 (mode 1 computes it from T).
 // The result is written blocked over crater columns c = f*NS+j as
-//   dst[b][c][k] = base_scale_f * base[b][c][k] + out      (base may be null).
+//   dst[b][c][k] = base_gain * base_scale_f * base[b][c][k] + out   (base may be null).
 // Workgroup: 256 lanes = 4 waves; wave w owns j in [25w, 25w+25), lane = s.
 // ---------------------------------------------------------------------------
 extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
     const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ drf,
     const float* __restrict__ w, int mode, float es, int nsmooth, int nfac, int ntau, int nb,
-    const float* __restrict__ base, const float* __restrict__ base_scale, float* __restrict__ dst) {
+    const float* __restrict__ base, const float* __restrict__ base_scale, float base_gain,
+    float* __restrict__ dst) {
   constexpr int KC = 16;
   __shared__ float ds[KC][BTPM_NS + 4];
   __shared__ float ws[KC][64];
@@ -200,7 +202,7 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
   }
   if (s >= nb * BTPM_BLK) return;
   const int ncols = nfac * BTPM_NS;
-  const float bsc = base_scale ? base_scale[f] : 1.0f;
+  const float bsc = base_gain * (base_scale ? base_scale[f] : 1.0f);
   for (int q = 0; q < 25; ++q) {
     const int c = f * BTPM_NS + jw + q;
     const size_t o = ((size_t)(s / BTPM_BLK) * ncols + c) * BTPM_BLK + (s % BTPM_BLK);
@@ -213,15 +215,52 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
 // ---------------------------------------------------------------------------
 // crater sunlight scattering (roughness_single_scattering /
 // roughness_multiple_scattering, T175706/thermal_GPU.cu:241-341).
-// Inputs shadow, cosang, rgi in the reference layout [f][s][j]; output
-// illum blocked [b][f*NS+j][k].  One workgroup (128 lanes) per (facet, block):
-// lane j holds the 8 steps of sub-facet j.
+// Direct sunlight comes either from shadow, cosang [f][s][j] or, when sun is
+// given, from the Sun direction in the facet's crater frame sun[f][s][3]
+// (its length is the facet's own sunlit fraction, 0 when eclipsed): the
+// sub-facet's sunlit fraction is interpolated bilinearly in vis_table
+// [n_el][n_az][100] (elevation i*90/(n_el-1) deg, azimuth i*360/n_az deg)
+// and its cosine is max(0, n_j . u) with the unit direction u.
+// rgi [f][s][j] may be null.  Output illum blocked [b][f*NS+j][k].  One
+// workgroup (128 lanes) per (facet, block): lane j holds the 8 steps of
+// sub-facet j.
 // ---------------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ float direct_from_sun(const float* __restrict__ d3,
+                                                 const float* __restrict__ vis, int n_el,
+                                                 int n_az, float nx, float ny, float nz, int j) {
+  const float dx = d3[0], dy = d3[1], dz = d3[2];
+  const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+  if (!(len > 0.0f) || !(dz > 0.0f)) return 0.0f;
+  const float il = 1.0f / len;
+  const double ux = (double)(dx * il), uy = (double)(dy * il), uz = (double)(dz * il);
+  const double kPi = 3.14159265358979323846;
+  const double el = asin(fmin(uz, 1.0));
+  double az = atan2(uy, ux);
+  if (az < 0.0) az += 2.0 * kPi;
+  const double fe = fmin(el / (0.5 * kPi / (n_el - 1)), (double)(n_el - 1) - 1e-9);
+  const double fa = az / (2.0 * kPi / n_az);
+  const int ie = (int)floor(fe);
+  const int ia = ((int)floor(fa)) % n_az;
+  const float we = (float)(fe - ie), wa = (float)(fa - floor(fa));
+  const int ie1 = min(ie + 1, n_el - 1), ia1 = (ia + 1) % n_az;
+  const float v00 = vis[((size_t)ie * n_az + ia) * BTPM_NS + j];
+  const float v01 = vis[((size_t)ie * n_az + ia1) * BTPM_NS + j];
+  const float v10 = vis[((size_t)ie1 * n_az + ia) * BTPM_NS + j];
+  const float v11 = vis[((size_t)ie1 * n_az + ia1) * BTPM_NS + j];
+  const float v = (1.0f - we) * ((1.0f - wa) * v00 + wa * v01) +
+                  we * ((1.0f - wa) * v10 + wa * v11);
+  const float c = (float)ux * nx + (float)uy * ny + (float)uz * nz;
+  return (v * len) * (c > 0.0f ? c : 0.0f);
+}
+}  // namespace
+
 extern "C" __global__ __launch_bounds__(128) void btpm_crater_scatter(
     const float* __restrict__ vf32, const float* __restrict__ albedo,
     const float* __restrict__ shadow, const float* __restrict__ cosang,
-    const float* __restrict__ rgi, float solar, int multiple, float acc, int max_iter, int nfac,
-    int ntau, int nb, float* __restrict__ illum_out) {
+    const float* __restrict__ sun, const float* __restrict__ vis_table, int n_el, int n_az,
+    const float* __restrict__ normals, const float* __restrict__ rgi, float solar, int multiple,
+    float acc, int max_iter, int nfac, int ntau, int nb, float* __restrict__ illum_out) {
   __shared__ float vfs[BTPM_NS * BTPM_NS];
   __shared__ float il[BTPM_BLK][BTPM_NS];
   const int f = blockIdx.y;
@@ -230,6 +269,12 @@ extern "C" __global__ __launch_bounds__(128) void btpm_crater_scatter(
   const bool ok = j < BTPM_NS;
   for (int e = j; e < BTPM_NS * BTPM_NS; e += 128) vfs[e] = vf32[e];
   const float a = albedo[f];
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  if (sun && ok) {
+    nx = normals[3 * j];
+    ny = normals[3 * j + 1];
+    nz = normals[3 * j + 2];
+  }
   float sp[BTPM_BLK], res[BTPM_BLK];
 #pragma unroll
   for (int k = 0; k < BTPM_BLK; ++k) {
@@ -237,7 +282,11 @@ extern "C" __global__ __launch_bounds__(128) void btpm_crater_scatter(
     sp[k] = 0.0f;
     if (ok) {
       const size_t o = ((size_t)f * ntau + s) * BTPM_NS + j;
-      sp[k] = solar * shadow[o] * cosang[o] + (rgi ? rgi[o] : 0.0f);
+      const float direct =
+          sun ? direct_from_sun(sun + ((size_t)f * ntau + s) * 3, vis_table, n_el, n_az, nx, ny,
+                                nz, j)
+              : shadow[o] * cosang[o];
+      sp[k] = solar * direct + (rgi ? rgi[o] : 0.0f);
       il[k][j] = sp[k];
     }
   }
@@ -392,12 +441,14 @@ extern "C" __global__ __launch_bounds__(256) void btpm_selfheat_spmm(
 }
 
 // elementwise helpers over blocked arrays of n floats
+//   y = gain * scale_c * x
 extern "C" __global__ void btpm_scale_cols(float* __restrict__ y, const float* __restrict__ x,
-                                           const float* __restrict__ scale, int ncols, size_t n) {
+                                           const float* __restrict__ scale, float gain, int ncols,
+                                           size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = (int)((i / BTPM_BLK) % ncols);
-  y[i] = scale[c] * x[i];
+  y[i] = (gain * scale[c]) * x[i];
 }
 
 // p_new = alpha*(d + q) has been written to y; record max |y - p| per

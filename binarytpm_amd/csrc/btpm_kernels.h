@@ -1,0 +1,58 @@
+// btpm_kernels.h -- prototypes of every kernel shared between translation
+// units.  The defining files include this header too, so a definition that
+// drifts from its prototype is a compile error (conflicting extern "C"
+// declarations) rather than a silent argument mismatch at launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "btpm_internal.h"
+
+// conduct.hip
+extern "C" __global__ void btpm_conduct_smooth(const btpm_tile* tiles, const btpm_set_dev* sets,
+                                               const btpm_status_dev* status);
+extern "C" __global__ void btpm_conduct_crater(const btpm_tile* tiles, const btpm_set_dev* sets,
+                                               const btpm_status_dev* status);
+extern "C" __global__ void btpm_contract_kernel(const _Float16* vfA, const float* e, float* q,
+                                                int ncol, float unscale);
+
+// aux.hip
+extern "C" __global__ void btpm_pack_blocked(const float* raw, float* blk, const float* scale,
+                                             int ncols, int ntau, int nb);
+extern "C" __global__ void btpm_unpack_blocked(const float* blk, float* raw, int ncols, int ntau,
+                                               int nb);
+extern "C" __global__ void btpm_pack_crater(const float* raw, float* blk, int nfac, int ntau,
+                                            int nb);
+extern "C" __global__ void btpm_unpack_crater(const float* blk, float* raw, int nfac, int ntau,
+                                              int nb);
+extern "C" __global__ void btpm_init_smooth(btpm_set_dev S);
+extern "C" __global__ void btpm_init_crater(btpm_set_dev S, const float* vf32);
+extern "C" __global__ void btpm_shape_to_crater(const int* rowptr, const int* colidx,
+                                                const float* drf, const float* w, int mode,
+                                                float es, int nsmooth, int nfac, int ntau, int nb,
+                                                const float* base, const float* base_scale,
+                                                float base_gain, float* dst);
+extern "C" __global__ void btpm_crater_scatter(const float* vf32, const float* albedo,
+                                               const float* shadow, const float* cosang,
+                                               const float* sun, const float* vis_table,
+                                               int n_el, int n_az, const float* normals,
+                                               const float* rgi, float solar, int multiple,
+                                               float acc, int max_iter, int nfac, int ntau,
+                                               int nb, float* illum_out);
+extern "C" __global__ void btpm_spmm_blocked(const int* rowptr, const int* colidx,
+                                             const float* vals, const float* x, const float* x0,
+                                             const float* alpha, float* y, int nrows, int nb,
+                                             int bpt, const btpm_status_dev* st);
+extern "C" __global__ void btpm_selfheat_spmm(const int* rowptr, const int* colidx,
+                                              const float* vals, const float* emis_t,
+                                              const float* src, float* y, int nrows, int nb,
+                                              const btpm_status_dev* st, const uint8_t* frozen);
+extern "C" __global__ void btpm_scale_cols(float* y, const float* x, const float* scale,
+                                           float gain, int ncols, size_t n);
+extern "C" __global__ void btpm_delta_max(float* p, const float* y, float* part, size_t n);
+extern "C" __global__ void btpm_finish_scatter(float* F, const float* p, const float* d,
+                                               const float* albedo, int ncols, size_t n);
+extern "C" __global__ void btpm_status_kernel(const int* ids, const btpm_set_dev* sets,
+                                              btpm_status_dev* status);

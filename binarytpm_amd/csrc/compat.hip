@@ -17,17 +17,11 @@
 #include "../../include/btpm.h"
 #include "../../include/btpm_compat.h"
 #include "btpm_internal.h"
+#include "btpm_kernels.h"
 #include "runtime.h"
 
 using namespace btpm;
 
-extern "C" __global__ void btpm_conduct_crater(const btpm_tile*, const btpm_set_dev*,
-                                               const btpm_status_dev*);
-extern "C" __global__ void btpm_init_crater(btpm_set_dev, const float*);
-extern "C" __global__ void btpm_crater_scatter(const float*, const float*, const float*,
-                                               const float*, const float*, float, int, float, int,
-                                               int, int, int, float*);
-extern "C" __global__ void btpm_unpack_crater(const float*, float*, int, int, int);
 
 namespace {
 
@@ -558,7 +552,8 @@ static void scatter(float solar, int multiple, float acc) {
   if (!c.illum.p) c.illum.alloc(blk() * 4);
   hipLaunchKernelGGL(btpm_crater_scatter, dim3(c.nb, c.nf), dim3(128), 0, c.ctx->stream,
                      c.vf32.as<float>(), c.albedo.as<float>(), c.sfc_shadow.as<float>(),
-                     c.illumangle.as<float>(), c.rgi.as<float>(), solar, multiple, acc, 100000,
+                     c.illumangle.as<float>(), (const float*)nullptr, (const float*)nullptr, 0, 0,
+                     (const float*)nullptr, c.rgi.as<float>(), solar, multiple, acc, 100000,
                      c.nf, c.ntau, c.nb, c.illum.as<float>());
   HIPCHK(hipGetLastError());
   c.src_dirty = true;

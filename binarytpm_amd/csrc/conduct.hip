@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "btpm_internal.h"
+#include "btpm_kernels.h"
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 // Pointers read from the set descriptor are generic; casting them to the

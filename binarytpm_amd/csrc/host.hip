@@ -22,37 +22,10 @@
 
 #include "../../include/btpm.h"
 #include "btpm_internal.h"
+#include "btpm_kernels.h"
 #include "runtime.h"
 
 // ---- kernels (conduct.hip, aux.hip) ----------------------------------------
-extern "C" __global__ void btpm_conduct_smooth(const btpm_tile*, const btpm_set_dev*,
-                                               const btpm_status_dev*);
-extern "C" __global__ void btpm_conduct_crater(const btpm_tile*, const btpm_set_dev*,
-                                               const btpm_status_dev*);
-extern "C" __global__ void btpm_contract_kernel(const _Float16*, const float*, float*, int, float);
-extern "C" __global__ void btpm_pack_blocked(const float*, float*, const float*, int, int, int);
-extern "C" __global__ void btpm_unpack_blocked(const float*, float*, int, int, int);
-extern "C" __global__ void btpm_pack_crater(const float*, float*, int, int, int);
-extern "C" __global__ void btpm_unpack_crater(const float*, float*, int, int, int);
-extern "C" __global__ void btpm_init_smooth(btpm_set_dev);
-extern "C" __global__ void btpm_init_crater(btpm_set_dev, const float*);
-extern "C" __global__ void btpm_shape_to_crater(const int*, const int*, const float*, const float*,
-                                                int, float, int, int, int, int, const float*,
-                                                const float*, float*);
-extern "C" __global__ void btpm_crater_scatter(const float*, const float*, const float*,
-                                               const float*, const float*, float, int, float, int,
-                                               int, int, int, float*);
-extern "C" __global__ void btpm_spmm_blocked(const int*, const int*, const float*, const float*,
-                                             const float*, const float*, float*, int, int, int,
-                                             const btpm_status_dev*);
-extern "C" __global__ void btpm_selfheat_spmm(const int*, const int*, const float*, const float*,
-                                              const float*, float*, int, int,
-                                              const btpm_status_dev*, const uint8_t*);
-extern "C" __global__ void btpm_scale_cols(float*, const float*, const float*, int, size_t);
-extern "C" __global__ void btpm_delta_max(float*, const float*, float*, size_t);
-extern "C" __global__ void btpm_finish_scatter(float*, const float*, const float*, const float*,
-                                               int, size_t);
-extern "C" __global__ void btpm_status_kernel(const int*, const btpm_set_dev*, btpm_status_dev*);
 
 using namespace btpm;
 
@@ -124,6 +97,8 @@ struct btpm_solve {
   float ti = 0.0f;
   std::vector<float> ti_facet;
   int with_craters = 0;
+  float flux_scale = 1.0f;  // multiplies every absorbed flux of the shape
+  bool queued = false;      // handed to a server (not yet admitted)
   // 0 created, 1 smooth running, 2 crater running, 3 done
   int phase = 0;
   std::unique_ptr<SetHost> smooth, crater;
@@ -208,7 +183,7 @@ void build_smooth(btpm_solve* s, hipStream_t st) {
   d.tile_cols = BTPM_SMOOTH_WG;
   // src = (1-A) F
   hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st, S->src.as<float>(),
-                     sh->F.as<float>(), sh->one_minus_a.as<float>(), d.ncols, nser);
+                     sh->F.as<float>(), sh->one_minus_a.as<float>(), s->flux_scale, d.ncols, nser);
   hipLaunchKernelGGL(btpm_init_smooth, dim3(grid1(d.ncols, 256)), dim3(256), 0, st, d);
   HIPCHK(hipGetLastError());
   s->smooth = std::move(S);
@@ -257,7 +232,8 @@ void build_crater(btpm_solve* s, hipStream_t st) {
     hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                        sh->colidx.as<int>(), sh->drf.as<float>(), s->smooth->T.as<float>(), 1,
                        sh->p.eps * sh->p.sigma, sh->nf, sh->nf, sh->p.ntau, d.nb,
-                       sh->cillum.as<float>(), sh->one_minus_a.as<float>(), S->src.as<float>());
+                       sh->cillum.as<float>(), sh->one_minus_a.as<float>(), s->flux_scale,
+                       S->src.as<float>());
   } else {
     // (1-A_f) per crater column: scale with a per-column expansion
     std::vector<float> oma((size_t)d.ncols);
@@ -266,7 +242,8 @@ void build_crater(btpm_solve* s, hipStream_t st) {
     DBuf tmp;
     upload(tmp, oma.data(), oma.size(), st);
     hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st,
-                       S->src.as<float>(), sh->cillum.as<float>(), tmp.as<float>(), d.ncols, nser);
+                       S->src.as<float>(), sh->cillum.as<float>(), tmp.as<float>(), s->flux_scale,
+                       d.ncols, nser);
     HIPCHK(hipStreamSynchronize(st));
   }
   hipLaunchKernelGGL(btpm_init_crater, dim3(d.ngroups), dim3(128), 0, st, d,
@@ -282,11 +259,15 @@ void shrink_smooth(SetHost* S) {
   S->emis.release();
 }
 
-size_t solve_bytes(const btpm_shape* sh, int with_craters) {
+// device bytes a solve allocates: its smooth set at admission, its crater
+// set at the smooth -> crater transition
+size_t smooth_bytes(const btpm_shape* sh) {
   const size_t nb = nblocks(sh->p.ntau);
-  size_t b = (size_t)sh->nf * (nb * BTPM_BLK * (sh->nnz ? 4 : 2) + BTPM_NX + 1) * 4;
-  if (with_craters) b += (size_t)sh->nf * BTPM_NS * (nb * BTPM_BLK * 2 + BTPM_NX + 1) * 4;
-  return b + (1 << 20);
+  return (size_t)sh->nf * (nb * BTPM_BLK * (sh->nnz ? 4 : 2) + BTPM_PSTRIDE + 16) * 4 + (1 << 20);
+}
+size_t crater_bytes(const btpm_shape* sh) {
+  const size_t nb = nblocks(sh->p.ntau);
+  return (size_t)sh->nf * BTPM_NS * (nb * BTPM_BLK * 2 + BTPM_PSTRIDE + 4) * 4 + (1 << 20);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,14 +290,18 @@ struct Sched {
   btpm_batch_stats stats{};
   // launch timing (opts.profile): events around each conduction launch
   std::vector<hipEvent_t> ev_s, ev_c;  // pairs (start, end) per rotation
-  size_t budget = 0, in_use = 0;
+  // Admission works on the device's live free memory, so results that
+  // completed solves keep until they are destroyed are accounted for:
+  size_t headroom = 0;  // kept free: (1 - mem_fraction) of what was free at start
+  size_t reserved = 0;  // crater bytes promised to admitted solves not yet allocated
   int craters_active = 0;
   std::chrono::steady_clock::time_point t0;
 
   explicit Sched(btpm_ctx* c, const btpm_batch_opts& o) : ctx(c), opt(o) {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
-    budget = (size_t)(fr * (opt.mem_fraction > 0 ? opt.mem_fraction : 0.8));
+    const double frac = opt.mem_fraction > 0 ? std::min((double)opt.mem_fraction, 1.0) : 0.8;
+    headroom = (size_t)((double)fr * (1.0 - frac));
     t0 = std::chrono::steady_clock::now();
   }
   ~Sched() {
@@ -380,16 +365,18 @@ struct Sched {
   bool can_admit(btpm_solve* s) {
     if (s->with_craters && opt.max_active_craters > 0 && craters_active >= opt.max_active_craters)
       return false;
-    const size_t need = solve_bytes(s->shape, s->with_craters);
     if (active.empty()) return true;  // always let one in
-    return in_use + need <= budget;
+    const size_t need = smooth_bytes(s->shape) + (s->with_craters ? crater_bytes(s->shape) : 0);
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    return fr > headroom + reserved && need <= fr - headroom - reserved;
   }
 
   void admit() {
     while (!pending.empty() && can_admit(pending.front())) {
       btpm_solve* s = pending.front();
       pending.pop_front();
-      in_use += solve_bytes(s->shape, s->with_craters);
+      if (s->with_craters) reserved += crater_bytes(s->shape);
       if (s->with_craters) craters_active++;
       build_smooth(s, ctx->stream);
       s->phase = 1;
@@ -559,7 +546,6 @@ struct Sched {
   void complete(btpm_solve* s) {
     s->info.latency_s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t_submit).count();
-    in_use -= std::min(in_use, solve_bytes(s->shape, s->with_craters));
     if (s->with_craters) craters_active--;
     stats.solves++;
     if (s->info.converged && (!s->with_craters || s->info.crater_converged)) stats.converged++;
@@ -581,6 +567,7 @@ struct Sched {
           finish_set(s, S);
           shrink_smooth(S);
           if (s->with_craters) {
+            reserved -= std::min(reserved, crater_bytes(s->shape));
             build_crater(s, ctx->stream);
             s->phase = 2;
             add_set(s, s->crater.get());
@@ -692,8 +679,10 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
   if ((body->nnz > 0) != (body->rowptr != nullptr))
     return fail(BTPM_ERR_ARG, "nnz and rowptr disagree");
   if (crater && !crater->vf) return fail(BTPM_ERR_ARG, "crater without view factors");
-  if (crater && !crater->illum && (!crater->shadow || !crater->cosang))
-    return fail(BTPM_ERR_ARG, "crater needs illum or shadow+cosang");
+  if (crater && !crater->illum && (!crater->shadow || !crater->cosang) &&
+      (!crater->sun_local || !crater->vis_table || !crater->normals || crater->n_el < 2 ||
+       crater->n_az < 1))
+    return fail(BTPM_ERR_ARG, "crater needs illum, shadow+cosang, or sun_local+vis_table+normals");
   if (p->max_rev < 1 || p->newton_cap < 1 || !(p->tacc > 0) || !(p->newton_tol > 0))
     return fail(BTPM_ERR_ARG, "bad convergence parameters");
   API_BEGIN
@@ -754,7 +743,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     part.alloc((size_t)nblk * 4);
     HIPCHK(hipMemcpyAsync(D.p, sh->F.p, nser * 4, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(btpm_scale_cols, dim3(nblk), dim3(256), 0, st, AD.as<float>(),
-                       D.as<float>(), sh->albedo.as<float>(), nf, nser);
+                       D.as<float>(), sh->albedo.as<float>(), 1.0f, nf, nser);
     const int bpt = 16;
     dim3 g(grid1(nf, 256), grid1(nb, bpt));
     hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
@@ -764,7 +753,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     if (body->scatter >= 2) {
       const float acc = body->scatter_acc > 0 ? body->scatter_acc : 1e-4f;
       hipLaunchKernelGGL(btpm_scale_cols, dim3(nblk), dim3(256), 0, st, p.as<float>(),
-                         sh->F.as<float>(), sh->albedo.as<float>(), nf, nser);
+                         sh->F.as<float>(), sh->albedo.as<float>(), 1.0f, nf, nser);
       std::vector<float> hpart(nblk);
       for (int it = 0; it < 1000; ++it) {
         hipLaunchKernelGGL(btpm_spmm_blocked, g, dim3(256), 0, st, sh->rowptr.as<int>(),
@@ -800,9 +789,16 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
                          0, st, raw.as<float>(), sh->cillum.as<float>(), nf, nt, nb);
       HIPCHK(hipStreamSynchronize(st));
     } else {
-      DBuf dsh, dcos, rgi_blk, rgi;
-      upload(dsh, crater->shadow, (size_t)nf * nt * BTPM_NS, st);
-      upload(dcos, crater->cosang, (size_t)nf * nt * BTPM_NS, st);
+      DBuf dsh, dcos, dsun, dvis, dnrm, rgi_blk, rgi;
+      const bool by_sun = crater->sun_local != nullptr;
+      if (by_sun) {  // shadow and cosine evaluated in the scattering kernel
+        upload(dsun, crater->sun_local, (size_t)nf * nt * 3, st);
+        upload(dvis, crater->vis_table, (size_t)crater->n_el * crater->n_az * BTPM_NS, st);
+        upload(dnrm, crater->normals, (size_t)BTPM_NS * 3, st);
+      } else {
+        upload(dsh, crater->shadow, (size_t)nf * nt * BTPM_NS, st);
+        upload(dcos, crater->cosang, (size_t)nf * nt * BTPM_NS, st);
+      }
       const float* rgip = nullptr;
       if (sh->drf.p) {
         // RGI = sum_g drf * A_g F_g(s): blocked over crater columns, then
@@ -810,12 +806,13 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
         DBuf AF;
         AF.alloc(nser * 4);
         hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st,
-                           AF.as<float>(), sh->F.as<float>(), sh->albedo.as<float>(), nf, nser);
+                           AF.as<float>(), sh->F.as<float>(), sh->albedo.as<float>(), 1.0f, nf,
+                           nser);
         rgi_blk.alloc(ncser * 4);
         dim3 g(grid1((size_t)nb * BTPM_BLK, 64), nf);
         hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                            sh->colidx.as<int>(), sh->drf.as<float>(), AF.as<float>(), 0, 0.0f, nf,
-                           nf, nt, nb, (const float*)nullptr, (const float*)nullptr,
+                           nf, nt, nb, (const float*)nullptr, (const float*)nullptr, 1.0f,
                            rgi_blk.as<float>());
         rgi.alloc((size_t)nf * nt * BTPM_NS * 4);
         hipLaunchKernelGGL(btpm_unpack_crater, dim3(grid1((size_t)nf * BTPM_NS * nb, 256)),
@@ -827,8 +824,10 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
       const float acc = crater->scatter_acc > 0 ? crater->scatter_acc : 1e-4f;
       hipLaunchKernelGGL(btpm_crater_scatter, dim3(nb, nf), dim3(128), 0, st,
                          sh->cvf32.as<float>(), sh->albedo.as<float>(), dsh.as<float>(),
-                         dcos.as<float>(), rgip, crater->solar, crater->multiple, acc, 10000, nf,
-                         nt, nb, sh->cillum.as<float>());
+                         dcos.as<float>(), dsun.as<float>(), dvis.as<float>(),
+                         by_sun ? crater->n_el : 0, by_sun ? crater->n_az : 0, dnrm.as<float>(),
+                         rgip, crater->solar, crater->multiple, acc, 10000, nf, nt, nb,
+                         sh->cillum.as<float>());
       HIPCHK(hipStreamSynchronize(st));
     }
   }
@@ -891,6 +890,15 @@ int btpm_solve_create(btpm_ctx* ctx, btpm_shape* shape, float ti, const float* t
   return BTPM_OK;
 }
 
+int btpm_solve_set_flux_scale(btpm_solve* s, float scale) {
+  if (!s) return fail(BTPM_ERR_ARG, "null argument");
+  if (!(scale > 0.0f) || !std::isfinite(scale))
+    return fail(BTPM_ERR_ARG, "flux scale must be positive and finite (got %g)", (double)scale);
+  if (s->phase != 0 || s->queued) return fail(BTPM_ERR_STATE, "solve already started");
+  s->flux_scale = scale;
+  return BTPM_OK;
+}
+
 int btpm_solve_destroy(btpm_solve* s) {
   if (!s) return BTPM_OK;
   (void)hipSetDevice(s->ctx->device);
@@ -903,7 +911,8 @@ int btpm_batch_run(btpm_ctx* ctx, btpm_solve** solves, int n, const btpm_batch_o
                    btpm_batch_stats* stats) {
   if (!ctx || (n > 0 && !solves)) return fail(BTPM_ERR_ARG, "null argument");
   for (int i = 0; i < n; ++i)
-    if (!solves[i] || solves[i]->phase != 0) return fail(BTPM_ERR_STATE, "solve %d not fresh", i);
+    if (!solves[i] || solves[i]->phase != 0 || solves[i]->queued)
+      return fail(BTPM_ERR_STATE, "solve %d not fresh", i);
   std::lock_guard<std::mutex> lk(ctx->mu);
   API_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
@@ -1082,10 +1091,11 @@ int btpm_server_start(btpm_ctx* ctx, const btpm_batch_opts* opts, btpm_server** 
 
 int btpm_server_submit(btpm_server* srv, btpm_solve* s) {
   if (!srv || !s) return fail(BTPM_ERR_ARG, "null argument");
-  if (s->phase != 0) return fail(BTPM_ERR_STATE, "solve already used");
+  if (s->phase != 0 || s->queued) return fail(BTPM_ERR_STATE, "solve already used");
   {
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return fail(BTPM_ERR_STATE, "server stopping");
+    s->queued = true;
     s->t_submit = std::chrono::steady_clock::now();
     srv->inbox.push_back(s);
   }

@@ -352,6 +352,16 @@ def crater_illumination(frames, sun, facet_mask, table: CraterTable, chunk: int 
     return shadow.reshape(nf, nt, NS), cosang.reshape(nf, nt, NS)
 
 
+def crater_sun_local(frames, sun, facet_mask=None) -> np.ndarray:
+    """Sun direction in every facet's crater frame, [nf][ntau][3] float32,
+    scaled by the facet's own sunlit fraction (the GPU-side crater
+    illumination input; see btpm_crater.sun_local)."""
+    loc = np.einsum("fij,sj->fsi", frames, sun)
+    if facet_mask is not None:
+        loc = loc * np.asarray(facet_mask)[:, :, None]
+    return np.ascontiguousarray(loc, dtype=np.float32)
+
+
 def shape_to_crater_vf(csr, dirs, table: CraterTable):
     """drf[k][j] = clamp(VF_k * vis_j(d_k) * max(0, n_j . d_k) / d_k,z, 0, 1)
     per CSR entry k (cf. calculate_surface_global_viewfactors,
@@ -445,8 +455,27 @@ def rough_primary(subdiv: int = 4, ntau: int = 650, r_au: float = 1.1, seed: int
                       scatter=2)
     k = nf if nfac is None else nfac
     shadow, cosang = crater_illumination(frames[:k], sun, np.ones((k, ntau), np.float32), table)
-    crater = dict(vf=cg.vf, shadow=shadow, cosang=cosang, solar=solar, drf=drf, geom=cg)
+    crater = dict(vf=cg.vf, shadow=shadow, cosang=cosang, solar=solar, drf=drf, geom=cg,
+                  table=table)
     return case, crater
+
+
+def serving_shape(subdiv: int, seed: int, ntau: int = 650):
+    """Config-4 request shape at 1 AU: a seeded icosphere-ellipsoid (semi-axes
+    a = 0.3..1.2 km, b = a*U[0.7,1], c = b*U[0.7,1]; 20*4^subdiv facets), spin
+    axis normal to the orbit, P = 4.3 h, A = 0.1, no facet view factors.
+    Other heliocentric distances r are requests with flux scale 1/r^2.
+    Returns (SmoothCase with the direct illumination, sun_local [nf][ntau][3])."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.3, 1.2)
+    b = a * rng.uniform(0.7, 1.0)
+    c = b * rng.uniform(0.7, 1.0)
+    V, F = ellipsoid(subdiv, (a, b, c))
+    n, _, _, frames = facet_geometry(V, F)
+    sun = sun_body_frame(ntau)
+    D = direct_illumination(n, sun, solar_at(1.0))
+    case = SmoothCase(D, 0.1, ntau, P_PRIMARY, frames=frames, sun=sun)
+    return case, crater_sun_local(frames, sun)
 
 
 def sweep_inertias(n: int = 32, lo: float = 10.0, hi: float = 2500.0):

@@ -87,6 +87,10 @@ class CraterModel:
     multiple: bool = True
     scatter_acc: float = 1e-4
     drf: Optional[np.ndarray] = None      # [nnz][100] shape->crater view factors
+    # direct sunlight evaluated on the GPU instead of shadow/cosang:
+    sun_local: Optional[np.ndarray] = None  # [nf][ntau][3] Sun in the crater frame x sunlit fraction
+    vis_table: Optional[np.ndarray] = None  # [n_el][n_az][100] sub-facet sunlit fraction
+    normals: Optional[np.ndarray] = None    # [100][3] sub-facet normals
 
 
 class Shape:
@@ -133,6 +137,15 @@ class Shape:
                     arr = _f32(arr, (nf, nt, NS))
                     keep.append(arr)
                     setattr(cr, name, _ptr(arr))
+            if crater.sun_local is not None:
+                sun = _f32(crater.sun_local, (nf, nt, 3))
+                vis = _f32(crater.vis_table)
+                if vis.ndim != 3 or vis.shape[2] != NS:
+                    raise ValueError("vis_table must be [n_el][n_az][100]")
+                nrm = _f32(crater.normals, (NS, 3))
+                keep += [sun, vis, nrm]
+                cr.sun_local, cr.vis_table, cr.normals = _ptr(sun), _ptr(vis), _ptr(nrm)
+                cr.n_el, cr.n_az = int(vis.shape[0]), int(vis.shape[1])
             if crater.drf is not None:
                 drf = _f32(crater.drf, (self.nnz, NS))
                 keep.append(drf)
@@ -171,12 +184,18 @@ class Solve:
     """One thermal-inertia solve of a shape (smooth facets, then craters)."""
 
     def __init__(self, ctx: Context, shape: Shape, ti: float, ti_per_facet=None,
-                 with_craters: bool = False):
+                 with_craters: bool = False, flux_scale: float = 1.0):
         tpf = None if ti_per_facet is None else _f32(ti_per_facet, (shape.nf,))
         h = C.c_void_p()
         check(lib().btpm_solve_create(ctx.handle, shape.handle, float(ti), _ptr(tpf),
                                       int(bool(with_craters)), C.byref(h)))
         self._h = h
+        if flux_scale != 1.0:
+            rc = lib().btpm_solve_set_flux_scale(h, float(flux_scale))
+            if rc != 0:
+                lib().btpm_solve_destroy(h)
+                self._h = None
+                check(rc)
         self.shape = shape
         self.ti = float(ti)
         self.with_craters = bool(with_craters)

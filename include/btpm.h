@@ -92,7 +92,17 @@ typedef struct {
  * (roughness_single/multiple_scattering, thermal_GPU.cu:241-341).
  * drf [nnz][100] (optional) are the shape->crater view factors per CSR entry
  * of the body (DepRoughFacet::viewfactor(1..100), thermal.cpp:318-334); they
- * drive RGI and RGH.                                                         */
+ * drive RGI and RGH.
+ * A third source of direct crater sunlight (no [nf][ntau][100] arrays at
+ * all): sun_local [nf][ntau][3] is the Sun direction in each facet's crater
+ * frame (z = facet normal) with length = the facet's own sunlit fraction
+ * (0 when shadowed or eclipsed); vis_table [n_el][n_az][100] the sunlit
+ * fraction of every sub-facet at elevation i*90/(n_el-1) deg and azimuth
+ * i*360/n_az deg, interpolated bilinearly; normals [100][3] the sub-facet
+ * normals.  The GPU evaluates shadow = |d| vis(d/|d|) and
+ * cosang = max(0, n_j . d/|d|) inside the scattering kernel (the role of
+ * sfcpartialshadow, 3905/Thermal/NominalPole/GPU_code.cu:152-210, with a
+ * tabulated cap instead of its hemisphere rule).                            */
 typedef struct {
     const float *vf;
     const float *illum;
@@ -102,6 +112,10 @@ typedef struct {
     int multiple;            /* multiple scattering */
     float scatter_acc;
     const float *drf;
+    const float *sun_local;
+    const float *vis_table;
+    int n_el, n_az;
+    const float *normals;
 } btpm_crater;
 
 typedef struct {
@@ -118,7 +132,10 @@ typedef struct {
     int rotations_per_sync;      /* rotations enqueued between host checks */
     int use_graph;               /* capture each rotation set in a hipGraph */
     int max_active_craters;      /* crater solves resident at once (0: by memory) */
-    double mem_fraction;         /* share of free HBM the batch may hold   */
+    double mem_fraction;         /* admission keeps (1 - mem_fraction) of
+                                    the HBM free at start unused; it checks
+                                    live free memory, so results of finished
+                                    solves count until they are destroyed  */
     int profile;                 /* time every conduction launch with events */
 } btpm_batch_opts;
 
@@ -176,6 +193,13 @@ int btpm_shape_crater_illum(btpm_shape *shape, float *out);
 int btpm_solve_create(btpm_ctx *ctx, btpm_shape *shape, float ti, const float *ti_per_facet,
                       int with_craters, btpm_solve **out);
 int btpm_solve_destroy(btpm_solve *s);
+/* Scales every absorbed flux of this solve: the facet illumination after
+ * scattering and the crater illumination, both linear in the solar flux
+ * (e.g. (r_ref / r)^2 places the shape at distance r instead of r_ref).
+ * Call before the solve runs.  No counterpart in the reference, which bakes
+ * the solar constant into its illumination (T3905/main.cpp:905); it lets
+ * requests at different heliocentric distances share one resident shape.  */
+int btpm_solve_set_flux_scale(btpm_solve *s, float scale);
 /* Runs one solve alone, checking convergence on the host after every
  * rotation (the reference's GPU control flow; the "plain" mode).            */
 int btpm_solve_run(btpm_solve *s);

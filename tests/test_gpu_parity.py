@@ -139,6 +139,44 @@ def test_crater_parity(ctx, rough80, ti):
     assert abs(info.crater_rotations - cr.rotations) <= 1
 
 
+def test_crater_sun_table_and_flux_scale(ctx, rough80):
+    """Direct crater sunlight evaluated on the GPU from Sun directions and the
+    tabulated sunlit fraction equals the host-expanded shadow/cosang inputs;
+    a solve's flux scale equals scaling the shape's illumination and solar
+    flux on the host (every absorbed flux is linear in the solar flux)."""
+    case, crater = rough80
+    p = th.params(650, case.rot_period, tacc=0.01)
+    cg, tab = crater["geom"], crater["table"]
+    sun_local = inputs.crater_sun_local(case.frames, case.sun)
+    common = dict(vf=crater["vf"], multiple=True, scatter_acc=1e-4, drf=crater["drf"])
+    by_arr = th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2,
+                      crater=th.CraterModel(shadow=crater["shadow"], cosang=crater["cosang"],
+                                            solar=crater["solar"], **common))
+    by_sun = th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2,
+                      crater=th.CraterModel(sun_local=sun_local, vis_table=tab.vis,
+                                            normals=cg.normals, solar=crater["solar"], **common))
+    ia, ib = by_arr.crater_illum(), by_sun.crater_illum()
+    rel = np.abs(ia - ib).max() / np.abs(ia).max()
+    print(f"[parity] crater illumination sun-table vs arrays: rel {rel:.2e}", flush=True)
+    assert rel < 1e-5
+    g = 0.37
+    scaled = th.Shape(ctx, p, case.illum * g, 0.1, csr=case.csr, scatter=2,
+                      crater=th.CraterModel(sun_local=sun_local, vis_table=tab.vis,
+                                            normals=cg.normals, solar=crater["solar"] * g,
+                                            **common))
+    a = th.Solve(ctx, by_sun, 300.0, with_craters=True, flux_scale=g).run()
+    b = th.Solve(ctx, scaled, 300.0, with_craters=True).run()
+    # the two runs round differently, so a facet or crater may freeze one
+    # rotation apart; frozen states are within TACC (0.01 K) of the next rotation
+    assert _report("flux scale smooth", a.surface_temps(), b.surface_temps()) <= 0.01
+    assert _report("flux scale crater", a.crater_temps(), b.crater_temps()) <= 0.01
+    assert abs(a.info().crater_rotations - b.info().crater_rotations) <= 1
+    with pytest.raises(_lib.BtpmError):
+        th.Solve(ctx, by_sun, 300.0, flux_scale=-1.0)
+    for x in (a, b, by_arr, by_sun, scaled):
+        x.close()
+
+
 def test_batch_equals_alone(ctx, rough80):
     """Batched (graph-captured, several solves resident) results are bitwise
     the results of each solve run alone."""
