@@ -147,23 +147,62 @@ def cpu_baseline(ctx, shape, case, crater, plan, solves, n_craters=16, check_ti_
     }
 
 
-def plain_reference(ctx, shape, plan, idx):
-    """The same solves run one at a time with a host convergence check per
-    rotation (no batching, no graphs): the plain single-solve GPU path."""
+def plain_stepwise(ctx, shape, plan, infos):
+    """The plain single-solve GPU kernel: the reference's GPU execution
+    structure (an FD and a boundary launch plus a host sync per time step,
+    profile in global memory, per-lane crater loop, host convergence check
+    per rotation).  Two complete solves at the middle TI (f = 0 and f > 0)
+    are timed; the sweep's plain time is extrapolated per phase with every
+    solve's own rotation counts from the batched run (same algorithm)."""
     from binarytpm_amd import thermal as th
-    t = []
-    for i in idx:
-        ti, f = plan[i]
-        s = th.Solve(ctx, shape, ti, with_craters=f > 0)
+    mid = plan[len(plan) // 2][0]
+    i0 = plan.index((mid, 0.0))
+    i1 = plan.index((mid, 0.5))
+    runs = {}
+    for key, with_c in (("smooth", False), ("crater", True)):
+        s = th.Solve(ctx, shape, mid, with_craters=with_c)
+        t0 = time.perf_counter()
+        s.run_stepwise()
+        runs[key] = (time.perf_counter() - t0, s.info())
+        s.close()
+    t_s, inf_s = runs["smooth"]
+    t_c = runs["crater"][0] - t_s
+    inf_c = runs["crater"][1]
+    rs = max(inf_s.rotations, 1)
+    rc = max(inf_c.crater_rotations, 1)
+    total = 0.0
+    for (ti, f), inf in zip(plan, infos):
+        total += t_s * inf.rotations / rs
+        if f > 0:
+            total += t_c * inf.crater_rotations / rc
+    log(f"stepwise plain path at TI {mid:.1f}: smooth {t_s:.2f} s ({inf_s.rotations} rot), "
+        f"crater {t_c:.2f} s ({inf_c.crater_rotations} rot); batched run had "
+        f"{infos[i0].rotations}/{infos[i1].crater_rotations} rotations there")
+    return {
+        "kind": "stepwise (reference GPU structure: 2 launches + host sync per step)",
+        "solves_per_hour": 3600.0 * len(plan) / total,
+        "sample": (f"complete solves at TI {mid:.1f}: smooth {t_s:.2f} s / {inf_s.rotations} "
+                   f"rotations, crater phase {t_c:.2f} s / {inf_c.crater_rotations} rotations; "
+                   "extrapolated per phase by each sweep solve's rotation counts"),
+        "sweep_s_extrapolated": total,
+    }
+
+
+def single_solve_persistent(ctx, shape, plan):
+    """The persistent kernels on one solve at a time (no batching, no graphs,
+    host check per rotation), at the middle TI, extrapolated as above."""
+    from binarytpm_amd import thermal as th
+    mid = plan[len(plan) // 2][0]
+    t = {}
+    for f in (0.0, 0.5):
+        s = th.Solve(ctx, shape, mid, with_craters=f > 0)
         t0 = time.perf_counter()
         s.run()
-        t.append((f > 0, time.perf_counter() - t0))
+        t[f] = time.perf_counter() - t0
         s.close()
-    ts = np.mean([x for c, x in t if not c]) if any(not c for c, _ in t) else 0.0
-    tc = np.mean([x for c, x in t if c]) if any(c for c, _ in t) else 0.0
     n0 = sum(1 for _, f in plan if f == 0)
-    total = n0 * ts + (len(plan) - n0) * tc
-    return 3600.0 * len(plan) / total, t
+    total = n0 * t[0.0] + (len(plan) - n0) * t[0.5]
+    return 3600.0 * len(plan) / total
 
 
 def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2):
@@ -362,12 +401,14 @@ def main():
             },
         }
         if not args.no_plain:
-            mid_ti = plan[len(plan) // 2][0]
-            idx = [i for i, x in enumerate(plan) if x[0] == mid_ti][:2]
-            v, t = plain_reference(ctx, shape, plan, idx)
-            line["plain_solves_per_hour"] = v
-            line["batched_over_plain"] = line["value"] / v
-            log(f"plain single-solve path: {t}")
+            pl = plain_stepwise(ctx, shape, plan, infos)
+            line["plain"] = pl
+            line["batched_over_plain"] = line["value"] / pl["solves_per_hour"]
+            v1 = single_solve_persistent(ctx, shape, plan)
+            line["single_solve_persistent_solves_per_hour"] = v1
+            line["batched_over_single_solve_persistent"] = line["value"] / v1
+            log(f"plain stepwise {pl['solves_per_hour']:.0f} solves/h, persistent single-solve "
+                f"{v1:.0f} solves/h")
         if not args.no_cpu:
             cb = cpu_baseline(ctx, shape, case, crater, plan, last)
             line["cpu_baseline"] = cb

@@ -101,6 +101,7 @@ PROTOTYPES = [
     ("btpm_solve_create", C.c_int, [_vp, _vp, C.c_float, _fp, C.c_int, C.POINTER(_vp)]),
     ("btpm_solve_destroy", C.c_int, [_vp]),
     ("btpm_solve_set_flux_scale", C.c_int, [_vp, C.c_float]),
+    ("btpm_solve_run_stepwise", C.c_int, [_vp, C.c_int]),
     ("btpm_solve_run", C.c_int, [_vp]),
     ("btpm_solve_get_info", C.c_int, [_vp, C.POINTER(SolveInfo)]),
     ("btpm_solve_surface_temps", C.c_int, [_vp, _fp]),

@@ -56,3 +56,13 @@ extern "C" __global__ void btpm_finish_scatter(float* F, const float* p, const f
                                                const float* albedo, int ncols, size_t n);
 extern "C" __global__ void btpm_status_kernel(const int* ids, const btpm_set_dev* sets,
                                               btpm_status_dev* status);
+
+// stepwise.hip
+extern "C" __global__ void btpm_step_profile(float* temp, float* prof, int ncols, int to_records);
+extern "C" __global__ void btpm_step_fd(btpm_set_dev S, float* temp);
+extern "C" __global__ void btpm_step_bc_smooth(btpm_set_dev S, float* temp, const float* Tp, int s,
+                                               const int* rowptr, const int* colidx,
+                                               const float* vf, float* dm);
+extern "C" __global__ void btpm_step_bc_crater(btpm_set_dev S, float* temp, const float* Tp, int s,
+                                               const float* vf32, float* dm);
+extern "C" __global__ void btpm_step_converged(btpm_set_dev S, const float* dm, int check);

@@ -28,6 +28,7 @@
 
 #include "btpm_internal.h"
 #include "btpm_kernels.h"
+#include "btpm_device.h"
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 // Pointers read from the set descriptor are generic; casting them to the
@@ -38,29 +39,6 @@ typedef GLOBAL_AS float gfloat;
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 namespace {
-
-// Newton root of the surface energy balance in FP64,
-//   g(T) = src + m1 (t1 - T) - es T^4 = 0,   T <- T + g / (m1 + 4 es T^3),
-// stopped when |dT| < tol.  g' is inverted with v_rcp_f64 plus one
-// Newton-Raphson refinement of the reciprocal (relative error ~1e-16, so the
-// iterates are the exact-division iterates to rounding); g is concave and
-// decreasing, so the iteration converges from any positive start.
-__device__ __forceinline__ double sbc_newton(double src, double m1, double t1, double es,
-                                             double x, double tol, int cap) {
-  const double es4 = 4.0 * es;
-  for (int it = 0; it < cap; ++it) {
-    const double x2 = x * x;
-    const double x3 = x2 * x;
-    const double g = fma(-es * x3, x, fma(m1, t1 - x, src));
-    const double gp = fma(es4, x3, m1);
-    double rg = __builtin_amdgcn_rcp(gp);
-    rg = fma(rg, fma(-gp, rg, 1.0), rg);
-    const double d = g * rg;
-    x += d;
-    if (fabs(d) < tol) break;
-  }
-  return x;
-}
 
 // FTCS in scaled form.  With c = 1 - 2r the step is u_i' = c u_i + r (u_{i-1}
 // + u_{i+1}); holding U = u / s with s = c^k after k steps turns it into
@@ -166,7 +144,7 @@ __device__ __forceinline__ double conduct_step(float (&u)[NX + 1], float kr, SC 
     t1 = fma(s, (double)u[1], Tref);
   else
     t1 = Tref + (double)(s * u[1]);
-  const double x = sbc_newton(src, m1, t1, es, fmax(x0, 1.0), tol, cap);
+  const double x = btpm_dev::sbc_newton(src, m1, t1, es, fmax(x0, 1.0), tol, cap);
   if constexpr (std::is_same<SC, double>::value)
     u[0] = (float)((x - Tref) * is);
   else

@@ -940,6 +940,95 @@ int btpm_solve_run(btpm_solve* s) {
   return btpm_batch_run(s->ctx, &s, 1, &o, nullptr);
 }
 
+namespace {
+// One set through the stepwise kernels until every group is frozen, the
+// rotation cap of the parameters, or `cap` rotations (cap > 0).
+void run_stepwise_set(btpm_solve* s, SetHost* S, int cap, int* rev_out, int* conv_out,
+                      long long* colsteps_out) {
+  btpm_shape* sh = s->shape;
+  hipStream_t st = s->ctx->stream;
+  const btpm_set_dev& d = S->d;
+  const bool crater = d.kind == BTPM_SET_CRATER;
+  const int nc = d.ncols, ng = d.ngroups;
+  DBuf temp, Tp, dm;
+  temp.alloc((size_t)(BTPM_NX + 1) * nc * 4);
+  Tp.alloc(S->T.n);
+  dm.alloc((size_t)nc * 4);
+  hipLaunchKernelGGL(btpm_step_profile, dim3(grid1(nc, 256)), dim3(256), 0, st, temp.as<float>(),
+                     S->prof.as<float>(), nc, 0);
+  std::vector<uint8_t> fr((size_t)ng);
+  HIPCHK(hipMemcpyAsync(fr.data(), S->frozen.p, ng, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const bool csr = !crater && sh->nnz > 0;
+  int rev = 0;
+  bool done = false;
+  long long cs = 0;
+  while (!done && rev < d.max_rev && (cap <= 0 || rev < cap)) {
+    // stemp0 <- stemp1 for the whole rotation, then the steps
+    HIPCHK(hipMemcpyAsync(Tp.p, S->T.p, S->T.n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemsetAsync(dm.p, 0, dm.n, st));
+    long long active = 0;
+    for (uint8_t x : fr) active += x ? 0 : 1;
+    for (int k = 0; k < d.ntau; ++k) {
+      hipLaunchKernelGGL(btpm_step_fd, dim3(grid1(nc, 256)), dim3(256), 0, st, d,
+                         temp.as<float>());
+      if (crater)
+        hipLaunchKernelGGL(btpm_step_bc_crater, dim3(ng), dim3(128), 0, st, d, temp.as<float>(),
+                           (const float*)Tp.as<float>(), k, sh->cvf32.as<float>(),
+                           dm.as<float>());
+      else
+        hipLaunchKernelGGL(btpm_step_bc_smooth, dim3(grid1(nc, 256)), dim3(256), 0, st, d,
+                           temp.as<float>(), (const float*)Tp.as<float>(), k,
+                           csr ? sh->rowptr.as<int>() : nullptr,
+                           csr ? sh->colidx.as<int>() : nullptr,
+                           csr ? sh->vf.as<float>() : nullptr, dm.as<float>());
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    cs += active * (crater ? BTPM_NS : 1) * d.ntau;
+    hipLaunchKernelGGL(btpm_step_converged, dim3(grid1(ng, 256)), dim3(256), 0, st, d,
+                       (const float*)dm.as<float>(), rev > d.check_after ? 1 : 0);
+    HIPCHK(hipMemcpyAsync(fr.data(), S->frozen.p, ng, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ++rev;
+    done = true;
+    for (uint8_t x : fr) done = done && x;
+  }
+  hipLaunchKernelGGL(btpm_step_profile, dim3(grid1(nc, 256)), dim3(256), 0, st, temp.as<float>(),
+                     S->prof.as<float>(), nc, 1);
+  HIPCHK(hipStreamSynchronize(st));
+  *rev_out = rev;
+  *conv_out = done ? 1 : 0;
+  *colsteps_out = cs;
+}
+}  // namespace
+
+int btpm_solve_run_stepwise(btpm_solve* s, int rotation_cap) {
+  if (!s) return fail(BTPM_ERR_ARG, "null solve");
+  if (s->phase != 0 || s->queued) return fail(BTPM_ERR_STATE, "solve not fresh");
+  btpm_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  s->t_submit = std::chrono::steady_clock::now();
+  build_smooth(s, st);
+  s->phase = 1;
+  run_stepwise_set(s, s->smooth.get(), rotation_cap, &s->info.rotations, &s->info.converged,
+                   &s->info.column_steps);
+  shrink_smooth(s->smooth.get());
+  if (s->with_craters) {
+    build_crater(s, st);
+    s->phase = 2;
+    run_stepwise_set(s, s->crater.get(), rotation_cap, &s->info.crater_rotations,
+                     &s->info.crater_converged, &s->info.crater_column_steps);
+    s->crater->src.release();
+  }
+  s->info.latency_s =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t_submit).count();
+  s->phase = 3;
+  API_END
+}
+
 int btpm_solve_get_info(btpm_solve* s, btpm_solve_info* info) {
   if (!s || !info) return fail(BTPM_ERR_ARG, "null argument");
   *info = s->info;

@@ -209,6 +209,12 @@ class Solve:
         check(lib().btpm_solve_run(self._h))
         return self
 
+    def run_stepwise(self, rotation_cap: int = 0):
+        """Run alone in the reference's GPU execution structure (two launches
+        and a host sync per time step); rotation_cap > 0 limits each phase."""
+        check(lib().btpm_solve_run_stepwise(self._h, int(rotation_cap)))
+        return self
+
     def info(self) -> SolveInfo:
         inf = SolveInfo()
         check(lib().btpm_solve_get_info(self._h, C.byref(inf)))

@@ -200,9 +200,19 @@ int btpm_solve_destroy(btpm_solve *s);
  * the solar constant into its illumination (T3905/main.cpp:905); it lets
  * requests at different heliocentric distances share one resident shape.  */
 int btpm_solve_set_flux_scale(btpm_solve *s, float scale);
-/* Runs one solve alone, checking convergence on the host after every
- * rotation (the reference's GPU control flow; the "plain" mode).            */
+/* Runs one solve alone through the persistent kernels, checking
+ * convergence on the host after every rotation (no batching, no graphs).   */
 int btpm_solve_run(btpm_solve *s);
+/* Runs one solve alone in the reference's GPU execution structure: one
+ * finite-difference and one boundary-condition launch per time step with a
+ * host synchronisation after each, the depth profile in global memory, the
+ * crater self-heating as a per-lane loop, and a convergence kernel whose
+ * flags the host reads every rotation (GPU_inertia_difference /
+ * GPU_converged, T175706/thermal_GPU.cu:430-692, 1021-1026).  Same
+ * algorithm and results to tolerance; the baseline the batched path is
+ * measured against.  rotation_cap > 0 stops each phase after that many
+ * rotations (timing samples); 0 runs to convergence or max_rev.            */
+int btpm_solve_run_stepwise(btpm_solve *s, int rotation_cap);
 int btpm_solve_get_info(btpm_solve *s, btpm_solve_info *info);
 /* final surface temperatures: smooth [nf][ntau] (writeglobaltemperature's
  * values), crater [nf][ntau][100] (obtain_GPU_roughness_temperatures'

@@ -177,6 +177,30 @@ def test_crater_sun_table_and_flux_scale(ctx, rough80):
         x.close()
 
 
+def test_stepwise_path_parity(ctx, rough80):
+    """The reference-structured path (an FD and a boundary launch plus a host
+    sync per step, profile in global memory, per-lane crater loop) reaches
+    the oracle's results and rotation counts."""
+    case, crater = rough80
+    p = th.params(650, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
+                        solar=crater["solar"], multiple=True, scatter_acc=1e-4,
+                        drf=crater["drf"])
+    shape = th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2, crater=cm)
+    s = th.Solve(ctx, shape, 200.0, with_craters=True).run_stepwise()
+    F, sm, il, cr = _oracle_rough(case, crater, 200.0)
+    info = s.info()
+    assert _report("stepwise smooth", s.surface_temps(), sm.stemp) <= TOL_K
+    assert _report("stepwise crater", s.crater_temps(), cr.stemp) <= TOL_K
+    assert info.converged and info.crater_converged
+    assert abs(info.rotations - sm.rotations) <= 1
+    assert abs(info.crater_rotations - cr.rotations) <= 1
+    capped = th.Solve(ctx, shape, 200.0, with_craters=True).run_stepwise(rotation_cap=2)
+    assert capped.info().rotations == 2 and capped.info().crater_rotations == 2
+    for x in (s, capped, shape):
+        x.close()
+
+
 def test_batch_equals_alone(ctx, rough80):
     """Batched (graph-captured, several solves resident) results are bitwise
     the results of each solve run alone."""
