@@ -313,6 +313,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (the known-size "
+                         "kernel that scales HBM traffic counters)")
     ap.add_argument("--serve-n", type=int, default=40, help="serving requests")
     ap.add_argument("--serve-rate", type=float, default=1.3, help="serving arrivals per second")
     args = ap.parse_args()
@@ -322,6 +325,16 @@ def main():
         sys.exit(2)
 
     from binarytpm_amd import thermal as th
+    if args.copy_mib > 0:
+        import torch
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
+        del src, dst
     case, crater = build_workload(args.subdiv, args.gamma)
     plan = sweep_plan(args.ti)
     with th.Context(0) as ctx:

@@ -491,13 +491,17 @@ extern "C" __global__ __launch_bounds__(256) void btpm_status_kernel(
     btpm_status_dev* __restrict__ status) {
   __shared__ long long red_w[256];
   __shared__ int red_a[256];
+  __shared__ int scan[256];
   const int sid = set_ids[blockIdx.x];
   const btpm_set_dev& S = sets[sid];
   btpm_status_dev* st = status + sid;
   if (st->done) return;
+  // the tiles this rotation could fill: every launch list covers them and
+  // each of their workgroups wrote its counts (later tiles may be stale)
+  const int ntl = min(S.ntiles, (st->norder + S.tile_cols - 1) / S.tile_cols);
   long long w = 0;
   int a = 0;
-  for (int t = threadIdx.x; t < S.ntiles; t += 256) {
+  for (int t = threadIdx.x; t < ntl; t += 256) {
     a += S.tile_active[t];
     w += S.tile_work[t];
   }
@@ -523,4 +527,24 @@ extern "C" __global__ __launch_bounds__(256) void btpm_status_kernel(
       st->done = 1;
     }
   }
+  // stable compaction of the groups still integrating into order[] for the
+  // next rotation (chunks of 256 with a block-wide inclusive scan)
+  const int tid = (int)threadIdx.x;
+  int base = 0;
+  for (int g0 = 0; g0 < S.ngroups; g0 += 256) {
+    const int g = g0 + tid;
+    const int live = (g < S.ngroups && !S.frozen[g]) ? 1 : 0;
+    __syncthreads();  // the previous chunk's scan[255] has been read
+    scan[tid] = live;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int v = tid >= off ? scan[tid - off] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    if (live) S.order[base + scan[tid] - 1] = g;
+    base += scan[255];
+  }
+  if (tid == 0) st->norder = base;
 }

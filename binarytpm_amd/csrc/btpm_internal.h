@@ -50,6 +50,8 @@ struct btpm_set_dev {
   const float* r;     // [ngroups] FTCS number dtau/dz^2
   const float* m1;    // [ngroups] Gamma sqrt(2 pi n / P) / dz
   uint8_t* frozen;    // [ngroups]
+  int* order;         // [ngroups] groups to integrate; the status kernel compacts
+                      // the unfrozen ones to the front after every rotation
   float* dmax;        // [ngroups] max |dT| of the last executed rotation
   const float* qext;  // [nb][ncols][BLK] facet self-heating flux (or null)
   float* emis;        // [ncols][nb*BLK] esia T^4 for the next rotation (or null),
@@ -69,6 +71,8 @@ struct btpm_status_dev {
   int done;           // 1 once every group froze or rev reached max_rev
   int converged;      // 1 if done by freezing (not by the cap)
   long long colsteps; // integrated column-steps (executed, excluding frozen)
+  int norder;         // leading entries of the set's order[] to integrate
+  int pad_;
 };
 
 // one workgroup's work item in a batched launch

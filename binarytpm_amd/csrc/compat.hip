@@ -135,7 +135,7 @@ struct Compat {
   // per facet
   std::vector<uint8_t> h_frozen;
   std::vector<float> h_albedo, h_ti, h_numrevs;
-  DBuf frozen, oma, albedo, r, m1, dmax;
+  DBuf frozen, order, oma, albedo, r, m1, dmax;
   // reference-layout inputs
   DBuf shadow, sfc_shadow, illumvec, illumangle, rgi, rgh;
   // blocked crater state
@@ -227,6 +227,7 @@ btpm_set_dev make_set(float es, float esia, float tol, int inst) {
   d.r = c.r.as<float>();
   d.m1 = c.m1.as<float>();
   d.frozen = c.frozen.as<uint8_t>();
+  d.order = c.order.as<int>();
   d.dmax = c.dmax.as<float>();
   d.vfA = reinterpret_cast<const float*>(c.vfA.p);
   d.tile_active = c.tile_active.as<int>();
@@ -356,7 +357,13 @@ void initialise_GPU_facet_properties_arrays(void) {
     c.tile_active.alloc((size_t)c.ntiles * 4);
     c.tile_work.alloc((size_t)c.ntiles * 4);
     c.d_set.alloc(sizeof(btpm_set_dev));
+    // the crater kernel integrates order[0..norder): here every facet, in
+    // order; frozen facets are skipped by their flag (GPU_converged sets it)
+    std::vector<int> id((size_t)c.nf);
+    for (int f = 0; f < c.nf; ++f) id[f] = f;
+    upload(c.order, id.data(), id.size(), c.ctx->stream);
     btpm_status_dev z{};
+    z.norder = c.nf;
     upload(c.d_status, &z, 1, c.ctx->stream);
     coefficients();
   });

@@ -201,9 +201,13 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
   const bool check = st.rev > S.check_after;
 
   const int ncols = S.ncols;
-  const int c = tl.tile * BTPM_SMOOTH_WG + (int)threadIdx.x;
-  const bool valid = c < ncols;
-  const int cc = valid ? c : ncols - 1;  // clamped index for loads
+  // lane -> column through the compacted order of unfrozen columns; lanes
+  // past its end take a valid column for their (unused) loads
+  const int idx = tl.tile * BTPM_SMOOTH_WG + (int)threadIdx.x;
+  const int norder = st.norder;
+  const bool valid = idx < norder;
+  const int c = S.order[valid ? idx : (norder > 0 ? norder - 1 : 0)];
+  const int cc = c;
   const bool active = valid && (S.inst || !S.frozen[cc]);
   const int nwork = __syncthreads_count(active);
   if (nwork == 0) {
@@ -413,11 +417,16 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   const int lc = tid < P * NS ? tid : P * NS - 1;  // local column (clamped)
   const int lcr = lc / NS;                         // local crater
   const int j = lc - lcr * NS;                     // sub-facet
-  const int g = tl.tile * P + lcr;                 // global crater
-  const bool valid = tid < P * NS && g < S.ngroups;
-  // idle lanes (tid >= 1000) mirror column 999, so they stay well-behaved
-  const int gg = g < S.ngroups ? g : 0;
-  const int c = gg * NS + j;                       // global column (clamped)
+  // local crater -> crater through the compacted order of unfrozen craters;
+  // slots past its end and the idle lanes (tid >= 1000) mirror a valid
+  // crater's column, so they stay well-behaved
+  const int norder = st.norder;
+  const int slot = tl.tile * P + lcr;
+  const bool valid = tid < P * NS && slot < norder;
+  const int last = norder > 0 ? norder - 1 : 0;
+  const int g = S.order[slot < norder ? slot : last];
+  const int gg = g;
+  const int c = gg * NS + j;                       // global column
   const bool active = valid && !S.frozen[gg];
   const int nwork = __syncthreads_count(active);
   if (nwork == 0) {
@@ -435,7 +444,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
       (&lds_b[0][0])[e] = (_Float16)0.0f;
     if (tid < P) {
       lds_cmax[tid] = 0u;
-      const int gt = tl.tile * P + tid < S.ngroups ? tl.tile * P + tid : 0;
+      const int gt = S.order[tl.tile * P + tid < norder ? tl.tile * P + tid : last];
       scale_table(S.r[gt], lds_cc[tid].sk, lds_cc[tid].pk);
       lds_cc[tid].r0[0] = 1.0;
       lds_cc[tid].m1 = (double)S.m1[gt];

@@ -85,9 +85,18 @@ namespace {
 struct SetHost {
   btpm_set_dev d{};
   int slot = -1;
-  DBuf src, T, prof, r, m1, frozen, dmax, qext, emis, tile_active, tile_work;
+  DBuf src, T, prof, r, m1, frozen, order, dmax, qext, emis, tile_active, tile_work;
   int ntiles = 0;
+  int listed = 0;  // tiles in the current launch lists
 };
+
+// order[] starts as the identity: every group integrates
+void alloc_order(SetHost* S, int ngroups, hipStream_t st) {
+  std::vector<int> id((size_t)ngroups);
+  for (int g = 0; g < ngroups; ++g) id[g] = g;
+  upload(S->order, id.data(), id.size(), st);
+  S->d.order = S->order.as<int>();
+}
 
 }  // namespace
 
@@ -159,6 +168,7 @@ void build_smooth(btpm_solve* s, hipStream_t st) {
   upload(S->r, r.data(), r.size(), st);
   upload(S->m1, m1.data(), m1.size(), st);
   S->frozen.alloc(d.ncols);
+  alloc_order(S.get(), d.ncols, st);
   S->dmax.alloc(d.ncols * sizeof(float));
   if (sh->nnz > 0) {
     S->qext.alloc(nser * sizeof(float));
@@ -207,6 +217,7 @@ void build_crater(btpm_solve* s, hipStream_t st) {
   upload(S->r, r.data(), r.size(), st);
   upload(S->m1, m1.data(), m1.size(), st);
   S->frozen.alloc(d.ngroups);
+  alloc_order(S.get(), d.ngroups, st);
   S->dmax.alloc(d.ngroups * sizeof(float));
   S->ntiles = (d.ngroups + BTPM_CRATERS_WG - 1) / BTPM_CRATERS_WG;
   S->tile_active.alloc(S->ntiles * sizeof(int));
@@ -344,7 +355,9 @@ struct Sched {
 
   void add_set(btpm_solve* s, SetHost* S) {
     S->slot = take_slot();
-    static const btpm_status_dev z{};
+    btpm_status_dev z{};
+    z.norder = S->d.ngroups;
+    ctx->h_status[S->slot] = z;
     HIPCHK(hipMemcpyAsync(ctx->d_sets + S->slot, &S->d, sizeof(btpm_set_dev),
                           hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_status + S->slot, &z, sizeof z, hipMemcpyHostToDevice,
@@ -385,6 +398,11 @@ struct Sched {
     }
   }
 
+  int needed_tiles(const SetHost* S) const {
+    const int n = ctx->h_status[S->slot].norder;
+    return std::min(S->ntiles, (n + S->d.tile_cols - 1) / S->d.tile_cols);
+  }
+
   void rebuild_lists() {
     std::vector<btpm_tile> ts, tc;
     std::vector<int> ids;
@@ -392,7 +410,9 @@ struct Sched {
       SetHost* S = kv.second.second;
       ids.push_back(S->slot);
       auto& v = S->d.kind == BTPM_SET_SMOOTH ? ts : tc;
-      for (int t = 0; t < S->ntiles; ++t) v.push_back(btpm_tile{S->slot, t});
+      // only the tiles the compacted order[] can fill (it only shrinks)
+      S->listed = needed_tiles(S);
+      for (int t = 0; t < S->listed; ++t) v.push_back(btpm_tile{S->slot, t});
     }
     n_tiles_s = (int)ts.size();
     n_tiles_c = (int)tc.size();
@@ -588,6 +608,12 @@ struct Sched {
       if (alive) keep.push_back(s);
     }
     active.swap(keep);
+    // re-tile (and re-capture) once compaction has emptied a quarter of a
+    // set's listed tiles
+    for (auto& kv : slots) {
+      const SetHost* S = kv.second.second;
+      if (4 * needed_tiles(S) < 3 * S->listed) dirty = true;
+    }
   }
 
   void step() {
