@@ -27,8 +27,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "conduct_crater_traffic.md")
 NX = 56
 BYTES_PER_COLSTEP = 12.0   # read absorbed flux + previous-rotation T, write T
+
+
+def measured_traffic_per_colstep():
+    """HBM bytes per integrated crater column-step from the committed
+    FETCH_SIZE/WRITE_SIZE counter summary (scaled by a copy of known size),
+    or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for ln in fh:
+                if ln.startswith("traffic_bytes_per_column_step:"):
+                    return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -373,6 +388,8 @@ def main():
         sk_cs = sum(s.smooth_column_steps for s in stats)
         lat = np.array([i.latency_s for i in infos])
         achieved = BYTES_PER_COLSTEP * ck_cs / ck_s / 1e9 if ck_s > 0 else 0.0
+        tpc = measured_traffic_per_colstep()
+        traffic = tpc * ck_cs / ck_n if (tpc is not None and ck_n > 0) else None
         line = {
             "metric": "converged solves/hour (128-solve TI x roughness sweep)",
             "value": 3600.0 * conv / T,
@@ -405,7 +422,10 @@ def main():
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None,
+                "traffic": traffic,
+                "traffic_source": ("profiles/conduct_crater_traffic.md: FETCH_SIZE + WRITE_SIZE "
+                                   "per integrated column-step, copy-calibrated, times this "
+                                   "run's column-steps per launch") if traffic else None,
                 "avg_launch_ms": 1000.0 * ck_s / max(ck_n, 1),
                 "launches": ck_n,
                 "bytes_per_column_step": BYTES_PER_COLSTEP,
