@@ -78,6 +78,7 @@ class BatchStats(C.Structure):
         ("smooth_kernel_s", C.c_double), ("crater_kernel_s", C.c_double),
         ("smooth_launches", C.c_int), ("crater_launches", C.c_int),
         ("smooth_column_steps", C.c_longlong), ("crater_column_steps", C.c_longlong),
+        ("max_resident", C.c_int),
     ]
 
 

@@ -30,6 +30,92 @@
 using namespace btpm;
 
 namespace btpm {
+namespace {
+struct Pool {
+  std::mutex mu;
+  std::map<int, std::multimap<size_t, void*>> cache;  // device -> capacity -> block
+  std::map<void*, std::pair<int, size_t>> live;       // block -> (device, capacity)
+  std::map<int, size_t> cached;
+};
+Pool& the_pool() {
+  static Pool* p = new Pool;  // never destroyed: DBufs of static objects may outlive main
+  return *p;
+}
+}  // namespace
+
+void* pool_alloc(size_t bytes) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  const size_t want = (bytes + 255) & ~(size_t)255;
+  Pool& P = the_pool();
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto& c = P.cache[dev];
+    auto it = c.lower_bound(want);
+    if (it != c.end() && it->first <= want + want / 4) {
+      void* p = it->second;
+      const size_t cap = it->first;
+      c.erase(it);
+      P.cached[dev] -= cap;
+      P.live[p] = {dev, cap};
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    pool_trim(dev);
+    e = hipMalloc(&p, want);
+  }
+  if (e != hipSuccess) {
+    const int code = e == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP;
+    fail(code, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    throw HipError{code};
+  }
+  std::lock_guard<std::mutex> lk(P.mu);
+  P.live[p] = {dev, want};
+  return p;
+}
+
+void pool_free(void* p) {
+  Pool& P = the_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.live.find(p);
+  if (it == P.live.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  const int dev = it->second.first;
+  const size_t cap = it->second.second;
+  P.live.erase(it);
+  P.cache[dev].emplace(cap, p);
+  P.cached[dev] += cap;
+}
+
+size_t pool_cached(int device) {
+  Pool& P = the_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.cached.find(device);
+  return it == P.cached.end() ? 0 : it->second;
+}
+
+void pool_trim(int device) {
+  Pool& P = the_pool();
+  std::vector<void*> blocks;
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    for (auto& kv : P.cache[device]) blocks.push_back(kv.second);
+    P.cache[device].clear();
+    P.cached[device] = 0;
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(device);
+  for (void* b : blocks) (void)hipFree(b);
+  (void)hipSetDevice(cur);
+}
+
 std::string& last_error() {
   thread_local std::string e;
   return e;
@@ -308,9 +394,16 @@ struct Sched {
   int craters_active = 0;
   std::chrono::steady_clock::time_point t0;
 
-  explicit Sched(btpm_ctx* c, const btpm_batch_opts& o) : ctx(c), opt(o) {
+  // device memory free for new sets: the driver's free memory plus the
+  // library's cached blocks
+  size_t free_bytes() const {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
+    return fr + pool_cached(ctx->device);
+  }
+
+  explicit Sched(btpm_ctx* c, const btpm_batch_opts& o) : ctx(c), opt(o) {
+    const size_t fr = free_bytes();
     const double frac = opt.mem_fraction > 0 ? std::min((double)opt.mem_fraction, 1.0) : 0.8;
     headroom = (size_t)((double)fr * (1.0 - frac));
     t0 = std::chrono::steady_clock::now();
@@ -380,8 +473,7 @@ struct Sched {
       return false;
     if (active.empty()) return true;  // always let one in
     const size_t need = smooth_bytes(s->shape) + (s->with_craters ? crater_bytes(s->shape) : 0);
-    size_t fr = 0, tot = 0;
-    HIPCHK(hipMemGetInfo(&fr, &tot));
+    const size_t fr = free_bytes();
     return fr > headroom + reserved && need <= fr - headroom - reserved;
   }
 
@@ -396,6 +488,7 @@ struct Sched {
       add_set(s, s->smooth.get());
       active.push_back(s);
     }
+    stats.max_resident = std::max(stats.max_resident, (int)active.size());
   }
 
   int needed_tiles(const SetHost* S) const {
@@ -680,6 +773,7 @@ int btpm_ctx_destroy(btpm_ctx* c) {
   if (!c) return BTPM_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  pool_trim(c->device);  // cached blocks are idle: hand them back to the driver
   (void)hipFree(c->d_sets);
   (void)hipFree(c->d_status);
   (void)hipHostFree(c->h_status);
@@ -693,6 +787,7 @@ int btpm_ctx_mem_info(btpm_ctx* c, size_t* fr, size_t* tot) {
   API_BEGIN
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemGetInfo(fr, tot));
+  *fr += pool_cached(c->device);
   API_END
 }
 

@@ -58,6 +58,17 @@ inline int nblocks(int ntau) { return (ntau + BTPM_BLK - 1) / BTPM_BLK; }
 inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 // device buffer owned by RAII
+// Device memory cache.  Blocks released by DBuf stay cached per device and
+// are handed out again best-fit (capacity within 1.25x of the request):
+// returning tens of GB to the driver and allocating them again can stall a
+// single hipMalloc for seconds.  A failed hipMalloc empties the device's
+// cache and retries once.  Every release in the library happens after the
+// owning stream has been synchronised, so a cached block is idle.
+void* pool_alloc(size_t bytes);   // throws HipError
+void pool_free(void* p);
+size_t pool_cached(int device);   // bytes held in the cache
+void pool_trim(int device);       // return the device's cache to the driver
+
 struct DBuf {
   void* p = nullptr;
   size_t n = 0;
@@ -66,14 +77,14 @@ struct DBuf {
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) pool_free(p);
     p = nullptr;
     n = 0;
   }
   void alloc(size_t bytes) {
     release();
     if (bytes == 0) return;
-    HIPCHK(hipMalloc(&p, bytes));
+    p = pool_alloc(bytes);
     n = bytes;
   }
   template <class T>

@@ -155,6 +155,7 @@ typedef struct {
     int crater_launches;
     long long smooth_column_steps;
     long long crater_column_steps;
+    int max_resident;            /* most solves resident at once           */
 } btpm_batch_stats;
 
 /* ---- library ------------------------------------------------------------ */
@@ -167,6 +168,9 @@ void btpm_batch_opts_default(btpm_batch_opts *o);
  * device, each with its own stream.                                         */
 int btpm_ctx_create(int device, btpm_ctx **out);
 int btpm_ctx_destroy(btpm_ctx *ctx);
+/* free_bytes counts the driver's free memory plus the device blocks the
+ * library keeps cached for reuse (released buffers are not handed back to
+ * the driver until the context is destroyed or an allocation fails).       */
 int btpm_ctx_mem_info(btpm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 
 /* ---- shapes (inputs shared by many solves) ---------------------------------
