@@ -395,48 +395,83 @@ extern "C" __global__ __launch_bounds__(256) void btpm_spmm_blocked(
   }
 }
 
-// Facet self-heating flux of the next rotation from the previous rotation's
-// emission, written as the total absorbed flux the conduction kernel reads:
-//   y[b][f][k] = src[b][f][k] + sum_{(f,g)} vf * emis_t[g][8b+k]
-// emis_t is facet-major ([g][npad], npad = nb*BLK) so a wavefront whose lanes
-// are 64 consecutive steps reads one coalesced 256-byte run per view factor;
-// the CSR row is uniform across the wavefront (scalar loads).  blockIdx.x
-// runs over rows fastest, so at any time every XCD works on the same 64-step
-// chunk, whose emission slice (nrows x 256 B) stays in its L2.
-extern "C" __global__ __launch_bounds__(256) void btpm_selfheat_spmm(
-    const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ vals,
-    const float* __restrict__ emis_t, const float* __restrict__ src, float* __restrict__ y,
-    int nrows, int nb, const btpm_status_dev* __restrict__ st,
-    const uint8_t* __restrict__ frozen) {
-  if (st && st->done) return;
-  const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int npad = nb * BTPM_BLK;
-  const int s = blockIdx.y * 64 + (threadIdx.x & 63);
-  if (f >= nrows) return;
-  if (frozen && frozen[f]) return;  // a frozen facet is not integrated again
-  const int sc = s < npad ? s : npad - 1;
-  const int k0 = rowptr[f], k1 = rowptr[f + 1];
-  // eight independent gathers in flight per lane
-  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  int kk = k0;
-  for (; kk + 8 <= k1; kk += 8) {
-    int g[8];
-    float v[8], e[8];
+// Facet self-heating of the next rotation for every smooth set of one shape
+// in one launch (grid z = set):
+//   qext[b][f][k] = src[b][f][k] + sum_g VF_fg emis[g][8b + k]
+// A workgroup owns 256 receiving facets (lane = facet) x 64 steps of one set
+// and walks the emitters in chunks of 256: each chunk's emission rows are
+// staged in LDS once for all 256 receivers (row stride 68 floats keeps the
+// lanes' 16-byte reads of different rows on different banks), and each lane
+// accumulates its 64 steps in registers from its CSR entries that fall in
+// the chunk (entries sorted by emitter; cptr[c][f] = first entry of row f in
+// chunk c).  The staged emission is reused by every receiver of the block,
+// instead of being gathered from L2 once per view factor.
+namespace {
+constexpr int kSHRows = 256;   // receivers per workgroup (one per lane)
+constexpr int kSHChunk = 256;  // emitters per staged chunk
+constexpr int kSHSteps = 64;   // steps per workgroup
+constexpr int kSHStride = 68;  // LDS row stride in floats
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
+    const int* __restrict__ cptr, const int* __restrict__ colidx, const float* __restrict__ vals,
+    int nf, int nchunk, const int* __restrict__ slots, const btpm_set_dev* __restrict__ sets,
+    const btpm_status_dev* __restrict__ status) {
+  __shared__ __align__(16) float E[kSHChunk * kSHStride];
+  const int slot = slots[blockIdx.z];
+  if (status[slot].done) return;
+  const btpm_set_dev& S = sets[slot];
+  const int npad = S.nb * BTPM_BLK;
+  const int s0 = blockIdx.y * kSHSteps;
+  if (s0 >= npad) return;
+  const int tid = (int)threadIdx.x;
+  const int f = blockIdx.x * kSHRows + tid;
+  const bool live = f < nf && (S.inst || !S.frozen[f]);
+  const float* __restrict__ emis = S.emis;
+  float acc[kSHSteps];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      g[q] = colidx[kk + q];
-      v[q] = vals[kk + q];
+  for (int q = 0; q < kSHSteps; ++q) acc[q] = 0.0f;
+  for (int c = 0; c < nchunk; ++c) {
+    const int g0 = c * kSHChunk;
+    __syncthreads();  // the previous chunk has been consumed
+    for (int e = tid; e < kSHChunk * (kSHSteps / 4); e += kSHRows) {
+      const int gl = e / (kSHSteps / 4), q4 = 4 * (e % (kSHSteps / 4));
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (g0 + gl < nf && s0 + q4 < npad)
+        v = *reinterpret_cast<const float4*>(emis + (size_t)(g0 + gl) * npad + s0 + q4);
+      *reinterpret_cast<float4*>(&E[gl * kSHStride + q4]) = v;
     }
+    __syncthreads();
+    if (live) {
+      const int k0 = cptr[(size_t)c * nf + f], k1 = cptr[(size_t)(c + 1) * nf + f];
+      for (int k = k0; k < k1; ++k) {
+        const float v = vals[k];
+        const float* er = &E[(colidx[k] - g0) * kSHStride];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) e[q] = emis_t[(size_t)g[q] * npad + sc];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = fmaf(v[q], e[q], acc[q]);
+        for (int q = 0; q < kSHSteps; q += 4) {
+          const float4 e4 = *reinterpret_cast<const float4*>(er + q);
+          acc[q] = fmaf(v, e4.x, acc[q]);
+          acc[q + 1] = fmaf(v, e4.y, acc[q + 1]);
+          acc[q + 2] = fmaf(v, e4.z, acc[q + 2]);
+          acc[q + 3] = fmaf(v, e4.w, acc[q + 3]);
+        }
+      }
+    }
   }
-  for (; kk < k1; ++kk) acc[0] = fmaf(vals[kk], emis_t[(size_t)colidx[kk] * npad + sc], acc[0]);
-  const float tot = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-  if (s < npad) {
-    const size_t o = ((size_t)(s / BTPM_BLK) * nrows + f) * BTPM_BLK + (s % BTPM_BLK);
-    y[o] = src[o] + tot;
+  if (!live) return;
+  const float* __restrict__ src = S.src;
+  float* __restrict__ y = S.qext;
+#pragma unroll
+  for (int bq = 0; bq < kSHSteps / BTPM_BLK; ++bq) {
+    if (s0 + bq * BTPM_BLK >= npad) break;
+    const size_t o = ((size_t)(s0 / BTPM_BLK + bq) * nf + f) * BTPM_BLK;
+    const float4 a = *reinterpret_cast<const float4*>(src + o);
+    const float4 b = *reinterpret_cast<const float4*>(src + o + 4);
+    const int q = bq * BTPM_BLK;
+    *reinterpret_cast<float4*>(y + o) =
+        make_float4(a.x + acc[q], a.y + acc[q + 1], a.z + acc[q + 2], a.w + acc[q + 3]);
+    *reinterpret_cast<float4*>(y + o + 4) =
+        make_float4(b.x + acc[q + 4], b.y + acc[q + 5], b.z + acc[q + 6], b.w + acc[q + 7]);
   }
 }
 

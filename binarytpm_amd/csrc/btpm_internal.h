@@ -53,7 +53,7 @@ struct btpm_set_dev {
   int* order;         // [ngroups] groups to integrate; the status kernel compacts
                       // the unfrozen ones to the front after every rotation
   float* dmax;        // [ngroups] max |dT| of the last executed rotation
-  const float* qext;  // [nb][ncols][BLK] facet self-heating flux (or null)
+  float* qext;        // [nb][ncols][BLK] total flux incl. facet self-heating (or null)
   float* emis;        // [ncols][nb*BLK] esia T^4 for the next rotation (or null),
                       // facet-major for the self-heating SpMM's coalesced reads
   const float* vfA;   // crater: fp16 hi/lo view-factor tiles (see crater kernel)

@@ -158,6 +158,10 @@ struct btpm_shape {
   DBuf one_minus_a;
   DBuf numrevs;    // [nf]
   DBuf rowptr, colidx, vf;
+  // self-heating SpMM layout: rows' entries sorted by emitter, and per
+  // emitter chunk of 256 the first entry of every row, cptr[c][f]
+  DBuf scol, sval, cptr;
+  int nchunk = 0;
   DBuf cvf32;      // [100][100] crater VF (fp32, vf[i*100+j] = VF(j->i))
   DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
   DBuf cillum;     // [nb][nf*100][BLK] crater illumination (after scattering)
@@ -383,6 +387,14 @@ struct Sched {
   hipGraphExec_t phase_exec[4] = {nullptr, nullptr, nullptr, nullptr};
   hipGraph_t phase_graph[4] = {nullptr, nullptr, nullptr, nullptr};
   DBuf d_tiles_s, d_tiles_c, d_ids;
+  // smooth sets with facet self-heating, grouped by shape: one tiled SpMM
+  // launch per group per rotation
+  struct HeatGroup {
+    btpm_shape* shape;
+    int nsets;
+    std::unique_ptr<DBuf> slots;
+  };
+  std::vector<HeatGroup> heat;
   int n_tiles_s = 0, n_tiles_c = 0, n_ids = 0;
   btpm_batch_stats stats{};
   // launch timing (opts.profile): events around each conduction launch
@@ -510,6 +522,18 @@ struct Sched {
     n_tiles_s = (int)ts.size();
     n_tiles_c = (int)tc.size();
     n_ids = (int)ids.size();
+    std::map<btpm_shape*, std::vector<int>> by_shape;
+    for (auto& kv : slots) {
+      SetHost* S = kv.second.second;
+      if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext)
+        by_shape[kv.second.first->shape].push_back(S->slot);
+    }
+    heat.clear();
+    for (auto& kv : by_shape) {
+      HeatGroup g{kv.first, (int)kv.second.size(), std::make_unique<DBuf>()};
+      upload(*g.slots, kv.second.data(), kv.second.size(), ctx->stream);
+      heat.push_back(std::move(g));
+    }
     upload(d_tiles_s, ts.data(), ts.size(), ctx->stream);
     upload(d_tiles_c, tc.data(), tc.size(), ctx->stream);
     upload(d_ids, ids.data(), ids.size(), ctx->stream);
@@ -519,19 +543,14 @@ struct Sched {
   // SpMMs, smooth conduction, crater conduction, status
   void enqueue_phase(hipStream_t st, int ph) {
     if (ph == 0) {
-      for (auto& kv : slots) {
-        SetHost* S = kv.second.second;
-        btpm_solve* s = kv.second.first;
-        if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext) {
-          btpm_shape* sh = s->shape;
-          // qext = src + VF * emis: the total absorbed flux of this rotation
-          dim3 g(grid1(sh->nf, 4), grid1((size_t)S->d.nb * BTPM_BLK, 64));
-          hipLaunchKernelGGL(btpm_selfheat_spmm, g, dim3(256), 0, st, sh->rowptr.as<int>(),
-                             sh->colidx.as<int>(), sh->vf.as<float>(), S->emis.as<float>(),
-                             (const float*)S->src.as<float>(), S->qext.as<float>(), sh->nf,
-                             S->d.nb, ctx->d_status + S->slot,
-                             S->d.inst ? (const uint8_t*)nullptr : S->frozen.as<uint8_t>());
-        }
+      // qext = src + VF * emis: the total absorbed flux of this rotation, for
+      // every smooth set of a shape in one launch
+      for (const HeatGroup& g : heat) {
+        btpm_shape* sh = g.shape;
+        dim3 grid(grid1(sh->nf, 256), grid1((size_t)sh->nb * BTPM_BLK, 64), g.nsets);
+        hipLaunchKernelGGL(btpm_selfheat_tiled, grid, dim3(256), 0, st, sh->cptr.as<int>(),
+                           sh->scol.as<int>(), sh->sval.as<float>(), sh->nf, sh->nchunk,
+                           g.slots->as<int>(), ctx->d_sets, ctx->d_status);
       }
     } else if (ph == 1) {
       if (n_tiles_s)
@@ -842,6 +861,35 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     upload(sh->rowptr, body->rowptr, nf + 1, st);
     upload(sh->colidx, body->colidx, body->nnz, st);
     upload(sh->vf, body->vf, body->nnz, st);
+    {
+      constexpr int kChunk = 256;  // emitter chunk of btpm_selfheat_tiled
+      const int nc = (nf + kChunk - 1) / kChunk;
+      std::vector<int> sc((size_t)body->nnz), cp((size_t)(nc + 1) * nf);
+      std::vector<float> sv((size_t)body->nnz);
+      std::vector<std::pair<int, float>> row;
+      for (int f = 0; f < nf; ++f) {
+        const int k0 = body->rowptr[f], k1 = body->rowptr[f + 1];
+        row.clear();
+        for (int k = k0; k < k1; ++k) row.emplace_back(body->colidx[k], body->vf[k]);
+        std::stable_sort(row.begin(), row.end(),
+                         [](const std::pair<int, float>& a, const std::pair<int, float>& b) {
+                           return a.first < b.first;
+                         });
+        int k = k0;
+        for (int c = 0; c <= nc; ++c) {
+          while (k < k1 && row[k - k0].first < c * kChunk) ++k;
+          cp[(size_t)c * nf + f] = c == nc ? k1 : k;
+        }
+        for (int i = 0; i < k1 - k0; ++i) {
+          sc[k0 + i] = row[i].first;
+          sv[k0 + i] = row[i].second;
+        }
+      }
+      sh->nchunk = nc;
+      upload(sh->scol, sc.data(), sc.size(), st);
+      upload(sh->sval, sv.data(), sv.size(), st);
+      upload(sh->cptr, cp.data(), cp.size(), st);
+    }
   }
   // facet illumination -> blocked
   const size_t nser = (size_t)nb * nf * BTPM_BLK;
