@@ -403,19 +403,19 @@ extern "C" __global__ __launch_bounds__(256) void btpm_spmm_blocked(
 // staged in LDS once for all 256 receivers (row stride 68 floats keeps the
 // lanes' 16-byte reads of different rows on different banks), and each lane
 // accumulates its 64 steps in registers from its CSR entries that fall in
-// the chunk (entries sorted by emitter; cptr[c][f] = first entry of row f in
-// chunk c).  The staged emission is reused by every receiver of the block,
+// the chunk (entries sorted by emitter, packed as (emitter, view factor)
+// pairs; cptr[c][f] = first entry of row f in chunk c).  The staged emission is reused by every receiver of the block,
 // instead of being gathered from L2 once per view factor.
 namespace {
-constexpr int kSHRows = 256;   // receivers per workgroup (one per lane)
-constexpr int kSHChunk = 256;  // emitters per staged chunk
-constexpr int kSHSteps = 64;   // steps per workgroup
-constexpr int kSHStride = 68;  // LDS row stride in floats
+constexpr int kSHRows = BTPM_SH_ROWS;    // receivers per workgroup (one per lane)
+constexpr int kSHChunk = BTPM_SH_CHUNK;  // emitters per staged chunk
+constexpr int kSHSteps = BTPM_SH_STEPS;  // steps per workgroup
+constexpr int kSHStride = kSHSteps + 4;  // LDS row stride in floats
 }  // namespace
 
 extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
-    const int* __restrict__ cptr, const int* __restrict__ colidx, const float* __restrict__ vals,
-    int nf, int nchunk, const int* __restrict__ slots, const btpm_set_dev* __restrict__ sets,
+    const int* __restrict__ cptr, const int2* __restrict__ ent, int nf, int nchunk,
+    const int* __restrict__ slots, const btpm_set_dev* __restrict__ sets,
     const btpm_status_dev* __restrict__ status) {
   __shared__ __align__(16) float E[kSHChunk * kSHStride];
   const int slot = slots[blockIdx.z];
@@ -445,8 +445,9 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
     if (live) {
       const int k0 = cptr[(size_t)c * nf + f], k1 = cptr[(size_t)(c + 1) * nf + f];
       for (int k = k0; k < k1; ++k) {
-        const float v = vals[k];
-        const float* er = &E[(colidx[k] - g0) * kSHStride];
+        const int2 gv = ent[k];
+        const float v = __int_as_float(gv.y);
+        const float* er = &E[(gv.x - g0) * kSHStride];
 #pragma unroll
         for (int q = 0; q < kSHSteps; q += 4) {
           const float4 e4 = *reinterpret_cast<const float4*>(er + q);

@@ -18,6 +18,11 @@
 #define BTPM_CRATERS_WG 10  // craters per crater-conduction workgroup (1000 columns)
 #define BTPM_CRATER_THREADS 1024
 #define BTPM_PSTRIDE 64     // floats per column in the profile array (57 used)
+// facet self-heating SpMM tiles: receivers per workgroup, emitters per
+// staged chunk, steps per workgroup
+#define BTPM_SH_ROWS 256
+#define BTPM_SH_CHUNK 256
+#define BTPM_SH_STEPS 64
 // largest accepted r = dtau/dz^2 (FTCS is stable for r <= 1/2): the
 // conduction kernels integrate the depth profile in a variable scaled by
 // (1 - 2r)^-k over 8-step blocks, which must stay inside the FP32 range

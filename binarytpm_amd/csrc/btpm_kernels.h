@@ -45,9 +45,9 @@ extern "C" __global__ void btpm_spmm_blocked(const int* rowptr, const int* colid
                                              const float* vals, const float* x, const float* x0,
                                              const float* alpha, float* y, int nrows, int nb,
                                              int bpt, const btpm_status_dev* st);
-extern "C" __global__ void btpm_selfheat_tiled(const int* cptr, const int* colidx,
-                                               const float* vals, int nf, int nchunk,
-                                               const int* slots, const btpm_set_dev* sets,
+extern "C" __global__ void btpm_selfheat_tiled(const int* cptr, const int2* ent, int nf,
+                                               int nchunk, const int* slots,
+                                               const btpm_set_dev* sets,
                                                const btpm_status_dev* status);
 extern "C" __global__ void btpm_scale_cols(float* y, const float* x, const float* scale,
                                            float gain, int ncols, size_t n);

@@ -159,8 +159,8 @@ struct btpm_shape {
   DBuf numrevs;    // [nf]
   DBuf rowptr, colidx, vf;
   // self-heating SpMM layout: rows' entries sorted by emitter, and per
-  // emitter chunk of 256 the first entry of every row, cptr[c][f]
-  DBuf scol, sval, cptr;
+  // emitter chunk of BTPM_SH_CHUNK the first entry of every row, cptr[c][f]
+  DBuf sent, cptr;  // sent: (emitter, float bits of VF) pairs
   int nchunk = 0;
   DBuf cvf32;      // [100][100] crater VF (fp32, vf[i*100+j] = VF(j->i))
   DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
@@ -547,9 +547,10 @@ struct Sched {
       // every smooth set of a shape in one launch
       for (const HeatGroup& g : heat) {
         btpm_shape* sh = g.shape;
-        dim3 grid(grid1(sh->nf, 256), grid1((size_t)sh->nb * BTPM_BLK, 64), g.nsets);
-        hipLaunchKernelGGL(btpm_selfheat_tiled, grid, dim3(256), 0, st, sh->cptr.as<int>(),
-                           sh->scol.as<int>(), sh->sval.as<float>(), sh->nf, sh->nchunk,
+        dim3 grid(grid1(sh->nf, BTPM_SH_ROWS), grid1((size_t)sh->nb * BTPM_BLK, BTPM_SH_STEPS),
+                  g.nsets);
+        hipLaunchKernelGGL(btpm_selfheat_tiled, grid, dim3(BTPM_SH_ROWS), 0, st, sh->cptr.as<int>(),
+                           sh->sent.as<int2>(), sh->nf, sh->nchunk,
                            g.slots->as<int>(), ctx->d_sets, ctx->d_status);
       }
     } else if (ph == 1) {
@@ -862,10 +863,9 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     upload(sh->colidx, body->colidx, body->nnz, st);
     upload(sh->vf, body->vf, body->nnz, st);
     {
-      constexpr int kChunk = 256;  // emitter chunk of btpm_selfheat_tiled
+      constexpr int kChunk = BTPM_SH_CHUNK;  // emitter chunk of btpm_selfheat_tiled
       const int nc = (nf + kChunk - 1) / kChunk;
-      std::vector<int> sc((size_t)body->nnz), cp((size_t)(nc + 1) * nf);
-      std::vector<float> sv((size_t)body->nnz);
+      std::vector<int> se(2 * (size_t)body->nnz), cp((size_t)(nc + 1) * nf);
       std::vector<std::pair<int, float>> row;
       for (int f = 0; f < nf; ++f) {
         const int k0 = body->rowptr[f], k1 = body->rowptr[f + 1];
@@ -881,13 +881,12 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
           cp[(size_t)c * nf + f] = c == nc ? k1 : k;
         }
         for (int i = 0; i < k1 - k0; ++i) {
-          sc[k0 + i] = row[i].first;
-          sv[k0 + i] = row[i].second;
+          se[2 * (size_t)(k0 + i)] = row[i].first;
+          std::memcpy(&se[2 * (size_t)(k0 + i) + 1], &row[i].second, 4);
         }
       }
       sh->nchunk = nc;
-      upload(sh->scol, sc.data(), sc.size(), st);
-      upload(sh->sval, sv.data(), sv.size(), st);
+      upload(sh->sent, se.data(), se.size(), st);
       upload(sh->cptr, cp.data(), cp.size(), st);
     }
   }
