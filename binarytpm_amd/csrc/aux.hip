@@ -155,7 +155,16 @@ extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev 
 //   RGI: w = A_g F_g(s);  RGH: w = es T_g(s)^4 (mode 1 computes it from T).
 // The result is written blocked over crater columns c = f*NS+j as
 //   dst[b][c][k] = base_gain * base_scale_f * base[b][c][k] + out   (base may be null).
-// Workgroup: 256 lanes = 4 waves; wave w owns j in [25w, 25w+25), lane = s.
+// Workgroup: 256 lanes = 4 waves per (facet, 64-step tile), on a 1-D grid
+// mapped so that all tiles of a facet run on one XCD (workgroups are dealt
+// to the 8 XCDs round-robin by id): the facet's drf rows, re-read by each of
+// its tiles, then come from that XCD's L2 instead of HBM.  The facet's
+// entries are staged in chunks of 16 (drf rows and weight rows in LDS) and
+// multiplied on the matrix cores with v_mfma_f32_16x16x4_f32 (FP32 products,
+// FP32 accumulation): M = 100 padded to 112 (7 tiles), N = 64 (4 tiles),
+// wave w owning output tiles w, w+4, ... of the 28.  Fragment layout: lane l
+// supplies A[m = l%16][k = l/16] and B[k = l/16][n = l%16] and receives
+// D[m = 4(l/16) + i][n = l%16], i = 0..3.
 // ---------------------------------------------------------------------------
 extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
     const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ drf,
@@ -163,52 +172,95 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
     const float* __restrict__ base, const float* __restrict__ base_scale, float base_gain,
     float* __restrict__ dst) {
   constexpr int KC = 16;
-  __shared__ float ds[KC][BTPM_NS + 4];
-  __shared__ float ws[KC][64];
-  const int f = blockIdx.y;
-  const int s0 = blockIdx.x * 64;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int jw = (tid >> 6) * 25;
-  const int s = s0 + lane;
-  float acc[25];
-#pragma unroll
-  for (int q = 0; q < 25; ++q) acc[q] = 0.0f;
-  const int k0 = rowptr[f], k1 = rowptr[f + 1];
-  for (int kb = k0; kb < k1; kb += KC) {
-    const int kn = min(KC, k1 - kb);
-    __syncthreads();
-    for (int e = tid; e < KC * BTPM_NS; e += 256) {
-      const int kk = e / BTPM_NS, jj = e - kk * BTPM_NS;
-      ds[kk][jj] = kk < kn ? drf[(size_t)(kb + kk) * BTPM_NS + jj] : 0.0f;
-    }
-    for (int e = tid; e < KC * 64; e += 256) {
-      const int kk = e >> 6, ss = min(s0 + (e & 63), ntau - 1);
-      float v = 0.0f;
-      if (kk < kn) {
-        const int g = colidx[kb + kk];
-        v = w[((size_t)(ss / BTPM_BLK) * nsmooth + g) * BTPM_BLK + (ss % BTPM_BLK)];
-        if (mode == 1) v = es * (v * v) * (v * v);
-      }
-      ws[kk][e & 63] = v;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < KC; ++kk) {
-      const float wv = ws[kk][lane];
-#pragma unroll
-      for (int q = 0; q < 25; ++q) acc[q] = fmaf(ds[kk][jw + q], wv, acc[q]);
-    }
-  }
-  if (s >= nb * BTPM_BLK) return;
+  constexpr int kMT = 7, kNT = 4, kTiles = kMT * kNT;
+  constexpr int kDS = 116, kWS = 68;  // LDS row strides (floats)
+  __shared__ float ds[KC][kDS];       // drf chunk, j padded to 112 with zeros
+  __shared__ float ws[KC][kWS];       // weight chunk over the 64 steps
+  const int ntile = (nb * BTPM_BLK + 63) / 64;
+  const int xcd = (int)(blockIdx.x & 7), local = (int)(blockIdx.x >> 3);
+  const int f = 8 * (local / ntile) + xcd;
+  const int s0 = 64 * (local % ntile);
+  if (f >= nfac) return;
+  const int tid = (int)threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int lm = lane & 15, lk = lane >> 4;
   const int ncols = nfac * BTPM_NS;
   const float bsc = base_gain * (base_scale ? base_scale[f] : 1.0f);
-  for (int q = 0; q < 25; ++q) {
-    const int c = f * BTPM_NS + jw + q;
-    const size_t o = ((size_t)(s / BTPM_BLK) * ncols + c) * BTPM_BLK + (s % BTPM_BLK);
-    float v = acc[q];
-    if (base) v += bsc * base[o];
-    dst[o] = v;
+  const int k0 = rowptr[f], k1 = rowptr[f + 1];
+  typedef float float4v __attribute__((ext_vector_type(4)));
+  float4v acc[(kTiles + 3) / 4];
+#pragma unroll
+  for (int t = 0; t < (kTiles + 3) / 4; ++t) acc[t] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+  // the next chunk is loaded into registers while the current one is on the
+  // matrix cores: thread tid stages drf elements tid + 256 i (i < 7) and
+  // weight elements tid + 256 i (i < 4) of every chunk
+  constexpr int kRD = KC * 112 / 256, kRW = KC * 64 / 256;
+  float rd[kRD], rw[kRW];
+  auto fetch = [&](int kb) {
+    const int kn = min(KC, k1 - kb);
+#pragma unroll
+    for (int i = 0; i < kRD; ++i) {
+      const int e = tid + 256 * i, kk = e / 112, jj = e - (e / 112) * 112;
+      rd[i] = (kk < kn && jj < BTPM_NS) ? drf[(size_t)(kb + kk) * BTPM_NS + jj] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < kRW; ++i) {
+      const int e = tid + 256 * i, kk = e >> 6, ss = min(s0 + (e & 63), ntau - 1);
+      rw[i] = 0.0f;
+      if (kk < kn) {
+        const int g = colidx[kb + kk];
+        rw[i] = w[((size_t)(ss / BTPM_BLK) * nsmooth + g) * BTPM_BLK + (ss % BTPM_BLK)];
+      }
+    }
+  };
+  if (k0 < k1) fetch(k0);
+  for (int kb = k0; kb < k1; kb += KC) {
+    __syncthreads();  // the previous chunk's products have read LDS
+#pragma unroll
+    for (int i = 0; i < kRD; ++i) {
+      const int e = tid + 256 * i;
+      ds[e / 112][e - (e / 112) * 112] = rd[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kRW; ++i) {
+      const int e = tid + 256 * i;
+      float v = rw[i];
+      if (mode == 1) v = es * (v * v) * (v * v);
+      ws[e >> 6][e & 63] = v;
+    }
+    __syncthreads();
+    if (kb + KC < k1) fetch(kb + KC);
+#pragma unroll
+    for (int t = 0; t < (kTiles + 3) / 4; ++t) {
+      const int tile = wave + 4 * t;
+      if (tile < kTiles) {
+        const int mt = tile / kNT, nt = tile - (tile / kNT) * kNT;
+#pragma unroll
+        for (int kq = 0; kq < KC; kq += 4) {
+          const float a = ds[kq + lk][mt * 16 + lm];
+          const float b = ws[kq + lk][nt * 16 + lm];
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < (kTiles + 3) / 4; ++t) {
+    const int tile = wave + 4 * t;
+    if (tile >= kTiles) continue;
+    const int mt = tile / kNT, nt = tile - (tile / kNT) * kNT;
+    const int s = s0 + nt * 16 + lm;
+    if (s >= nb * BTPM_BLK) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int j = mt * 16 + 4 * lk + i;
+      if (j >= BTPM_NS) continue;
+      const int c = f * BTPM_NS + j;
+      const size_t o = ((size_t)(s / BTPM_BLK) * ncols + c) * BTPM_BLK + (s % BTPM_BLK);
+      float v = acc[t][i];
+      if (base) v += bsc * base[o];
+      dst[o] = v;
+    }
   }
 }
 

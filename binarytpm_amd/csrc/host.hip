@@ -329,7 +329,7 @@ void build_crater(btpm_solve* s, hipStream_t st) {
   d.tile_cols = BTPM_CRATERS_WG;
   // src = (1-A) illum + RGH (RGH from the converged smooth T)
   if (sh->nnz > 0 && sh->drf.p) {
-    dim3 g(grid1((size_t)d.nb * BTPM_BLK, 64), sh->nf);
+    dim3 g(8 * grid1(sh->nf, 8) * grid1((size_t)d.nb * BTPM_BLK, 64));  // XCD-grouped facets
     hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                        sh->colidx.as<int>(), sh->drf.as<float>(), s->smooth->T.as<float>(), 1,
                        sh->p.eps * sh->p.sigma, sh->nf, sh->nf, sh->p.ntau, d.nb,
@@ -977,7 +977,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
                            AF.as<float>(), sh->F.as<float>(), sh->albedo.as<float>(), 1.0f, nf,
                            nser);
         rgi_blk.alloc(ncser * 4);
-        dim3 g(grid1((size_t)nb * BTPM_BLK, 64), nf);
+        dim3 g(8 * grid1(nf, 8) * grid1((size_t)nb * BTPM_BLK, 64));  // XCD-grouped facets
         hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                            sh->colidx.as<int>(), sh->drf.as<float>(), AF.as<float>(), 0, 0.0f, nf,
                            nf, nt, nb, (const float*)nullptr, (const float*)nullptr, 1.0f,
