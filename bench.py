@@ -243,7 +243,8 @@ def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2
                                 scatter_acc=1e-4)
             pool[(sd, k)] = th.Shape(ctx, p, case.illum, 0.1, crater=cm)
             del sun_local, case
-    log(f"serving pool: {len(pool)} shapes resident in {time.time() - t0:.1f} s")
+    log(f"serving pool: {len(pool)} shapes resident in {time.time() - t0:.1f} s, "
+        f"{ctx.mem_info()[0] / 1e9:.0f} GB of HBM free")
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, n_req)
     arrive = np.cumsum(gaps) - gaps[0]
@@ -255,22 +256,34 @@ def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2
     lock = threading.Condition()
 
     def waiter():
-        i = 0
-        while i < n_req:
+        # consume results in completion order (btpm_server_wait_any), so a
+        # finished request never holds device memory while an earlier one runs
+        outstanding, taken, finished = [], 0, 0
+        while finished < n_req:
             with lock:
-                while len(submitted) <= i:
+                while taken < len(submitted):
+                    outstanding.append(submitted[taken] + (taken,))
+                    taken += 1
+                if not outstanding:
                     lock.wait()
-                s, req = submitted[i]
+                    continue
             try:
-                srv.wait(s, timeout_s=900.0)
-                inf = s.info()
-                results.append((req, inf.latency_s, inf.converged and
-                                (not s.with_craters or inf.crater_converged),
-                                inf.column_steps + inf.crater_column_steps, time.perf_counter()))
-            except Exception as e:  # noqa: BLE001 -- reported below
+                k = srv.wait_any([o[0] for o in outstanding], timeout_s=0.5)
+            except Exception as e:  # noqa: BLE001 -- timeouts poll for new arrivals
+                if "timeout" in str(e):
+                    continue
                 errors.append(repr(e))
+                return
+            s, req, i = outstanding.pop(k)
+            inf = s.info()
+            results.append((req, inf.latency_s, inf.converged and
+                            (not s.with_craters or inf.crater_converged),
+                            inf.column_steps + inf.crater_column_steps, time.perf_counter()))
             s.close()
-            i += 1
+            finished += 1
+            fr, _ = ctx.mem_info()
+            log(f"serving: request {i} done ({20 * 4 ** req[0]} facets, TI {req[2]:.0f}, "
+                f"latency {inf.latency_s:.2f} s; {fr / 1e9:.0f} GB of HBM free)")
 
     th_w = threading.Thread(target=waiter, daemon=True)
     th_w.start()
@@ -313,6 +326,8 @@ def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2
         "latency_p99_s": float(np.percentile(lat, 99)),
         "latency_max_s": float(lat.max()),
         "graph_launches": int(st.graph_launches),
+        "max_resident": int(st.max_resident),
+        "deferred_set_builds": int(st.deferrals),
     }
 
 

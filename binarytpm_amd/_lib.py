@@ -78,7 +78,7 @@ class BatchStats(C.Structure):
         ("smooth_kernel_s", C.c_double), ("crater_kernel_s", C.c_double),
         ("smooth_launches", C.c_int), ("crater_launches", C.c_int),
         ("smooth_column_steps", C.c_longlong), ("crater_column_steps", C.c_longlong),
-        ("max_resident", C.c_int),
+        ("max_resident", C.c_int), ("deferrals", C.c_int),
     ]
 
 
@@ -114,6 +114,8 @@ PROTOTYPES = [
     ("btpm_server_start", C.c_int, [_vp, C.POINTER(BatchOpts), C.POINTER(_vp)]),
     ("btpm_server_submit", C.c_int, [_vp, _vp]),
     ("btpm_server_wait", C.c_int, [_vp, _vp, C.c_double]),
+    ("btpm_server_wait_any", C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_double,
+                                       C.POINTER(C.c_int)]),
     ("btpm_server_stop", C.c_int, [_vp, C.POINTER(BatchStats)]),
     ("btpm_crater_contract", C.c_int, [_vp, _fp, _fp, C.c_int, _fp]),
 ]

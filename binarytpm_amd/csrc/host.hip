@@ -11,8 +11,10 @@
 #include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -36,6 +38,7 @@ struct Pool {
   std::map<int, std::multimap<size_t, void*>> cache;  // device -> capacity -> block
   std::map<void*, std::pair<int, size_t>> live;       // block -> (device, capacity)
   std::map<int, size_t> cached;
+  std::map<int, size_t> live_bytes;
 };
 Pool& the_pool() {
   static Pool* p = new Pool;  // never destroyed: DBufs of static objects may outlive main
@@ -58,23 +61,37 @@ void* pool_alloc(size_t bytes) {
       c.erase(it);
       P.cached[dev] -= cap;
       P.live[p] = {dev, cap};
+      P.live_bytes[dev] += cap;
       return p;
     }
   }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, want);
+  size_t fr0 = 0, fr1 = 0, tot = 0, trimmed = 0;
   if (e != hipSuccess) {
     (void)hipGetLastError();
+    (void)hipMemGetInfo(&fr0, &tot);
+    trimmed = pool_cached(dev);
     pool_trim(dev);
+    (void)hipMemGetInfo(&fr1, &tot);
     e = hipMalloc(&p, want);
   }
   if (e != hipSuccess) {
+    (void)hipGetLastError();
     const int code = e == hipErrorOutOfMemory ? BTPM_ERR_NOMEM : BTPM_ERR_HIP;
-    fail(code, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    size_t live = 0;
+    {
+      std::lock_guard<std::mutex> lk(P.mu);
+      live = P.live_bytes[dev];
+    }
+    fail(code, "hipMalloc(%zu): %s (device free %.1f GB, %.1f GB after returning %.1f GB of "
+         "cached blocks, of %.1f GB; the library holds %.1f GB)", want, hipGetErrorString(e),
+         fr0 / 1e9, fr1 / 1e9, trimmed / 1e9, tot / 1e9, live / 1e9);
     throw HipError{code};
   }
   std::lock_guard<std::mutex> lk(P.mu);
   P.live[p] = {dev, want};
+  P.live_bytes[dev] += want;
   return p;
 }
 
@@ -89,6 +106,7 @@ void pool_free(void* p) {
   const int dev = it->second.first;
   const size_t cap = it->second.second;
   P.live.erase(it);
+  P.live_bytes[dev] -= cap;
   P.cache[dev].emplace(cap, p);
   P.cached[dev] += cap;
 }
@@ -198,8 +216,9 @@ struct btpm_solve {
   int with_craters = 0;
   float flux_scale = 1.0f;  // multiplies every absorbed flux of the shape
   bool queued = false;      // handed to a server (not yet admitted)
-  // 0 created, 1 smooth running, 2 crater running, 3 done
-  int phase = 0;
+  // 0 created, 1 smooth running, 2 crater running, 3 done,
+  // 4 smooth done and the crater set waiting for device memory
+  std::atomic<int> phase{0};
   std::unique_ptr<SetHost> smooth, crater;
   btpm_solve_info info{};
   std::chrono::steady_clock::time_point t_submit;
@@ -374,6 +393,13 @@ size_t crater_bytes(const btpm_shape* sh) {
 // ---------------------------------------------------------------------------
 // scheduler
 // ---------------------------------------------------------------------------
+// Scheduler trace on stderr (admissions, crater-set builds, completions with
+// the memory figures behind them) when the environment sets BTPM_TRACE.
+bool trace_on() {
+  static const bool on = std::getenv("BTPM_TRACE") != nullptr;
+  return on;
+}
+
 struct Sched {
   btpm_ctx* ctx;
   btpm_batch_opts opt;
@@ -405,6 +431,7 @@ struct Sched {
   size_t reserved = 0;  // crater bytes promised to admitted solves not yet allocated
   int craters_active = 0;
   std::chrono::steady_clock::time_point t0;
+  std::function<void()> on_complete;  // server: wakes btpm_server_wait_any
 
   // device memory free for new sets: the driver's free memory plus the
   // library's cached blocks
@@ -492,15 +519,57 @@ struct Sched {
   void admit() {
     while (!pending.empty() && can_admit(pending.front())) {
       btpm_solve* s = pending.front();
+      try {
+        build_smooth(s, ctx->stream);
+      } catch (const HipError& e) {
+        // the footprint estimate was short: wait for running solves to finish
+        if (e.code != BTPM_ERR_NOMEM || active.empty()) throw;
+        stats.deferrals++;
+        break;
+      }
       pending.pop_front();
+      if (trace_on())
+        std::fprintf(stderr, "[btpm] admit %p nf %d craters %d: free %.1f GB, reserved %.1f GB, "
+                     "headroom %.1f GB, crater need %.1f GB, active %zu\n", (void*)s,
+                     s->shape->nf, s->with_craters, free_bytes() / 1e9, reserved / 1e9,
+                     headroom / 1e9, s->with_craters ? crater_bytes(s->shape) / 1e9 : 0.0,
+                     active.size());
       if (s->with_craters) reserved += crater_bytes(s->shape);
       if (s->with_craters) craters_active++;
-      build_smooth(s, ctx->stream);
       s->phase = 1;
       add_set(s, s->smooth.get());
       active.push_back(s);
     }
     stats.max_resident = std::max(stats.max_resident, (int)active.size());
+  }
+
+  // Builds the crater set of a solve whose smooth phase is done (phase 4).
+  // Out of memory while other solves still run is not an error: the solve
+  // waits and is retried after the next chunk.
+  bool try_build_crater(btpm_solve* s) {
+    bool running = false;
+    for (btpm_solve* o : active) running = running || o->phase == 1 || o->phase == 2;
+    try {
+      build_crater(s, ctx->stream);
+    } catch (const HipError& e) {
+      if (e.code == BTPM_ERR_NOMEM && !running)
+        throw HipError{fail(BTPM_ERR_NOMEM, "%s; no other solve is running -- finished solves keep "
+                            "their results on the device until they are destroyed",
+                            last_error().c_str())};
+      if (e.code != BTPM_ERR_NOMEM) throw;
+      stats.deferrals++;
+      if (trace_on())
+        std::fprintf(stderr, "[btpm] crater set of %p deferred: free %.1f GB, reserved %.1f GB\n",
+                     (void*)s, free_bytes() / 1e9, reserved / 1e9);
+      return false;
+    }
+    if (trace_on())
+      std::fprintf(stderr, "[btpm] crater set of %p built: free %.1f GB, reserved %.1f GB\n",
+                   (void*)s, free_bytes() / 1e9, reserved / 1e9);
+    reserved -= std::min(reserved, crater_bytes(s->shape));
+    s->phase = 2;
+    add_set(s, s->crater.get());
+    return true;
   }
 
   int needed_tiles(const SetHost* S) const {
@@ -682,11 +751,15 @@ struct Sched {
     if (s->with_craters) craters_active--;
     stats.solves++;
     if (s->info.converged && (!s->with_craters || s->info.crater_converged)) stats.converged++;
+    if (trace_on())
+      std::fprintf(stderr, "[btpm] complete %p after %.2f s: free %.1f GB\n", (void*)s,
+                   s->info.latency_s, free_bytes() / 1e9);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       s->phase = 3;
     }
     s->cv.notify_all();
+    if (on_complete) on_complete();
   }
 
   // inspect statuses, advance phases, retire finished solves
@@ -700,15 +773,15 @@ struct Sched {
           finish_set(s, S);
           shrink_smooth(S);
           if (s->with_craters) {
-            reserved -= std::min(reserved, crater_bytes(s->shape));
-            build_crater(s, ctx->stream);
-            s->phase = 2;
-            add_set(s, s->crater.get());
+            s->phase = 4;
+            try_build_crater(s);
           } else {
             complete(s);
             alive = false;
           }
         }
+      } else if (s->phase == 4) {
+        try_build_crater(s);
       } else if (s->phase == 2) {
         SetHost* S = s->crater.get();
         if (ctx->h_status[S->slot].done) {
@@ -826,6 +899,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     return fail(BTPM_ERR_ARG, "crater needs illum, shadow+cosang, or sun_local+vis_table+normals");
   if (p->max_rev < 1 || p->newton_cap < 1 || !(p->tacc > 0) || !(p->newton_tol > 0))
     return fail(BTPM_ERR_ARG, "bad convergence parameters");
+  CtxLock lk(ctx);
   API_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -1007,6 +1081,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
 
 int btpm_shape_destroy(btpm_shape* sh) {
   if (!sh) return BTPM_OK;
+  CtxLock lk(sh->ctx);
   (void)hipSetDevice(sh->ctx->device);
   (void)hipStreamSynchronize(sh->ctx->stream);
   delete sh;
@@ -1015,6 +1090,7 @@ int btpm_shape_destroy(btpm_shape* sh) {
 
 int btpm_shape_facet_illum(btpm_shape* sh, float* out) {
   if (!sh || !out) return fail(BTPM_ERR_ARG, "null argument");
+  CtxLock lk(sh->ctx);
   API_BEGIN
   btpm_ctx* c = sh->ctx;
   HIPCHK(hipSetDevice(c->device));
@@ -1030,6 +1106,7 @@ int btpm_shape_facet_illum(btpm_shape* sh, float* out) {
 int btpm_shape_crater_illum(btpm_shape* sh, float* out) {
   if (!sh || !out) return fail(BTPM_ERR_ARG, "null argument");
   if (!sh->has_crater) return fail(BTPM_ERR_STATE, "shape has no crater model");
+  CtxLock lk(sh->ctx);
   API_BEGIN
   btpm_ctx* c = sh->ctx;
   HIPCHK(hipSetDevice(c->device));
@@ -1069,6 +1146,7 @@ int btpm_solve_set_flux_scale(btpm_solve* s, float scale) {
 
 int btpm_solve_destroy(btpm_solve* s) {
   if (!s) return BTPM_OK;
+  CtxLock lk(s->ctx);
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   delete s;
@@ -1081,7 +1159,8 @@ int btpm_batch_run(btpm_ctx* ctx, btpm_solve** solves, int n, const btpm_batch_o
   for (int i = 0; i < n; ++i)
     if (!solves[i] || solves[i]->phase != 0 || solves[i]->queued)
       return fail(BTPM_ERR_STATE, "solve %d not fresh", i);
-  std::lock_guard<std::mutex> lk(ctx->mu);
+  CtxLock lk(ctx);
+  if (ctx->servers > 0) return fail(BTPM_ERR_STATE, "a server is running on this context");
   API_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   btpm_batch_opts o;
@@ -1174,7 +1253,8 @@ int btpm_solve_run_stepwise(btpm_solve* s, int rotation_cap) {
   if (!s) return fail(BTPM_ERR_ARG, "null solve");
   if (s->phase != 0 || s->queued) return fail(BTPM_ERR_STATE, "solve not fresh");
   btpm_ctx* ctx = s->ctx;
-  std::lock_guard<std::mutex> lk(ctx->mu);
+  CtxLock lk(ctx);
+  if (ctx->servers > 0) return fail(BTPM_ERR_STATE, "a server is running on this context");
   API_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -1205,6 +1285,7 @@ int btpm_solve_get_info(btpm_solve* s, btpm_solve_info* info) {
 
 static int fetch_series(btpm_solve* s, SetHost* S, float* out, int crater) {
   btpm_ctx* c = s->ctx;
+  CtxLock lk(c);
   API_BEGIN
   HIPCHK(hipSetDevice(c->device));
   const int nt = s->shape->p.ntau, nb = S->d.nb, nf = s->shape->nf;
@@ -1235,6 +1316,7 @@ int btpm_solve_crater_temps(btpm_solve* s, float* out) {
 
 static int fetch_profile(btpm_solve* s, SetHost* S, float* out, int crater) {
   btpm_ctx* c = s->ctx;
+  CtxLock lk(c);
   API_BEGIN
   HIPCHK(hipSetDevice(c->device));
   const int nc = S->d.ncols, np = BTPM_NX + 1;
@@ -1269,6 +1351,7 @@ int btpm_solve_crater_profile(btpm_solve* s, float* out) {
 
 int btpm_crater_contract(btpm_ctx* ctx, const float* vf, const float* e, int ncol, float* q) {
   if (!ctx || !vf || !e || !q || ncol < 1) return fail(BTPM_ERR_ARG, "bad argument");
+  CtxLock lk(ctx);
   API_BEGIN
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -1300,43 +1383,81 @@ struct btpm_server {
   std::condition_variable cv;
   std::deque<btpm_solve*> inbox;
   bool stop = false;
+  std::mutex done_mu;  // completions, for btpm_server_wait_any
+  std::condition_variable done_cv;
   std::atomic<int> err{BTPM_OK};
   std::string errmsg;
   btpm_batch_stats stats{};
 };
 
 static void server_loop(btpm_server* srv) {
-  std::lock_guard<std::mutex> glk(srv->ctx->mu);
+  btpm_ctx* ctx = srv->ctx;
+  std::unique_ptr<Sched> sc;
   try {
-    HIPCHK(hipSetDevice(srv->ctx->device));
-    Sched sc(srv->ctx, srv->opt);
+    {
+      std::lock_guard<std::mutex> glk(ctx->mu);
+      HIPCHK(hipSetDevice(ctx->device));
+      sc = std::make_unique<Sched>(ctx, srv->opt);
+      sc->on_complete = [srv] {
+        std::lock_guard<std::mutex> lk(srv->done_mu);
+        srv->done_cv.notify_all();
+      };
+    }
     for (;;) {
       {
         std::unique_lock<std::mutex> lk(srv->mu);
-        if (sc.active.empty() && sc.pending.empty())
+        if (sc->active.empty() && sc->pending.empty())
           srv->cv.wait(lk, [&] { return srv->stop || !srv->inbox.empty(); });
         while (!srv->inbox.empty()) {
           btpm_solve* s = srv->inbox.front();
           srv->inbox.pop_front();
-          sc.pending.push_back(s);
+          sc->pending.push_back(s);
         }
-        if (srv->stop && sc.active.empty() && sc.pending.empty()) break;
+        if (srv->stop && sc->active.empty() && sc->pending.empty()) break;
       }
-      sc.step();
+      // the stream is the server's for one scheduling step at a time; API
+      // calls waiting for it (results, destroys) go first
+      while (ctx->waiting.load() > 0) std::this_thread::yield();
+      std::lock_guard<std::mutex> glk(ctx->mu);
+      sc->step();
     }
-    sc.stats.wall_s =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - sc.t0).count();
-    srv->stats = sc.stats;
+    sc->stats.wall_s =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - sc->t0).count();
+    srv->stats = sc->stats;
   } catch (const HipError& e) {
-    srv->err = e.code;
     srv->errmsg = last_error();
+    srv->err = e.code;  // after errmsg: a waiter that sees err reads a complete message
+    // wake every waiter now (btpm_server_wait's predicate sees the error)
+    std::vector<btpm_solve*> all;
+    if (sc) {
+      all.insert(all.end(), sc->active.begin(), sc->active.end());
+      all.insert(all.end(), sc->pending.begin(), sc->pending.end());
+    }
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      all.insert(all.end(), srv->inbox.begin(), srv->inbox.end());
+    }
+    for (btpm_solve* s : all) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->cv.notify_all();
+    }
+    std::lock_guard<std::mutex> lk(srv->done_mu);
+    srv->done_cv.notify_all();
   }
+  std::lock_guard<std::mutex> glk(ctx->mu);
+  sc.reset();
+  ctx->servers--;
 }
 
 extern "C" {
 
 int btpm_server_start(btpm_ctx* ctx, const btpm_batch_opts* opts, btpm_server** out) {
   if (!ctx || !out) return fail(BTPM_ERR_ARG, "null argument");
+  {
+    CtxLock lk(ctx);
+    if (ctx->servers > 0) return fail(BTPM_ERR_STATE, "a server is already running on this context");
+    ctx->servers++;
+  }
   auto srv = std::make_unique<btpm_server>();
   srv->ctx = ctx;
   btpm_batch_opts_default(&srv->opt);
@@ -1367,6 +1488,28 @@ int btpm_server_wait(btpm_server* srv, btpm_solve* s, double timeout_s) {
                                  [&] { return s->phase == 3 || srv->err != BTPM_OK; });
   if (srv->err != BTPM_OK) return fail(srv->err, "server failed: %s", srv->errmsg.c_str());
   if (!ok) return fail(BTPM_ERR_STATE, "timeout");
+  return BTPM_OK;
+}
+
+int btpm_server_wait_any(btpm_server* srv, btpm_solve* const* solves, int n, double timeout_s,
+                         int* index) {
+  if (!srv || !solves || n < 1 || !index) return fail(BTPM_ERR_ARG, "bad argument");
+  for (int i = 0; i < n; ++i)
+    if (!solves[i]) return fail(BTPM_ERR_ARG, "null solve %d", i);
+  int found = -1;
+  std::unique_lock<std::mutex> lk(srv->done_mu);
+  const bool ok = srv->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
+    for (int i = 0; i < n; ++i)
+      if (solves[i]->phase == 3) {
+        found = i;
+        return true;
+      }
+    return srv->err != BTPM_OK;
+  });
+  if (srv->err != BTPM_OK && found < 0)
+    return fail(srv->err, "server failed: %s", srv->errmsg.c_str());
+  if (!ok) return fail(BTPM_ERR_STATE, "timeout");
+  *index = found;
   return BTPM_OK;
 }
 

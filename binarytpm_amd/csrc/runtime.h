@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -117,6 +118,25 @@ struct btpm_ctx {
   btpm_status_dev* d_status = nullptr;
   btpm_status_dev* h_status = nullptr;  // pinned mirror
   std::vector<int> free_slots;
-  std::mutex mu;                        // one scheduler at a time
+  // Serialises every use of `stream` across threads (a server thread
+  // capturing a graph on it must not see another thread's work or syncs):
+  // the server holds it per scheduling step, the other entry points per call.
+  std::mutex mu;
+  std::atomic<int> waiting{0};  // callers blocked on mu: a server yields to them
+  int servers = 0;              // running servers (at most one; guarded by mu)
+};
+
+// Takes ctx->mu for one API call, announcing itself so that a running
+// server lets it in between two scheduling steps (std::mutex is not fair).
+struct CtxLock {
+  btpm_ctx* c;
+  explicit CtxLock(btpm_ctx* ctx) : c(ctx) {
+    c->waiting++;
+    c->mu.lock();
+    c->waiting--;
+  }
+  ~CtxLock() { c->mu.unlock(); }
+  CtxLock(const CtxLock&) = delete;
+  CtxLock& operator=(const CtxLock&) = delete;
 };
 

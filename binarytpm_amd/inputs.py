@@ -479,5 +479,7 @@ def serving_shape(subdiv: int, seed: int, ntau: int = 650):
 
 
 def sweep_inertias(n: int = 32, lo: float = 10.0, hi: float = 2500.0):
-    """Log-spaced thermal inertias Gamma_k = lo * (hi/lo)^(k/(n-1))."""
+    """Log-spaced thermal inertias Gamma_k = lo * (hi/lo)^(k/(n-1)) (n = 1: lo)."""
+    if n <= 1:
+        return np.full(max(n, 0), lo)
     return lo * (hi / lo) ** (np.arange(n) / (n - 1))

@@ -282,6 +282,14 @@ class Server:
     def wait(self, s: Solve, timeout_s: float = 600.0):
         check(lib().btpm_server_wait(self._h, s.handle, float(timeout_s)))
 
+    def wait_any(self, solves: Sequence["Solve"], timeout_s: float = 600.0) -> int:
+        """Index of a completed solve among `solves` (blocks until one is)."""
+        arr = (C.c_void_p * len(solves))(*[s.handle for s in solves])
+        idx = C.c_int(-1)
+        check(lib().btpm_server_wait_any(self._h, arr, len(solves), float(timeout_s),
+                                         C.byref(idx)))
+        return idx.value
+
     def stop(self) -> BatchStats:
         st = BatchStats()
         h, self._h = self._h, None

@@ -156,6 +156,7 @@ typedef struct {
     long long smooth_column_steps;
     long long crater_column_steps;
     int max_resident;            /* most solves resident at once           */
+    int deferrals;               /* set builds postponed for lack of memory */
 } btpm_batch_stats;
 
 /* ---- library ------------------------------------------------------------ */
@@ -243,6 +244,13 @@ int btpm_server_start(btpm_ctx *ctx, const btpm_batch_opts *opts, btpm_server **
 int btpm_server_submit(btpm_server *srv, btpm_solve *s);
 /* blocks until s completes (or timeout_s elapses: BTPM_ERR_STATE)          */
 int btpm_server_wait(btpm_server *srv, btpm_solve *s, double timeout_s);
+/* Blocks until one of solves[0..n) has completed (or timeout_s elapses:
+ * BTPM_ERR_STATE) and stores its index in *index (the lowest index among
+ * completed ones).  Lets a client consume results in completion order, so
+ * finished solves do not hold device memory while it waits for earlier
+ * requests.  No counterpart in the reference (which serves no requests).  */
+int btpm_server_wait_any(btpm_server *srv, btpm_solve *const *solves, int n, double timeout_s,
+                         int *index);
 int btpm_server_stop(btpm_server *srv, btpm_batch_stats *stats);
 
 /* ---- kernel-level entry points (tests and tools) ----------------------------

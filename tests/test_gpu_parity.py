@@ -248,6 +248,57 @@ def test_server_results_match_alone(ctx):
         assert s.info().latency_s > 0
 
 
+def test_server_concurrent_readback_and_destroy(ctx):
+    """Results read back and solves destroyed from another thread while the
+    server keeps capturing and launching graphs for later arrivals: every
+    result equals the solo run, and a batch run is refused while the server
+    owns the context."""
+    import threading
+    import time
+    case = inputs.config0(subdiv=1)
+    p = th.params(650, case.rot_period, tacc=0.01)
+    shape = th.Shape(ctx, p, case.illum, 0.1)
+    tis = [15.0 * 1.4 ** k for k in range(16)]
+    alone = {t: th.Solve(ctx, shape, t).run().surface_temps() for t in tis}
+    srv = th.Server(ctx, th.batch_opts(rotations_per_sync=2, use_graph=True))
+    extra = th.Solve(ctx, shape, 100.0)
+    with pytest.raises(_lib.BtpmError, match="server"):
+        th.batch_run(ctx, [extra])
+    extra.close()
+    queue, got, errs = [], {}, []
+    cond = threading.Condition()
+
+    def client():
+        for i in range(len(tis)):
+            with cond:
+                while len(queue) <= i:
+                    cond.wait()
+                t, s = queue[i]
+            try:
+                srv.wait(s, 120.0)
+                got[t] = s.surface_temps()
+            except Exception as e:  # noqa: BLE001 -- reported below
+                errs.append(repr(e))
+            s.close()
+
+    th_c = threading.Thread(target=client)
+    th_c.start()
+    for t in tis:
+        s = th.Solve(ctx, shape, t)
+        srv.submit(s)
+        with cond:
+            queue.append((t, s))
+            cond.notify_all()
+        time.sleep(0.01)
+    th_c.join(timeout=300)
+    stats = srv.stop()
+    assert not errs, errs
+    assert stats.solves == len(tis)
+    for t in tis:
+        np.testing.assert_array_equal(alone[t], got[t])
+    shape.close()
+
+
 def test_full_size_config0_parity(ctx):
     """Full config-0 size (5120 facets, 650 steps, TI 200) against the
     oracle, plus the periodic energy balance of the GPU result."""
