@@ -28,22 +28,29 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "conduct_crater_traffic.md")
+VALU_FILE = os.path.join(ROOT, "profiles", "conduct_crater_valu.md")
+MFMA_PEAK_TFLOPS = 2500.0  # dense FP16
 NX = 56
 BYTES_PER_COLSTEP = 12.0   # read absorbed flux + previous-rotation T, write T
 
 
-def measured_traffic_per_colstep():
-    """HBM bytes per integrated crater column-step from the committed
-    FETCH_SIZE/WRITE_SIZE counter summary (scaled by a copy of known size),
-    or None when the summary is absent."""
+def summary_value(path, key):
+    """A `key: value` line of a committed counter summary under profiles/,
+    or None when absent."""
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(path) as fh:
             for ln in fh:
-                if ln.startswith("traffic_bytes_per_column_step:"):
+                if ln.startswith(key + ":"):
                     return float(ln.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic_per_colstep():
+    """HBM bytes per integrated crater column-step from the FETCH_SIZE /
+    WRITE_SIZE counter summary (scaled by a copy of known size)."""
+    return summary_value(TRAFFIC_FILE, "traffic_bytes_per_column_step")
 
 
 def log(msg):
@@ -446,6 +453,15 @@ def main():
                 "bytes_per_column_step": BYTES_PER_COLSTEP,
                 "smooth_kernel_gbs": BYTES_PER_COLSTEP * sk_cs / sk_s / 1e9 if sk_s > 0 else 0.0,
                 "smooth_avg_launch_ms": 1000.0 * sk_s / max(sk_n, 1),
+                # the kernel's actual bound: VALU issue (SQ counters, committed summary)
+                "valu_instr_per_column_step": summary_value(VALU_FILE,
+                                                            "valu_instr_per_column_step"),
+                "valu_issue_frac": summary_value(VALU_FILE, "valu_issue_frac"),
+                # the crater contraction on the matrix cores (executed FLOP incl. split)
+                "mfma_crater_product_tflops": (
+                    summary_value(VALU_FILE, "mfma_flop_per_column_step") * ck_cs / ck_s / 1e12
+                    if (ck_s > 0 and summary_value(VALU_FILE, "mfma_flop_per_column_step"))
+                    else None),
             },
         }
         if not args.no_plain:
