@@ -479,6 +479,7 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
   const int tid = (int)threadIdx.x;
   const int f = blockIdx.x * kSHRows + tid;
   const bool live = f < nf && (S.inst || !S.frozen[f]);
+  if (__syncthreads_count(live) == 0) return;  // every receiver here is frozen
   const float* __restrict__ emis = S.emis;
   float acc[kSHSteps];
 #pragma unroll
