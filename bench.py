@@ -413,7 +413,8 @@ def main():
         tpc = measured_traffic_per_colstep()
         traffic = tpc * ck_cs / ck_n if (tpc is not None and ck_n > 0) else None
         line = {
-            "metric": "converged solves/hour (128-solve TI x roughness sweep)",
+            "metric": ("converged solves/hour & facet\u00b7layer\u00b7steps/s, p50/p99 latency, "
+                       "at |\u0394T|\u22640.05 K vs FP64"),
             "value": 3600.0 * conv / T,
             "unit": "converged solves/hour",
             "n_gpus": 1,
@@ -423,7 +424,9 @@ def main():
             "higher_is_better": True,
             "scaling": "weak",
             "vs_baseline": None,
-            "dtype": "f32 state, f64 Newton, fp16x3 MFMA crater product",
+            "dtype": "f32",
+            "precision": ("f32 state and conduction, f64 Newton surface balance, crater product "
+                          "fp16 hi/lo x3 on MFMA with f32 accumulation"),
             "data": "synthetic (seeded shapes, illumination, view factors, crater model)",
             "config": {
                 "workload": "128-solve sweep: 32 TI x f{0,.25,.5,1}, 5120-facet primary, "
