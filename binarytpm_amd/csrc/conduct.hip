@@ -31,6 +31,7 @@
 #include "btpm_device.h"
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 // Pointers read from the set descriptor are generic; casting them to the
 // global address space lets the compiler emit global (not flat) memory ops,
 // whose completion is tracked by vmcnt alone so prefetches stay in flight.
@@ -339,46 +340,118 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
 namespace {
 constexpr int kMP = 112;           // padded M (receiving sub-facet j)
 constexpr int kKP = 112;           // padded K (emitting sub-facet i)
-constexpr int kLdsK = 116;         // LDS row stride in halves (bank spread)
+// LDS row stride in halves: 60 dwords, so the 8 rows of one 16-byte fragment
+// read (8 lanes) start 4 banks apart and cover all 32 banks
+constexpr int kLdsK = 120;
 constexpr int kNP = BTPM_CRATERS_WG * BTPM_BLK;  // 80 columns of the product
-constexpr int kQCols = 1024;       // Q row length: 1000 columns + 24 idle lanes
+// Q row stride: 1000 columns + 24 idle lanes + 4 floats of padding, so the
+// 8 step rows start in different LDS banks and the product's float4 stores
+// for steps 0..7 of one column (8 lanes) do not collide on one bank
+constexpr int kQCols = 1028;
 constexpr int kMT = kMP / 16;      // 7 M tiles
 constexpr int kNT = kNP / 16;      // 5 N tiles
-constexpr int kKT = kKP / 16;      // 7 K chunks
 
 // Q[n][m] = sum_k A[m][k] B[n][k] over the padded 112 x 112 x 80 tile held in
-// LDS (A = VF^T hi/lo, B = E hi/lo), 7 x 5 output tiles spread over the 16
-// waves.  Q is written column-major for the conduction lanes: product column
-// n = crater * 8 + step goes to lds_q[step][crater * 100 + m], m < 100.  Fragment layout of v_mfma_f32_16x16x16f16: lane l supplies
-// A[m = l%16][k = 4(l/16) .. +3] and B[k = 4(l/16) .. +3][n = l%16] and
-// receives D[m = 4(l/16) + i][n = l%16], i = 0..3.
+// LDS (A = VF^T hi/lo, B = E hi/lo).  Q is written column-major for the
+// conduction lanes: product column n = crater * 8 + step goes to
+// lds_q[step][crater * 100 + m], m < 100.  Output layout of the 16 x 16 MFMAs:
+// lane l receives D[m = 4(l/16) + i][n = l%16], i = 0..3.
+//
+// The phase is bound by the latency of its LDS fragment reads, so a wave
+// computes a strip of one m tile and 3 (or 2) n tiles, reading each A fragment
+// once per K chunk for the whole strip; K runs as 3 chunks of 32
+// (v_mfma_f32_16x16x32_f16, 16-byte fragment reads) plus the tail k = 96..99
+// as one 16x16x16 chunk in which only the 16 lanes holding sub-facets read.
+// The 14 strips (7 m tiles x {n 0-2, n 3-4}) sit on the waves so that each
+// SIMD (waves s, s+4, s+8, s+12) carries at most 9 of the 35 output tiles;
+// waves 12 and 13 have none.  Within a lane, fragment slot j of A and of B
+// hold the same k, which is all the product needs of the K layout.
+//
+// One K chunk of 32 (v_mfma_f32_16x16x32_f16): lane l supplies
+// A[m = l%16][k = 8(l/16) .. +7] and B[k = 8(l/16) .. +7][n = l%16].
+template <int NNT>
+__device__ __forceinline__ void strip_chunk32(const _Float16 (*lds_a)[kMP * kLdsK],
+                                              const _Float16 (*lds_b)[kNP * kLdsK], int am, int k0,
+                                              int nt0, int r, float4_t (&acc)[NNT]) {
+  const half8_t ahi = *reinterpret_cast<const half8_t*>(&lds_a[0][am * kLdsK + k0]);
+  const half8_t alo = *reinterpret_cast<const half8_t*>(&lds_a[1][am * kLdsK + k0]);
+#pragma unroll
+  for (int t = 0; t < NNT; ++t) {
+    const int bn = (nt0 + t) * 16 + r;
+    const half8_t bhi = *reinterpret_cast<const half8_t*>(&lds_b[0][bn * kLdsK + k0]);
+    const half8_t blo = *reinterpret_cast<const half8_t*>(&lds_b[1][bn * kLdsK + k0]);
+    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, acc[t], 0, 0, 0);
+    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blo, acc[t], 0, 0, 0);
+    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, acc[t], 0, 0, 0);
+  }
+}
+
+template <int NNT>
+__device__ __forceinline__ void product_strip(const _Float16 (*lds_a)[kMP * kLdsK],
+                                              const _Float16 (*lds_b)[kNP * kLdsK],
+                                              float* lds_q, int mt, int nt0, int lane,
+                                              float unscale) {
+  constexpr int kFull = BTPM_NS / 32 * 32;  // 96: sub-facets in full chunks of 32
+  static_assert(BTPM_NS - kFull == 4 && kKP >= kFull + 16, "one partial K chunk of 4");
+  float4_t acc[NNT];
+#pragma unroll
+  for (int t = 0; t < NNT; ++t) acc[t] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  const int r = lane & 15;
+  const int g = lane >> 4;
+  const int am = mt * 16 + r;
+#pragma unroll 1
+  for (int kt = 0; kt < kFull / 32; ++kt)
+    strip_chunk32<NNT>(lds_a, lds_b, am, kt * 32 + 8 * g, nt0, r, acc);
+  // the tail k = 96..99 as one v_mfma_f32_16x16x16f16 chunk (k = 4(l/16)..+3):
+  // only lanes 0-15 hold sub-facets, the others contribute zero operands
+  // instead of reading the zero padding
+  {
+    const half4_t zero = {0, 0, 0, 0};
+    half4_t ahi = zero, alo = zero, bhi[NNT], blo[NNT];
+#pragma unroll
+    for (int t = 0; t < NNT; ++t) bhi[t] = blo[t] = zero;
+    if (g == 0) {
+      ahi = *reinterpret_cast<const half4_t*>(&lds_a[0][am * kLdsK + kFull]);
+      alo = *reinterpret_cast<const half4_t*>(&lds_a[1][am * kLdsK + kFull]);
+#pragma unroll
+      for (int t = 0; t < NNT; ++t) {
+        const int bn = (nt0 + t) * 16 + r;
+        bhi[t] = *reinterpret_cast<const half4_t*>(&lds_b[0][bn * kLdsK + kFull]);
+        blo[t] = *reinterpret_cast<const half4_t*>(&lds_b[1][bn * kLdsK + kFull]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NNT; ++t) {
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo, bhi[t], acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, blo[t], acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, bhi[t], acc[t], 0, 0, 0);
+    }
+  }
+  const int m0 = mt * 16 + 4 * g;  // 4 receiving sub-facets m0 .. m0 + 3
+  if (m0 < BTPM_NS) {
+#pragma unroll
+    for (int t = 0; t < NNT; ++t) {
+      const int bn = (nt0 + t) * 16 + r;
+      float* q = &lds_q[(bn & (BTPM_BLK - 1)) * kQCols + (bn / BTPM_BLK) * BTPM_NS + m0];
+      *reinterpret_cast<float4*>(q) = make_float4(acc[t][0] * unscale, acc[t][1] * unscale,
+                                                  acc[t][2] * unscale, acc[t][3] * unscale);
+    }
+  }
+}
+
+// wave -> strip, one nibble per wave: 0..6 = m tile with n tiles 0-2,
+// 7..13 = m tile (value - 7) with n tiles 3-4, 15 = none
+constexpr unsigned long long kStripOfWave = 0xD9FFC852B741A630ull;
+
 __device__ __forceinline__ void crater_product(const _Float16 (*lds_a)[kMP * kLdsK],
                                                const _Float16 (*lds_b)[kNP * kLdsK],
                                                float* lds_q, int wave, int lane, float unscale) {
-  for (int pt = wave; pt < kMT * kNT; pt += BTPM_CRATER_THREADS / 64) {
-    const int mt = pt / kNT, nt = pt - (pt / kNT) * kNT;
-    float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    const int am = mt * 16 + (lane & 15);
-    const int bn = nt * 16 + (lane & 15);
-    const int kq = 4 * (lane >> 4);
-#pragma unroll
-    for (int kt = 0; kt < kKT; ++kt) {
-      const int k0 = kt * 16 + kq;
-      const half4_t ahi = *reinterpret_cast<const half4_t*>(&lds_a[0][am * kLdsK + k0]);
-      const half4_t alo = *reinterpret_cast<const half4_t*>(&lds_a[1][am * kLdsK + k0]);
-      const half4_t bhi = *reinterpret_cast<const half4_t*>(&lds_b[0][bn * kLdsK + k0]);
-      const half4_t blo = *reinterpret_cast<const half4_t*>(&lds_b[1][bn * kLdsK + k0]);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(alo, bhi, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, blo, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi, bhi, acc, 0, 0, 0);
-    }
-    const int m0 = mt * 16 + kq;  // 4 receiving sub-facets m0 .. m0 + 3
-    if (m0 < BTPM_NS) {
-      float* q = &lds_q[(bn & (BTPM_BLK - 1)) * kQCols + (bn / BTPM_BLK) * BTPM_NS + m0];
-      *reinterpret_cast<float4*>(q) =
-          make_float4(acc[0] * unscale, acc[1] * unscale, acc[2] * unscale, acc[3] * unscale);
-    }
-  }
+  static_assert(kMT == 7 && kNT == 5 && BTPM_CRATER_THREADS == 1024, "strip table shape");
+  const int s = (int)((kStripOfWave >> (4 * wave)) & 15ull);
+  if (s < kMT)
+    product_strip<3>(lds_a, lds_b, lds_q, s, 0, lane, unscale);
+  else if (s < 2 * kMT)
+    product_strip<2>(lds_a, lds_b, lds_q, s - kMT, 3, lane, unscale);
 }
 
 // Stage VF^T hi/lo ([2][112][112] fp16 in global memory) into LDS.
