@@ -462,7 +462,7 @@ namespace {
 constexpr int kSHRows = BTPM_SH_ROWS;    // receivers per workgroup (one per lane)
 constexpr int kSHChunk = BTPM_SH_CHUNK;  // emitters per staged chunk
 constexpr int kSHSteps = BTPM_SH_STEPS;  // steps per workgroup
-constexpr int kSHStride = kSHSteps + 4;  // LDS row stride in floats
+constexpr int kSHStride = BTPM_SH_STRIDE;  // LDS row stride in floats
 }  // namespace
 
 extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
@@ -509,6 +509,100 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
           acc[q + 2] = fmaf(v, e4.z, acc[q + 2]);
           acc[q + 3] = fmaf(v, e4.w, acc[q + 3]);
         }
+      }
+    }
+  }
+  if (!live) return;
+  const float* __restrict__ src = S.src;
+  float* __restrict__ y = S.qext;
+#pragma unroll
+  for (int bq = 0; bq < kSHSteps / BTPM_BLK; ++bq) {
+    if (s0 + bq * BTPM_BLK >= npad) break;
+    const size_t o = ((size_t)(s0 / BTPM_BLK + bq) * nf + f) * BTPM_BLK;
+    const float4 a = *reinterpret_cast<const float4*>(src + o);
+    const float4 b = *reinterpret_cast<const float4*>(src + o + 4);
+    const int q = bq * BTPM_BLK;
+    *reinterpret_cast<float4*>(y + o) =
+        make_float4(a.x + acc[q], a.y + acc[q + 1], a.z + acc[q + 2], a.w + acc[q + 3]);
+    *reinterpret_cast<float4*>(y + o + 4) =
+        make_float4(b.x + acc[q + 4], b.y + acc[q + 5], b.z + acc[q + 6], b.w + acc[q + 7]);
+  }
+}
+
+// The same product in the lane-interleaved layout: for receiver block rb,
+// emitter chunk c and wave w, wseg[(rb * nchunk + c) * 4 + w] = (first, n)
+// and entry k of lane l is ent[first + 64 k + l] = (LDS offset of its emitter
+// row in the staged chunk, view-factor bits), lanes with fewer than n
+// entries padded with (0, +0.0f).  Each entry load of a wave is one 512-byte
+// contiguous read whose address is known ahead, so the next two are
+// prefetched; each lane still adds its entries in emitter order, so the sums
+// equal btpm_selfheat_tiled's bit for bit.  The next chunk's emission rows
+// are fetched into registers while the current chunk is consumed.
+extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell(
+    const int2* __restrict__ wseg, const int2* __restrict__ ent, int nf, int nchunk,
+    const int* __restrict__ slots, const btpm_set_dev* __restrict__ sets,
+    const btpm_status_dev* __restrict__ status) {
+  constexpr int kQ4 = kSHSteps / 4;                  // float4 per emission row
+  constexpr int kPer = kSHChunk * kQ4 / kSHRows;     // float4 staged per lane
+  static_assert(kSHChunk * kQ4 % kSHRows == 0 && kSHRows % kQ4 == 0, "staging shape");
+  __shared__ __align__(16) float E[kSHChunk * kSHStride];
+  const int slot = slots[blockIdx.z];
+  if (status[slot].done) return;
+  const btpm_set_dev& S = sets[slot];
+  const int npad = S.nb * BTPM_BLK;
+  const int s0 = blockIdx.y * kSHSteps;
+  if (s0 >= npad) return;
+  const int tid = (int)threadIdx.x;
+  const int f = blockIdx.x * kSHRows + tid;
+  const bool live = f < nf && (S.inst || !S.frozen[f]);
+  if (__syncthreads_count(live) == 0) return;  // every receiver here is frozen
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const bool wave_live = __ballot(live) != 0;
+  const float* __restrict__ emis = S.emis;
+  // staging: lane tid holds rows tid / kQ4 + i * (kSHRows / kQ4), float4 column tid % kQ4
+  const int srow = tid / kQ4, sq4 = 4 * (tid % kQ4);
+  const bool scol = s0 + sq4 < npad;
+  float4 pre[kPer];
+  auto fetch = [&](int c) {
+    const int g0 = c * kSHChunk;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int g = g0 + srow + i * (kSHRows / kQ4);
+      pre[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (g < nf && scol) pre[i] = *reinterpret_cast<const float4*>(emis + (size_t)g * npad + s0 + sq4);
+    }
+  };
+  float acc[kSHSteps];
+#pragma unroll
+  for (int q = 0; q < kSHSteps; ++q) acc[q] = 0.0f;
+  fetch(0);
+  for (int c = 0; c < nchunk; ++c) {
+    __syncthreads();  // the previous chunk has been consumed
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+      *reinterpret_cast<float4*>(&E[(srow + i * (kSHRows / kQ4)) * kSHStride + sq4]) = pre[i];
+    if (c + 1 < nchunk) fetch(c + 1);
+    __syncthreads();
+    const int2 sg = wseg[((size_t)blockIdx.x * nchunk + c) * BTPM_SH_WAVES + wave];
+    const int n = wave_live ? sg.y : 0;
+    const int2* __restrict__ e = ent + (size_t)sg.x + lane;
+    int2 g1 = make_int2(0, 0), g2 = make_int2(0, 0);
+    if (n > 0) g1 = e[0];
+    if (n > 1) g2 = e[64];
+    for (int k = 0; k < n; ++k) {
+      const int2 gv = g1;
+      g1 = g2;
+      if (k + 2 < n) g2 = e[(size_t)(k + 2) * 64];
+      const float v = __int_as_float(gv.y);
+      const float* er = &E[gv.x];
+#pragma unroll
+      for (int q = 0; q < kSHSteps; q += 4) {
+        const float4 e4 = *reinterpret_cast<const float4*>(er + q);
+        acc[q] = fmaf(v, e4.x, acc[q]);
+        acc[q + 1] = fmaf(v, e4.y, acc[q + 1]);
+        acc[q + 2] = fmaf(v, e4.z, acc[q + 2]);
+        acc[q + 3] = fmaf(v, e4.w, acc[q + 3]);
       }
     }
   }

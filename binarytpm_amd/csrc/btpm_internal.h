@@ -23,6 +23,11 @@
 #define BTPM_SH_ROWS 256
 #define BTPM_SH_CHUNK 256
 #define BTPM_SH_STEPS 64
+#define BTPM_SH_STRIDE (BTPM_SH_STEPS + 4)  // LDS row stride of a staged chunk, floats
+#define BTPM_SH_WAVES (BTPM_SH_ROWS / 64)     // waves per self-heating workgroup
+// the lane-interleaved (ELL) self-heating layout is used while its padded
+// entry count stays within this multiple of the view-factor count
+#define BTPM_SH_ELL_MAX_PAD 4
 // largest accepted r = dtau/dz^2 (FTCS is stable for r <= 1/2): the
 // conduction kernels integrate the depth profile in a variable scaled by
 // (1 - 2r)^-k over 8-step blocks, which must stay inside the FP32 range

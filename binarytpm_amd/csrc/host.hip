@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -180,6 +181,12 @@ struct btpm_shape {
   // emitter chunk of BTPM_SH_CHUNK the first entry of every row, cptr[c][f]
   DBuf sent, cptr;  // sent: (emitter, float bits of VF) pairs
   int nchunk = 0;
+  // the same entries lane-interleaved per (receiver block, chunk, wave) for
+  // btpm_selfheat_ell: wseg = (first entry, entries per lane), ell = (LDS
+  // offset of the emitter row, float bits of VF); used while the padding
+  // stays within BTPM_SH_ELL_MAX_PAD times the view-factor count
+  DBuf wseg, ell;
+  bool use_ell = false;
   DBuf cvf32;      // [100][100] crater VF (fp32, vf[i*100+j] = VF(j->i))
   DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
   DBuf cillum;     // [nb][nf*100][BLK] crater illumination (after scattering)
@@ -618,9 +625,14 @@ struct Sched {
         btpm_shape* sh = g.shape;
         dim3 grid(grid1(sh->nf, BTPM_SH_ROWS), grid1((size_t)sh->nb * BTPM_BLK, BTPM_SH_STEPS),
                   g.nsets);
-        hipLaunchKernelGGL(btpm_selfheat_tiled, grid, dim3(BTPM_SH_ROWS), 0, st, sh->cptr.as<int>(),
-                           sh->sent.as<int2>(), sh->nf, sh->nchunk,
-                           g.slots->as<int>(), ctx->d_sets, ctx->d_status);
+        if (sh->use_ell)
+          hipLaunchKernelGGL(btpm_selfheat_ell, grid, dim3(BTPM_SH_ROWS), 0, st,
+                             sh->wseg.as<int2>(), sh->ell.as<int2>(), sh->nf, sh->nchunk,
+                             g.slots->as<int>(), ctx->d_sets, ctx->d_status);
+        else
+          hipLaunchKernelGGL(btpm_selfheat_tiled, grid, dim3(BTPM_SH_ROWS), 0, st,
+                             sh->cptr.as<int>(), sh->sent.as<int2>(), sh->nf, sh->nchunk,
+                             g.slots->as<int>(), ctx->d_sets, ctx->d_status);
       }
     } else if (ph == 1) {
       if (n_tiles_s)
@@ -937,7 +949,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     upload(sh->colidx, body->colidx, body->nnz, st);
     upload(sh->vf, body->vf, body->nnz, st);
     {
-      constexpr int kChunk = BTPM_SH_CHUNK;  // emitter chunk of btpm_selfheat_tiled
+      constexpr int kChunk = BTPM_SH_CHUNK;  // emitter chunk of the self-heating kernels
       const int nc = (nf + kChunk - 1) / kChunk;
       std::vector<int> se(2 * (size_t)body->nnz), cp((size_t)(nc + 1) * nf);
       std::vector<std::pair<int, float>> row;
@@ -960,8 +972,57 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
         }
       }
       sh->nchunk = nc;
-      upload(sh->sent, se.data(), se.size(), st);
-      upload(sh->cptr, cp.data(), cp.size(), st);
+      // lane-interleaved copy: one segment per (receiver block, chunk, wave)
+      const int nrb = (nf + BTPM_SH_ROWS - 1) / BTPM_SH_ROWS;
+      std::vector<int> ws(2 * (size_t)nrb * nc * BTPM_SH_WAVES);
+      size_t total = 0;
+      for (int rb = 0; rb < nrb; ++rb)
+        for (int c = 0; c < nc; ++c)
+          for (int w = 0; w < BTPM_SH_WAVES; ++w) {
+            int n = 0;
+            for (int l = 0; l < 64; ++l) {
+              const int f = rb * BTPM_SH_ROWS + w * 64 + l;
+              if (f < nf)
+                n = std::max(n, cp[(size_t)(c + 1) * nf + f] - cp[(size_t)c * nf + f]);
+            }
+            const size_t i = ((size_t)rb * nc + c) * BTPM_SH_WAVES + w;
+            ws[2 * i] = (int)total;
+            ws[2 * i + 1] = n;
+            total += (size_t)n * 64;
+          }
+      // BTPM_SH_LAYOUT=csr|ell overrides the choice (diagnostics and tests)
+      const char* lay = std::getenv("BTPM_SH_LAYOUT");
+      const bool fits = total < (size_t)INT32_MAX;
+      if (lay && std::strcmp(lay, "csr") == 0)
+        sh->use_ell = false;
+      else if (lay && std::strcmp(lay, "ell") == 0)
+        sh->use_ell = fits;
+      else
+        sh->use_ell = fits && total <= (size_t)BTPM_SH_ELL_MAX_PAD * body->nnz;
+      if (!sh->use_ell) {
+        upload(sh->sent, se.data(), se.size(), st);
+        upload(sh->cptr, cp.data(), cp.size(), st);
+      } else {
+        std::vector<int> el(2 * total, 0);
+        for (int rb = 0; rb < nrb; ++rb)
+          for (int c = 0; c < nc; ++c)
+            for (int w = 0; w < BTPM_SH_WAVES; ++w) {
+              const size_t i = ((size_t)rb * nc + c) * BTPM_SH_WAVES + w;
+              const size_t first = (size_t)ws[2 * i];
+              for (int l = 0; l < 64; ++l) {
+                const int f = rb * BTPM_SH_ROWS + w * 64 + l;
+                if (f >= nf) continue;
+                const int k0 = cp[(size_t)c * nf + f], k1 = cp[(size_t)(c + 1) * nf + f];
+                for (int k = k0; k < k1; ++k) {
+                  const size_t o = first + (size_t)(k - k0) * 64 + l;
+                  el[2 * o] = (se[2 * (size_t)k] - c * kChunk) * BTPM_SH_STRIDE;
+                  el[2 * o + 1] = se[2 * (size_t)k + 1];
+                }
+              }
+            }
+        upload(sh->wseg, ws.data(), ws.size(), st);
+        upload(sh->ell, el.data(), el.size(), st);
+      }
     }
   }
   // facet illumination -> blocked

@@ -81,9 +81,12 @@ def test_smooth_inst_parity(ctx):
     assert _report("smooth TI=0", s.surface_temps(), ref.stemp) <= TOL_K
 
 
-def test_self_heating_super_cycle_parity(ctx):
+@pytest.mark.parametrize("layout", ["ell", "csr"])
+def test_self_heating_super_cycle_parity(ctx, monkeypatch, layout):
     """Binary-like body: 2:1 super-cycle (numrevs 2 and 1 on ntau = 1300),
-    sparse facet view factors, facet multiple scattering."""
+    sparse facet view factors, facet multiple scattering; both self-heating
+    kernel layouts (lane-interleaved ELL and per-chunk CSR)."""
+    monkeypatch.setenv("BTPM_SH_LAYOUT", layout)
     nt = 1300
     Vp, Fp = inputs.ellipsoid(1, (0.6, 0.5, 0.45))
     Vs, Fs = inputs.ellipsoid(0, (0.3, 0.3, 0.3))
@@ -101,6 +104,51 @@ def test_self_heating_super_cycle_parity(ctx):
     s = th.Solve(ctx, shape, 150.0).run()
     ref = O.smooth_solve(O.params(nt, P, tacc=0.01), F_ref, 0.1, 150.0, numrevs=numrevs, csr=csr)
     assert _report("binary self-heating", s.surface_temps(), ref.stemp) <= TOL_K
+
+
+def _skewed_viewfactors(nf, seed=11):
+    """Rows with 5 random emitters, except every 97th row, which sees every
+    other facet: per-wave entry counts differ by up to 50x, so the
+    lane-interleaved layout would pad far beyond its limit."""
+    rng = np.random.default_rng(seed)
+    rowptr, cols, vals = [0], [], []
+    for f in range(nf):
+        if f % 97 == 0:
+            c = np.delete(np.arange(nf), f)
+            v = np.full(c.size, 0.5 / c.size)
+        else:
+            c = rng.choice(nf - 1, size=5, replace=False)
+            c = np.sort(np.where(c >= f, c + 1, c))
+            v = rng.uniform(0.01, 0.1, size=5)
+        cols.append(c)
+        vals.append(v)
+        rowptr.append(rowptr[-1] + c.size)
+    return (np.asarray(rowptr, np.int32), np.concatenate(cols).astype(np.int32),
+            np.concatenate(vals).astype(np.float32))
+
+
+def test_self_heating_layouts_bitwise(ctx, monkeypatch):
+    """1280 facets (5 receiver blocks x 5 emitter chunks) with skewed view
+    factor rows: the automatic choice (per-chunk CSR, since the ELL padding
+    would exceed its limit) and the forced ELL layout add each row's entries
+    in the same order, so their temperatures agree bit for bit; both match
+    the oracle."""
+    case, _ = inputs.serving_shape(3, 5)
+    nf = case.illum.shape[0]
+    csr = _skewed_viewfactors(nf)
+    p = th.params(case.ntau, case.rot_period, tacc=0.01)
+    out = {}
+    for layout in (None, "ell"):
+        if layout is None:
+            monkeypatch.delenv("BTPM_SH_LAYOUT", raising=False)
+        else:
+            monkeypatch.setenv("BTPM_SH_LAYOUT", layout)
+        shape = th.Shape(ctx, p, case.illum, 0.1, csr=csr, scatter=1)
+        out[layout] = th.Solve(ctx, shape, 200.0).run().surface_temps()
+    assert np.array_equal(out[None], out["ell"])
+    F = O.facet_scatter(case.illum, 0.1, csr, multiple=False)
+    ref = O.smooth_solve(O.params(case.ntau, case.rot_period, tacc=0.01), F, 0.1, 200.0, csr=csr)
+    assert _report("skewed self-heating", out["ell"], ref.stemp) <= TOL_K
 
 
 @pytest.fixture(scope="module")
