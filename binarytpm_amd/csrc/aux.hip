@@ -113,9 +113,13 @@ extern "C" __global__ void btpm_init_smooth(btpm_set_dev S) {
 // roughness_initialise_temperatures, T175706/thermal_GPU.cu:374-423: two
 // passes per crater; src = (1-A) illum + RGH (blocked).  vf32[i*NS+j] =
 // VF(j->i).  One workgroup (128 lanes) per parent facet.
+// Memory traffic is coalesced: a lane reads and writes its column's 8-step
+// blocks as float4 pairs, and the facet's 100 profile records (contiguous,
+// 64 floats each) are written by the whole workgroup as one float4 stream.
 extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev S,
                                                                    const float* __restrict__ vf32) {
   __shared__ float e1[BTPM_NS];
+  __shared__ float tcol[BTPM_NS];
   const int f = blockIdx.x;
   const int j = threadIdx.x;
   const bool ok = j < BTPM_NS;
@@ -123,23 +127,35 @@ extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev 
   double sum = 0.0;
   if (ok) {
     for (int b = 0; b < S.nb; ++b) {
-      const float* p = S.src + ((size_t)b * S.ncols + c) * BTPM_BLK;
+      const float4* p = reinterpret_cast<const float4*>(S.src + ((size_t)b * S.ncols + c) * BTPM_BLK);
+      const float4 x = p[0], y = p[1];
+      const float v[BTPM_BLK] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
       const int kend = min(BTPM_BLK, S.ntau - b * BTPM_BLK);
-      for (int k = 0; k < kend; ++k) sum += (double)p[k];
+      for (int k = 0; k < kend; ++k) sum += (double)v[k];
     }
   }
   const double mean = sum / S.ntau;
   const double t1 = pow(mean / (double)S.es, 0.25);
   if (ok) e1[j] = (float)((double)S.esia * t1 * t1 * t1 * t1);
   __syncthreads();
-  if (!ok) return;
-  double self = 0.0;
-  for (int i = 0; i < BTPM_NS; ++i) self += (double)e1[i] * (double)vf32[i * BTPM_NS + j];
-  const float t = (float)pow((mean + self) / (double)S.es, 0.25);
-  for (int i = 0; i < BTPM_PSTRIDE; ++i) S.prof[(size_t)c * BTPM_PSTRIDE + i] = t;
-  for (int b = 0; b < S.nb; ++b) {
-    float* p = S.T + ((size_t)b * S.ncols + c) * BTPM_BLK;
-    for (int k = 0; k < BTPM_BLK; ++k) p[k] = t;
+  if (ok) {
+    double self = 0.0;
+    for (int i = 0; i < BTPM_NS; ++i) self += (double)e1[i] * (double)vf32[i * BTPM_NS + j];
+    const float t = (float)pow((mean + self) / (double)S.es, 0.25);
+    tcol[j] = t;
+    const float4 t4 = make_float4(t, t, t, t);
+    for (int b = 0; b < S.nb; ++b) {
+      float4* p = reinterpret_cast<float4*>(S.T + ((size_t)b * S.ncols + c) * BTPM_BLK);
+      p[0] = t4;
+      p[1] = t4;
+    }
+  }
+  __syncthreads();
+  static_assert(BTPM_PSTRIDE % 4 == 0, "float4 profile records");
+  float4* rec = reinterpret_cast<float4*>(S.prof + (size_t)f * BTPM_NS * BTPM_PSTRIDE);
+  for (int e = j; e < BTPM_NS * BTPM_PSTRIDE / 4; e += (int)blockDim.x) {
+    const float t = tcol[e / (BTPM_PSTRIDE / 4)];
+    rec[e] = make_float4(t, t, t, t);
   }
   if (j == 0) {
     S.frozen[f] = 0;
