@@ -174,108 +174,199 @@ extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev 
 // Workgroup: 256 lanes = 4 waves per (facet, 64-step tile), on a 1-D grid
 // mapped so that all tiles of a facet run on one XCD (workgroups are dealt
 // to the 8 XCDs round-robin by id): the facet's drf rows, re-read by each of
-// its tiles, then come from that XCD's L2 instead of HBM.  The facet's
-// entries are staged in chunks of 16 (drf rows and weight rows in LDS) and
-// multiplied on the matrix cores with v_mfma_f32_16x16x4_f32 (FP32 products,
-// FP32 accumulation): M = 100 padded to 112 (7 tiles), N = 64 (4 tiles),
-// wave w owning output tiles w, w+4, ... of the 28.  Fragment layout: lane l
-// supplies A[m = l%16][k = l/16] and B[k = l/16][n = l%16] and receives
-// D[m = 4(l/16) + i][n = l%16], i = 0..3.
+// its tiles, then come from that XCD's L2 instead of HBM.
+//
+// The product runs on v_mfma_f32_16x16x32_f16 with the three-product split
+//   drf w ~ drf_hi w_hi + drf_hi w_lo + drf_lo w_hi     (FP32 accumulation),
+// x_hi = fp16(x), x_lo = fp16(x - x_hi); drf is multiplied by the facet's
+// power-of-two drf_scale (largest |drf| of the facet in [0.5, 1)) so its low
+// halves stay normal fp16 numbers, and the accumulator by 1 / drf_scale,
+// which is exact.  The facet's entries go through LDS in chunks of 32 as
+// drf^T [m = sub-facet][k] and w^T [n = step][k] hi/lo halves: the drf^T
+// tiles are packed once per shape (btpm_pack_drf) and copied in 16-byte
+// pieces, the w^T halves are split here, a thread writing 8 consecutive k of
+// one step as one 16-byte store (the gathers stay coalesced: lanes run over
+// steps); the next chunk is loaded into registers while the current one is
+// on the matrix cores.  Wave
+// w owns the 16 steps of n tile w and all 7 m tiles (M = 100 padded to 112),
+// so each w fragment is read once per chunk.  Fragment layout: lane l
+// supplies A[m = l%16][k = 8(l/16) .. +7] and B[k = 8(l/16) .. +7][n = l%16]
+// and receives D[m = 4(l/16) + i][n = l%16], i = 0..3.
 // ---------------------------------------------------------------------------
+namespace {
+constexpr int kRK = 32;        // entries per K chunk (one 16x16x32 product)
+constexpr int kRS = kRK + 8;   // LDS row stride in halves: 20 dwords, so the 8
+                               // rows of one 16-byte access hit distinct bank quads
+constexpr int kRM = 112;       // padded sub-facets (7 m tiles)
+constexpr int kRN = 64;        // steps per workgroup (4 n tiles, one per wave)
+constexpr int kRDrfTile = 2 * kRM * kRK;  // halves per packed drf^T chunk tile [2][112][32]
+static_assert(kRDrfTile == BTPM_DRF_TILE, "packed drf tile size");
+typedef _Float16 rhalf8 __attribute__((ext_vector_type(8)));
+typedef float rfloat4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split8(const float (&v)[8], float sc, rhalf8& hi, rhalf8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const float x = v[i] * sc;
+    const _Float16 h = (_Float16)x;
+    hi[i] = h;
+    lo[i] = (_Float16)(x - (float)h);
+  }
+}
+}  // namespace
+
 extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
-    const int* __restrict__ rowptr, const int* __restrict__ colidx, const float* __restrict__ drf,
-    const float* __restrict__ w, int mode, float es, int nsmooth, int nfac, int ntau, int nb,
-    const float* __restrict__ base, const float* __restrict__ base_scale, float base_gain,
-    float* __restrict__ dst) {
-  constexpr int KC = 16;
-  constexpr int kMT = 7, kNT = 4, kTiles = kMT * kNT;
-  constexpr int kDS = 116, kWS = 68;  // LDS row strides (floats)
-  __shared__ float ds[KC][kDS];       // drf chunk, j padded to 112 with zeros
-  __shared__ float ws[KC][kWS];       // weight chunk over the 64 steps
-  const int ntile = (nb * BTPM_BLK + 63) / 64;
+    const int* __restrict__ rowptr, const int* __restrict__ colidx,
+    const _Float16* __restrict__ drfA, const int* __restrict__ cbase,
+    const float* __restrict__ drf_scale, const float* __restrict__ w, int mode, float es,
+    int nsmooth, int nfac, int ntau, int nb, const float* __restrict__ base,
+    const float* __restrict__ base_scale, float base_gain, float* __restrict__ dst) {
+  constexpr int kMT = kRM / 16;
+  constexpr int kAPieces = kRDrfTile / 8;  // 16-byte pieces of a packed drf^T tile: 896
+  constexpr int kAPer = (kAPieces + 255) / 256;
+  static_assert(kRN * (kRK / 8) == 256, "staging shape");
+  __shared__ __align__(16) _Float16 la[2][kRM * kRS];  // drf^T hi / lo
+  __shared__ __align__(16) _Float16 lb[2][kRN * kRS];  // w^T hi / lo
+  const int ntile = (nb * BTPM_BLK + kRN - 1) / kRN;
   const int xcd = (int)(blockIdx.x & 7), local = (int)(blockIdx.x >> 3);
   const int f = 8 * (local / ntile) + xcd;
-  const int s0 = 64 * (local % ntile);
+  const int s0 = kRN * (local % ntile);
   if (f >= nfac) return;
   const int tid = (int)threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int lm = lane & 15, lk = lane >> 4;
+  const int r = lane & 15, g = lane >> 4;
   const int ncols = nfac * BTPM_NS;
+  const float unsc = 1.0f / drf_scale[f];  // exact: a power of two
+  const rhalf8* __restrict__ tiles = reinterpret_cast<const rhalf8*>(drfA) +
+                                     (size_t)cbase[f] * (kRDrfTile / 8);
   const float bsc = base_gain * (base_scale ? base_scale[f] : 1.0f);
   const int k0 = rowptr[f], k1 = rowptr[f + 1];
-  typedef float float4v __attribute__((ext_vector_type(4)));
-  float4v acc[(kTiles + 3) / 4];
-#pragma unroll
-  for (int t = 0; t < (kTiles + 3) / 4; ++t) acc[t] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
-  // the next chunk is loaded into registers while the current one is on the
-  // matrix cores: thread tid stages drf elements tid + 256 i (i < 7) and
-  // weight elements tid + 256 i (i < 4) of every chunk
-  constexpr int kRD = KC * 112 / 256, kRW = KC * 64 / 256;
-  float rd[kRD], rw[kRW];
+  // staging roles: drf^T tile pieces p = tid + 256 q (a straight copy of the
+  // packed tile); w^T row tid % 64 (a step), group tid / 64 = wave
+  const int sb = min(s0 + (tid & (kRN - 1)), ntau - 1);
+  const size_t wofs = (size_t)(sb / BTPM_BLK) * nsmooth * BTPM_BLK + (sb % BTPM_BLK);
+  rhalf8 ra[kAPer];
+  float rb[8];
   auto fetch = [&](int kb) {
-    const int kn = min(KC, k1 - kb);
+    const int kn = min(kRK, k1 - kb);
+    const rhalf8* __restrict__ tile = tiles + (size_t)((kb - k0) / kRK) * (kRDrfTile / 8);
 #pragma unroll
-    for (int i = 0; i < kRD; ++i) {
-      const int e = tid + 256 * i, kk = e / 112, jj = e - (e / 112) * 112;
-      rd[i] = (kk < kn && jj < BTPM_NS) ? drf[(size_t)(kb + kk) * BTPM_NS + jj] : 0.0f;
+    for (int q = 0; q < kAPer; ++q) {
+      const int p = tid + 256 * q;
+      if (p < kAPieces) ra[q] = tile[p];
     }
 #pragma unroll
-    for (int i = 0; i < kRW; ++i) {
-      const int e = tid + 256 * i, kk = e >> 6, ss = min(s0 + (e & 63), ntau - 1);
-      rw[i] = 0.0f;
-      if (kk < kn) {
-        const int g = colidx[kb + kk];
-        rw[i] = w[((size_t)(ss / BTPM_BLK) * nsmooth + g) * BTPM_BLK + (ss % BTPM_BLK)];
-      }
+    for (int i = 0; i < 8; ++i) {
+      const int k = 8 * wave + i;
+      rb[i] = 0.0f;
+      if (k < kn) rb[i] = w[wofs + (size_t)colidx[kb + k] * BTPM_BLK];
     }
   };
+  rfloat4 acc[kMT];
+#pragma unroll
+  for (int t = 0; t < kMT; ++t) acc[t] = rfloat4{0.0f, 0.0f, 0.0f, 0.0f};
   if (k0 < k1) fetch(k0);
-  for (int kb = k0; kb < k1; kb += KC) {
+  for (int kb = k0; kb < k1; kb += kRK) {
     __syncthreads();  // the previous chunk's products have read LDS
 #pragma unroll
-    for (int i = 0; i < kRD; ++i) {
-      const int e = tid + 256 * i;
-      ds[e / 112][e - (e / 112) * 112] = rd[i];
-    }
-#pragma unroll
-    for (int i = 0; i < kRW; ++i) {
-      const int e = tid + 256 * i;
-      float v = rw[i];
-      if (mode == 1) v = es * (v * v) * (v * v);
-      ws[e >> 6][e & 63] = v;
-    }
-    __syncthreads();
-    if (kb + KC < k1) fetch(kb + KC);
-#pragma unroll
-    for (int t = 0; t < (kTiles + 3) / 4; ++t) {
-      const int tile = wave + 4 * t;
-      if (tile < kTiles) {
-        const int mt = tile / kNT, nt = tile - (tile / kNT) * kNT;
-#pragma unroll
-        for (int kq = 0; kq < KC; kq += 4) {
-          const float a = ds[kq + lk][mt * 16 + lm];
-          const float b = ws[kq + lk][nt * 16 + lm];
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
-        }
+    for (int q = 0; q < kAPer; ++q) {
+      const int p = tid + 256 * q;  // piece (h, m, 8-entry group) of the [2][112][32] tile
+      if (p < kAPieces) {
+        const int h = p / (kAPieces / 2), rem = p - h * (kAPieces / 2);
+        *reinterpret_cast<rhalf8*>(&la[h][(rem / (kRK / 8)) * kRS + 8 * (rem % (kRK / 8))]) = ra[q];
       }
     }
-  }
+    {
+      float v[8];
 #pragma unroll
-  for (int t = 0; t < (kTiles + 3) / 4; ++t) {
-    const int tile = wave + 4 * t;
-    if (tile >= kTiles) continue;
-    const int mt = tile / kNT, nt = tile - (tile / kNT) * kNT;
-    const int s = s0 + nt * 16 + lm;
-    if (s >= nb * BTPM_BLK) continue;
+      for (int i = 0; i < 8; ++i) v[i] = mode == 1 ? es * (rb[i] * rb[i]) * (rb[i] * rb[i]) : rb[i];
+      rhalf8 hi, lo;
+      split8(v, 1.0f, hi, lo);
+      const int n = tid & (kRN - 1);
+      *reinterpret_cast<rhalf8*>(&lb[0][n * kRS + 8 * wave]) = hi;
+      *reinterpret_cast<rhalf8*>(&lb[1][n * kRS + 8 * wave]) = lo;
+    }
+    __syncthreads();
+    if (kb + kRK < k1) fetch(kb + kRK);
+    const int bn = 16 * wave + r;
+    const rhalf8 bhi = *reinterpret_cast<const rhalf8*>(&lb[0][bn * kRS + 8 * g]);
+    const rhalf8 blo = *reinterpret_cast<const rhalf8*>(&lb[1][bn * kRS + 8 * g]);
+#pragma unroll
+    for (int t = 0; t < kMT; ++t) {
+      const int am = 16 * t + r;
+      const rhalf8 ahi = *reinterpret_cast<const rhalf8*>(&la[0][am * kRS + 8 * g]);
+      const rhalf8 alo = *reinterpret_cast<const rhalf8*>(&la[1][am * kRS + 8 * g]);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blo, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, acc[t], 0, 0, 0);
+    }
+  }
+  const int s = s0 + 16 * wave + r;
+  if (s >= nb * BTPM_BLK) return;
+#pragma unroll
+  for (int t = 0; t < kMT; ++t) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int j = mt * 16 + 4 * lk + i;
+      const int j = 16 * t + 4 * g + i;
       if (j >= BTPM_NS) continue;
       const int c = f * BTPM_NS + j;
       const size_t o = ((size_t)(s / BTPM_BLK) * ncols + c) * BTPM_BLK + (s % BTPM_BLK);
-      float v = acc[t][i];
+      float v = acc[t][i] * unsc;
       if (base) v += bsc * base[o];
       dst[o] = v;
+    }
+  }
+}
+
+// Packs the shape->crater view factors once per shape for btpm_shape_to_crater:
+// facet f's entries rowptr[f] .. rowptr[f+1]-1 in chunks of 32, chunk c as the
+// tile drfA[cbase[f] + c] = [hi, lo][m = sub-facet, 112][k = entry in chunk, 32]
+// of drf * drf_scale[f], where drf_scale[f] is the power of two that puts the
+// facet's largest |drf| in [0.5, 1) (zero padding for m >= 100 and for entries
+// past the facet's last).  One workgroup per facet; each chunk goes through
+// LDS so the fp32 rows are read and the fp16 tiles written contiguously.
+extern "C" __global__ __launch_bounds__(256) void btpm_pack_drf(const int* __restrict__ rowptr,
+                                                               const float* __restrict__ drf,
+                                                               const int* __restrict__ cbase,
+                                                               float* __restrict__ drf_scale,
+                                                               _Float16* __restrict__ drfA) {
+  __shared__ float red[256];
+  __shared__ float chunk[kRK][BTPM_NS + 1];
+  const int f = blockIdx.x;
+  const int tid = (int)threadIdx.x;
+  const int k0 = rowptr[f], k1 = rowptr[f + 1];
+  float mx = 0.0f;
+  for (size_t e = (size_t)k0 * BTPM_NS + tid; e < (size_t)k1 * BTPM_NS; e += 256)
+    mx = fmaxf(mx, fabsf(drf[e]));
+  red[tid] = mx;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] = fmaxf(red[tid], red[tid + h]);
+    __syncthreads();
+  }
+  mx = red[0];
+  float sc = 1.0f;
+  if (mx > 0.0f && mx <= 3.0e38f) {
+    int ex = 0;
+    (void)frexpf(mx, &ex);  // mx = m 2^ex, m in [0.5, 1)
+    sc = ldexpf(1.0f, max(-120, min(120, -ex)));
+  }
+  if (tid == 0) drf_scale[f] = sc;
+  for (int kb = k0, c = 0; kb < k1; kb += kRK, ++c) {
+    const int kn = min(kRK, k1 - kb);
+    __syncthreads();
+    for (int e = tid; e < kRK * BTPM_NS; e += 256) {
+      const int k = e / BTPM_NS, m = e - (e / BTPM_NS) * BTPM_NS;
+      chunk[k][m] = k < kn ? drf[(size_t)(kb + k) * BTPM_NS + m] * sc : 0.0f;
+    }
+    __syncthreads();
+    _Float16* tile = drfA + (size_t)(cbase[f] + c) * kRDrfTile;
+    for (int e = tid; e < kRM * kRK; e += 256) {
+      const int m = e / kRK, k = e - (e / kRK) * kRK;
+      const float x = m < BTPM_NS ? chunk[k][m] : 0.0f;
+      const _Float16 hi = (_Float16)x;
+      tile[e] = hi;
+      tile[kRM * kRK + e] = (_Float16)(x - (float)hi);
     }
   }
 }

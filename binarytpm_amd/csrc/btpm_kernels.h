@@ -29,8 +29,11 @@ extern "C" __global__ void btpm_unpack_crater(const float* blk, float* raw, int 
                                               int nb);
 extern "C" __global__ void btpm_init_smooth(btpm_set_dev S);
 extern "C" __global__ void btpm_init_crater(btpm_set_dev S, const float* vf32);
+extern "C" __global__ void btpm_pack_drf(const int* rowptr, const float* drf, const int* cbase,
+                                         float* drf_scale, _Float16* drfA);
 extern "C" __global__ void btpm_shape_to_crater(const int* rowptr, const int* colidx,
-                                                const float* drf, const float* w, int mode,
+                                                const _Float16* drfA, const int* cbase,
+                                                const float* drf_scale, const float* w, int mode,
                                                 float es, int nsmooth, int nfac, int ntau, int nb,
                                                 const float* base, const float* base_scale,
                                                 float base_gain, float* dst);

@@ -190,7 +190,11 @@ struct btpm_shape {
   DBuf cvf32;      // [100][100] crater VF (fp32, vf[i*100+j] = VF(j->i))
   DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
   DBuf cillum;     // [nb][nf*100][BLK] crater illumination (after scattering)
-  DBuf drf;        // [nnz][100]
+  // shape->crater view factors ([nnz][100] on input) packed per facet in
+  // chunks of 32 entries as fp16 hi/lo tiles of drf * drf_scale (btpm_pack_drf)
+  DBuf drfA;       // [chunks][2][112][32] halves
+  DBuf cbase;      // [nf] first chunk of each facet
+  DBuf drf_scale;  // [nf] power of two putting the facet's largest |drf| in [0.5, 1)
   float vf_unscale = 1.0f;
   std::vector<float> h_albedo, h_numrevs;
 };
@@ -354,10 +358,11 @@ void build_crater(btpm_solve* s, hipStream_t st) {
   d.ntiles = S->ntiles;
   d.tile_cols = BTPM_CRATERS_WG;
   // src = (1-A) illum + RGH (RGH from the converged smooth T)
-  if (sh->nnz > 0 && sh->drf.p) {
+  if (sh->nnz > 0 && sh->drfA.p) {
     dim3 g(8 * grid1(sh->nf, 8) * grid1((size_t)d.nb * BTPM_BLK, 64));  // XCD-grouped facets
     hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
-                       sh->colidx.as<int>(), sh->drf.as<float>(), s->smooth->T.as<float>(), 1,
+                       sh->colidx.as<int>(), sh->drfA.as<_Float16>(), sh->cbase.as<int>(),
+                       sh->drf_scale.as<float>(), s->smooth->T.as<float>(), 1,
                        sh->p.eps * sh->p.sigma, sh->nf, sh->nf, sh->p.ntau, d.nb,
                        sh->cillum.as<float>(), sh->one_minus_a.as<float>(), s->flux_scale,
                        S->src.as<float>());
@@ -1084,7 +1089,24 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     upload(sh->cvfA, a.data(), a.size(), st);
     const size_t ncser = (size_t)nb * nf * BTPM_NS * BTPM_BLK;
     sh->cillum.alloc(ncser * sizeof(float));
-    if (crater->drf && body->nnz > 0) upload(sh->drf, crater->drf, (size_t)body->nnz * BTPM_NS, st);
+    if (crater->drf && body->nnz > 0) {
+      std::vector<int> cb(nf);
+      size_t nchunks = 0;
+      for (int f = 0; f < nf; ++f) {
+        cb[f] = (int)nchunks;
+        nchunks += (size_t)(body->rowptr[f + 1] - body->rowptr[f] + 31) / 32;
+      }
+      DBuf raw;
+      upload(raw, crater->drf, (size_t)body->nnz * BTPM_NS, st);
+      upload(sh->cbase, cb.data(), cb.size(), st);
+      sh->drf_scale.alloc((size_t)nf * sizeof(float));
+      sh->drfA.alloc(std::max<size_t>(nchunks, 1) * BTPM_DRF_TILE * sizeof(_Float16));
+      hipLaunchKernelGGL(btpm_pack_drf, dim3(nf), dim3(256), 0, st, sh->rowptr.as<int>(),
+                         raw.as<float>(), sh->cbase.as<int>(), sh->drf_scale.as<float>(),
+                         sh->drfA.as<_Float16>());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+    }
     if (crater->illum) {
       DBuf raw;
       upload(raw, crater->illum, (size_t)nf * nt * BTPM_NS, st);
@@ -1103,7 +1125,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
         upload(dcos, crater->cosang, (size_t)nf * nt * BTPM_NS, st);
       }
       const float* rgip = nullptr;
-      if (sh->drf.p) {
+      if (sh->drfA.p) {
         // RGI = sum_g drf * A_g F_g(s): blocked over crater columns, then
         // unpacked to the reference layout the scattering kernel reads
         DBuf AF;
@@ -1114,7 +1136,8 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
         rgi_blk.alloc(ncser * 4);
         dim3 g(8 * grid1(nf, 8) * grid1((size_t)nb * BTPM_BLK, 64));  // XCD-grouped facets
         hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
-                           sh->colidx.as<int>(), sh->drf.as<float>(), AF.as<float>(), 0, 0.0f, nf,
+                           sh->colidx.as<int>(), sh->drfA.as<_Float16>(), sh->cbase.as<int>(),
+                           sh->drf_scale.as<float>(), AF.as<float>(), 0, 0.0f, nf,
                            nf, nt, nb, (const float*)nullptr, (const float*)nullptr, 1.0f,
                            rgi_blk.as<float>());
         rgi.alloc((size_t)nf * nt * BTPM_NS * 4);
