@@ -151,6 +151,48 @@ def test_self_heating_layouts_bitwise(ctx, monkeypatch):
     assert _report("skewed self-heating", out["ell"], ref.stemp) <= TOL_K
 
 
+def test_config1_binary_full_size(ctx):
+    """Config 1 at full size: 5120-facet primary + 1280-facet secondary, 15:4
+    super-cycle over 9750 steps, 3 %-dense facet view factors (154 per
+    primary row), facet multiple scattering, TI 200.  A coupled solve of this
+    size is beyond the oracle's reach in a test, so the check is by pieces:
+    (i) the GPU's scattered illumination satisfies the multiple-scattering
+    fixed point F = D + VF (A F) on sampled rows; (ii) 64 sampled facets
+    (48 primary, 16 secondary) re-solved by the FP64 oracle on their own, with
+    the self-heating Q = sum_g VF es T_g^4 of the GPU's converged temperatures
+    added to their source, match the GPU's temperatures (the coupling enters
+    only through Q, which is lagged by one rotation)."""
+    c = inputs.config1()
+    nf, nt = c.illum.shape
+    p = th.params(nt, c.rot_period, tacc=0.01)
+    shape = th.Shape(ctx, p, c.illum, 0.1, numrevs=c.numrevs, csr=c.csr, scatter=2,
+                     scatter_acc=1e-4)
+    F = shape.facet_illum().astype(np.float64)
+    rp, ci, vv = c.csr
+    rng = np.random.default_rng(3)
+    samp = np.sort(np.concatenate([rng.choice(5120, 48, replace=False),
+                                   5120 + rng.choice(nf - 5120, 16, replace=False)]))
+
+    def gather(f, X):  # sum_k vf_k X[colidx_k], one facet row
+        k0, k1 = rp[f], rp[f + 1]
+        return vv[k0:k1].astype(np.float64) @ X[ci[k0:k1]] if k1 > k0 else np.zeros(nt)
+
+    resid = max(np.abs(F[f] - (c.illum[f] + gather(f, 0.1 * F))).max() for f in samp)
+    print(f"[parity] config 1 scattering fixed-point residual {resid:.2e} W/m^2", flush=True)
+    assert resid < 0.01
+    s = th.Solve(ctx, shape, 200.0).run()
+    info = s.info()
+    assert info.converged
+    T = s.surface_temps().astype(np.float64)
+    E = ES * T ** 4
+    Q = np.stack([gather(f, E) for f in samp])
+    ref = O.smooth_solve(O.params(nt, c.rot_period, tacc=0.01), F[samp] + Q / 0.9, 0.1, 200.0,
+                         numrevs=c.numrevs[samp])
+    print(f"[parity] config 1: GPU {info.rotations} cycles, oracle replay {ref.rotations}",
+          flush=True)
+    assert _report("config 1 full size, sampled facets", T[samp], ref.stemp) <= TOL_K
+
+
 @pytest.fixture(scope="module")
 def rough80():
     return inputs.rough_primary(subdiv=1)          # 80 facets x 100 sub-facets
@@ -185,6 +227,43 @@ def test_crater_parity(ctx, rough80, ti):
     assert info.converged and info.crater_converged
     assert _report(f"crater TI={ti}", s.crater_temps(), cr.stemp) <= TOL_K
     assert abs(info.crater_rotations - cr.rotations) <= 1
+
+
+def test_crater_ragged_workgroup(ctx, rough80):
+    """37 crater parents (the first 37 facets of the 80-facet body, view
+    factors restricted to them): crater workgroups of 10 + 10 + 10 + 7 craters,
+    so the last one carries 300 idle lanes from the start, and one smooth
+    workgroup of 37 of its 256 lanes."""
+    import types
+    case, crater = rough80
+    k = 37
+    rp, ci, vv = case.csr
+    rows, cols, vals, ents = [0], [], [], []
+    for f in range(k):
+        for e in range(rp[f], rp[f + 1]):
+            if ci[e] < k:
+                cols.append(ci[e])
+                vals.append(vv[e])
+                ents.append(e)
+        rows.append(len(cols))
+    csr = (np.asarray(rows, np.int32), np.asarray(cols, np.int32), np.asarray(vals, np.float32))
+    sub = types.SimpleNamespace(illum=np.ascontiguousarray(case.illum[:k]), csr=csr,
+                                rot_period=case.rot_period)
+    cr = dict(crater, shadow=np.ascontiguousarray(crater["shadow"][:k]),
+              cosang=np.ascontiguousarray(crater["cosang"][:k]),
+              drf=np.ascontiguousarray(crater["drf"][np.asarray(ents, np.int64)]))
+    p = th.params(650, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=cr["vf"], shadow=cr["shadow"], cosang=cr["cosang"], solar=cr["solar"],
+                        multiple=True, scatter_acc=1e-4, drf=cr["drf"])
+    shape = th.Shape(ctx, p, sub.illum, 0.1, csr=csr, scatter=2, crater=cm)
+    s = th.Solve(ctx, shape, 300.0, with_craters=True).run()
+    F, sm, il, ref = _oracle_rough(sub, cr, 300.0)
+    assert np.abs(shape.crater_illum() - il).max() < 0.05
+    assert _report("ragged smooth", s.surface_temps(), sm.stemp) <= TOL_K
+    info = s.info()
+    assert info.converged and info.crater_converged
+    assert _report("ragged craters (37 parents)", s.crater_temps(), ref.stemp) <= TOL_K
+    assert abs(info.crater_rotations - ref.rotations) <= 1
 
 
 def test_crater_sun_table_and_flux_scale(ctx, rough80):
