@@ -67,6 +67,7 @@ class BatchOpts(C.Structure):
     _fields_ = [
         ("rotations_per_sync", C.c_int), ("use_graph", C.c_int),
         ("max_active_craters", C.c_int), ("mem_fraction", C.c_double), ("profile", C.c_int),
+        ("autotune", C.c_int),
     ]
 
 
@@ -79,6 +80,7 @@ class BatchStats(C.Structure):
         ("smooth_launches", C.c_int), ("crater_launches", C.c_int),
         ("smooth_column_steps", C.c_longlong), ("crater_column_steps", C.c_longlong),
         ("max_resident", C.c_int), ("deferrals", C.c_int),
+        ("tune_rotations", C.c_int), ("tune_decisions", C.c_int),
     ]
 
 

@@ -638,26 +638,30 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
 
 // The same product in the lane-interleaved layout: for receiver block rb,
 // emitter chunk c and wave w, wseg[(rb * nchunk + c) * 4 + w] = (first, n)
-// and entry k of lane l is ent[first + 64 k + l] = (LDS offset of its emitter
-// row in the staged chunk, view-factor bits), lanes with fewer than n
-// entries padded with (0, +0.0f).  Each entry load of a wave is one 512-byte
+// and entry k of lane l is ent[first + 64 k + l] = (its emitter's index in
+// the chunk, view-factor bits), lanes with fewer than n entries padded with
+// (0, +0.0f).  H steps per workgroup (64 or 32: one body, two entry points;
+// 32 halves the staged chunk's LDS and doubles the workgroups per CU).  Each entry load of a wave is one 512-byte
 // contiguous read whose address is known ahead, so the next two are
 // prefetched; each lane still adds its entries in emitter order, so the sums
 // equal btpm_selfheat_tiled's bit for bit.  The next chunk's emission rows
 // are fetched into registers while the current chunk is consumed.
-extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell(
-    const int2* __restrict__ wseg, const int2* __restrict__ ent, int nf, int nchunk,
-    const int* __restrict__ slots, const btpm_set_dev* __restrict__ sets,
-    const btpm_status_dev* __restrict__ status) {
-  constexpr int kQ4 = kSHSteps / 4;                  // float4 per emission row
+template <int H>
+__device__ __forceinline__ void selfheat_ell_body(const int2* __restrict__ wseg,
+                                                  const int2* __restrict__ ent, int nf,
+                                                  int nchunk, const int* __restrict__ slots,
+                                                  const btpm_set_dev* __restrict__ sets,
+                                                  const btpm_status_dev* __restrict__ status) {
+  constexpr int kStride = H + 4;                     // LDS row stride in floats
+  constexpr int kQ4 = H / 4;                         // float4 per emission row
   constexpr int kPer = kSHChunk * kQ4 / kSHRows;     // float4 staged per lane
   static_assert(kSHChunk * kQ4 % kSHRows == 0 && kSHRows % kQ4 == 0, "staging shape");
-  __shared__ __align__(16) float E[kSHChunk * kSHStride];
+  __shared__ __align__(16) float E[kSHChunk * kStride];
   const int slot = slots[blockIdx.z];
   if (status[slot].done) return;
   const btpm_set_dev& S = sets[slot];
   const int npad = S.nb * BTPM_BLK;
-  const int s0 = blockIdx.y * kSHSteps;
+  const int s0 = blockIdx.y * H;
   if (s0 >= npad) return;
   const int tid = (int)threadIdx.x;
   const int f = blockIdx.x * kSHRows + tid;
@@ -680,15 +684,15 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell(
       if (g < nf && scol) pre[i] = *reinterpret_cast<const float4*>(emis + (size_t)g * npad + s0 + sq4);
     }
   };
-  float acc[kSHSteps];
+  float acc[H];
 #pragma unroll
-  for (int q = 0; q < kSHSteps; ++q) acc[q] = 0.0f;
+  for (int q = 0; q < H; ++q) acc[q] = 0.0f;
   fetch(0);
   for (int c = 0; c < nchunk; ++c) {
     __syncthreads();  // the previous chunk has been consumed
 #pragma unroll
     for (int i = 0; i < kPer; ++i)
-      *reinterpret_cast<float4*>(&E[(srow + i * (kSHRows / kQ4)) * kSHStride + sq4]) = pre[i];
+      *reinterpret_cast<float4*>(&E[(srow + i * (kSHRows / kQ4)) * kStride + sq4]) = pre[i];
     if (c + 1 < nchunk) fetch(c + 1);
     __syncthreads();
     const int2 sg = wseg[((size_t)blockIdx.x * nchunk + c) * BTPM_SH_WAVES + wave];
@@ -702,9 +706,9 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell(
       g1 = g2;
       if (k + 2 < n) g2 = e[(size_t)(k + 2) * 64];
       const float v = __int_as_float(gv.y);
-      const float* er = &E[gv.x];
+      const float* er = &E[gv.x * kStride];
 #pragma unroll
-      for (int q = 0; q < kSHSteps; q += 4) {
+      for (int q = 0; q < H; q += 4) {
         const float4 e4 = *reinterpret_cast<const float4*>(er + q);
         acc[q] = fmaf(v, e4.x, acc[q]);
         acc[q + 1] = fmaf(v, e4.y, acc[q + 1]);
@@ -717,7 +721,7 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell(
   const float* __restrict__ src = S.src;
   float* __restrict__ y = S.qext;
 #pragma unroll
-  for (int bq = 0; bq < kSHSteps / BTPM_BLK; ++bq) {
+  for (int bq = 0; bq < H / BTPM_BLK; ++bq) {
     if (s0 + bq * BTPM_BLK >= npad) break;
     const size_t o = ((size_t)(s0 / BTPM_BLK + bq) * nf + f) * BTPM_BLK;
     const float4 a = *reinterpret_cast<const float4*>(src + o);
@@ -728,6 +732,17 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell(
     *reinterpret_cast<float4*>(y + o + 4) =
         make_float4(b.x + acc[q + 4], b.y + acc[q + 5], b.z + acc[q + 6], b.w + acc[q + 7]);
   }
+}
+
+extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell(
+    const int2* wseg, const int2* ent, int nf, int nchunk, const int* slots,
+    const btpm_set_dev* sets, const btpm_status_dev* status) {
+  selfheat_ell_body<64>(wseg, ent, nf, nchunk, slots, sets, status);
+}
+extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell_h32(
+    const int2* wseg, const int2* ent, int nf, int nchunk, const int* slots,
+    const btpm_set_dev* sets, const btpm_status_dev* status) {
+  selfheat_ell_body<32>(wseg, ent, nf, nchunk, slots, sets, status);
 }
 
 // elementwise helpers over blocked arrays of n floats

@@ -23,7 +23,7 @@
 #define BTPM_SH_ROWS 256
 #define BTPM_SH_CHUNK 256
 #define BTPM_SH_STEPS 64
-#define BTPM_SH_STRIDE (BTPM_SH_STEPS + 4)  // LDS row stride of a staged chunk, floats
+#define BTPM_SH_STRIDE (BTPM_SH_STEPS + 4)  // LDS row stride of a staged chunk (CSR kernel)
 #define BTPM_SH_WAVES (BTPM_SH_ROWS / 64)     // waves per self-heating workgroup
 // the lane-interleaved (ELL) self-heating layout is used while its padded
 // entry count stays within this multiple of the view-factor count

@@ -11,8 +11,16 @@
 #include "btpm_internal.h"
 
 // conduct.hip
-extern "C" __global__ void btpm_conduct_smooth(const btpm_tile* tiles, const btpm_set_dev* sets,
-                                               const btpm_status_dev* status);
+// smooth conduction, W = 256, 128 or 64 columns per workgroup
+template <int W>
+__global__ void btpm_conduct_smooth(const btpm_tile* tiles, const btpm_set_dev* sets,
+                                    const btpm_status_dev* status);
+extern template __global__ void btpm_conduct_smooth<256>(const btpm_tile*, const btpm_set_dev*,
+                                                         const btpm_status_dev*);
+extern template __global__ void btpm_conduct_smooth<128>(const btpm_tile*, const btpm_set_dev*,
+                                                         const btpm_status_dev*);
+extern template __global__ void btpm_conduct_smooth<64>(const btpm_tile*, const btpm_set_dev*,
+                                                        const btpm_status_dev*);
 extern "C" __global__ void btpm_conduct_crater(const btpm_tile* tiles, const btpm_set_dev* sets,
                                                const btpm_status_dev* status);
 extern "C" __global__ void btpm_contract_kernel(const _Float16* vfA, const float* e, float* q,
@@ -56,6 +64,10 @@ extern "C" __global__ void btpm_selfheat_ell(const int2* wseg, const int2* ent, 
                                              int nchunk, const int* slots,
                                              const btpm_set_dev* sets,
                                              const btpm_status_dev* status);
+extern "C" __global__ void btpm_selfheat_ell_h32(const int2* wseg, const int2* ent, int nf,
+                                                 int nchunk, const int* slots,
+                                                 const btpm_set_dev* sets,
+                                                 const btpm_status_dev* status);
 extern "C" __global__ void btpm_scale_cols(float* y, const float* x, const float* scale,
                                            float gain, int ncols, size_t n);
 extern "C" __global__ void btpm_delta_max(float* p, const float* y, float* part, size_t n);

@@ -189,11 +189,16 @@ __device__ __forceinline__ void store8(gfloat* p, const float (&v)[BTPM_BLK]) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Smooth facets: one lane per column, BTPM_SMOOTH_WG columns per workgroup.
+// Smooth facets: one lane per column, W columns per workgroup (W = 256, 128
+// or 64, instantiated below; the set's tile_cols is W).  The width changes
+// only which lanes share a workgroup, never a column's arithmetic.
 // ---------------------------------------------------------------------------
-extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smooth(
-    const btpm_tile* __restrict__ tiles, const btpm_set_dev* __restrict__ sets,
-    const btpm_status_dev* __restrict__ status) {
+template <int W>
+__global__ __launch_bounds__(W, 4) void btpm_conduct_smooth(const btpm_tile* __restrict__ tiles,
+                                                           const btpm_set_dev* __restrict__ sets,
+                                                           const btpm_status_dev* __restrict__ status) {
+  // previous-rotation T of the block, parked in LDS (lane-private slots)
+  __shared__ float lds_p[BTPM_BLK][W];
   constexpr int NX = BTPM_NX;
   const btpm_tile tl = tiles[blockIdx.x];
   const btpm_set_dev& S = sets[tl.set];
@@ -204,7 +209,7 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
   const int ncols = S.ncols;
   // lane -> column through the compacted order of unfrozen columns; lanes
   // past its end take a valid column for their (unused) loads
-  const int idx = tl.tile * BTPM_SMOOTH_WG + (int)threadIdx.x;
+  const int idx = tl.tile * W + (int)threadIdx.x;
   const int norder = st.norder;
   const bool valid = idx < norder;
   const int c = S.order[valid ? idx : (norder > 0 ? norder - 1 : 0)];
@@ -241,8 +246,6 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
   const size_t bstride = (size_t)ncols * BTPM_BLK;
   const size_t coff = (size_t)cc * BTPM_BLK;
 
-  // previous-rotation T of the block, parked in LDS (lane-private slots)
-  __shared__ float lds_p[BTPM_BLK][BTPM_SMOOTH_WG];
   const int tid = (int)threadIdx.x;
   float nsrc[BTPM_BLK], nprev[BTPM_BLK];
   load8(srcp + coff, nsrc);
@@ -323,6 +326,14 @@ extern "C" __global__ __launch_bounds__(BTPM_SMOOTH_WG, 4) void btpm_conduct_smo
     S.tile_work[tl.tile] = S.inst ? 0 : nwork;
   }
 }
+
+// every width with the same register budget (4 waves per SIMD, 128 VGPRs)
+template __global__ void btpm_conduct_smooth<256>(const btpm_tile*, const btpm_set_dev*,
+                                                  const btpm_status_dev*);
+template __global__ void btpm_conduct_smooth<128>(const btpm_tile*, const btpm_set_dev*,
+                                                  const btpm_status_dev*);
+template __global__ void btpm_conduct_smooth<64>(const btpm_tile*, const btpm_set_dev*,
+                                                 const btpm_status_dev*);
 
 // ---------------------------------------------------------------------------
 // Crater columns: BTPM_CRATERS_WG craters x 100 sub-facets per workgroup of

@@ -17,6 +17,7 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -182,8 +183,8 @@ struct btpm_shape {
   DBuf sent, cptr;  // sent: (emitter, float bits of VF) pairs
   int nchunk = 0;
   // the same entries lane-interleaved per (receiver block, chunk, wave) for
-  // btpm_selfheat_ell: wseg = (first entry, entries per lane), ell = (LDS
-  // offset of the emitter row, float bits of VF); used while the padding
+  // btpm_selfheat_ell: wseg = (first entry, entries per lane), ell = (emitter
+  // within the chunk, float bits of VF); used while the padding
   // stays within BTPM_SH_ELL_MAX_PAD times the view-factor count
   DBuf wseg, ell;
   bool use_ell = false;
@@ -294,7 +295,8 @@ void build_smooth(btpm_solve* s, hipStream_t st) {
     S->qext.alloc(nser * sizeof(float));
     S->emis.alloc(nser * sizeof(float));
   }
-  S->ntiles = (d.ncols + BTPM_SMOOTH_WG - 1) / BTPM_SMOOTH_WG;
+  // tile arrays for the narrowest tuned width (64 columns per workgroup)
+  S->ntiles = (d.ncols + 63) / 64;
   S->tile_active.alloc(S->ntiles * sizeof(int));
   S->tile_work.alloc(S->ntiles * sizeof(int));
   d.src = S->src.as<float>();
@@ -424,15 +426,115 @@ struct Sched {
   hipGraph_t graph = nullptr;
   hipGraphExec_t phase_exec[4] = {nullptr, nullptr, nullptr, nullptr};
   hipGraph_t phase_graph[4] = {nullptr, nullptr, nullptr, nullptr};
-  DBuf d_tiles_s, d_tiles_c, d_ids;
+  DBuf d_tiles_c, d_ids;
+  // smooth tiles, one launch per workgroup width (the set on trial, if any,
+  // in a launch of its own)
+  struct SmoothList {
+    int width;
+    int n;
+    bool trial;
+    std::unique_ptr<DBuf> tiles;
+  };
+  std::vector<SmoothList> smooth_lists;
   // smooth sets with facet self-heating, grouped by shape: one tiled SpMM
   // launch per group per rotation
   struct HeatGroup {
     btpm_shape* shape;
     int nsets;
     std::unique_ptr<DBuf> slots;
+    int steps = BTPM_SH_STEPS;  // steps per workgroup of the ELL kernel
+    bool trial = false;
   };
   std::vector<HeatGroup> heat;
+
+  // ---- autotuner ----------------------------------------------------------
+  // Tile choices per size class, measured on the device:
+  //   kind 0: smooth conduction columns per workgroup {256, 128, 64}, per
+  //           column-count class (ceil log2 of the set's columns);
+  //   kind 1: facet self-heating (ELL kernel) steps per workgroup {64, 32},
+  //           per (log2 nnz, log2 facets, step blocks) class.
+  // Every candidate computes bit-identical results (only the launch shape
+  // differs), so a trial is an ordinary rotation of the running solves:
+  // while a class is untuned the scheduler syncs after every rotation, runs
+  // that class's sets in a launch of their own between two events and
+  // charges the time to the candidate per unfrozen column.  Candidates run in
+  // palindromic order (A B C C B A), which cancels a linear drift of the
+  // per-rotation cost as the solves converge.  The cheapest is kept in the
+  // context for every later run, and the graphs are re-captured with it.
+  using TuneKey = std::pair<int, long long>;
+  struct Trial {
+    TuneKey key;
+    std::vector<int> cand, seq;
+    size_t next = 0;
+    std::vector<double> cost, n;
+    double units = 0.0;  // unfrozen columns of the rotation being timed
+    bool timed = false;  // the events bracket a launch of this chunk
+  };
+  std::deque<Trial> trials;
+  hipEvent_t tev[2] = {nullptr, nullptr};
+
+  static long long log2_class(long long v) {
+    long long c = 0;
+    while ((1LL << c) < v) ++c;
+    return c;
+  }
+  static TuneKey smooth_key(const SetHost* S) { return {0, log2_class(S->d.ncols)}; }
+  static TuneKey heat_key(const btpm_shape* sh) {
+    return {1, (log2_class(sh->nnz) << 40) | (log2_class(sh->nf) << 24) | sh->nb};
+  }
+  bool tuning() const { return opt.autotune && !trials.empty(); }
+  bool on_trial(const TuneKey& k) const { return tuning() && trials.front().key == k; }
+  int choice(const TuneKey& k, int dflt) const {
+    if (on_trial(k)) {
+      const Trial& t = trials.front();
+      return t.cand[t.seq[t.next]];
+    }
+    auto it = ctx->tuned.find(k);
+    return it == ctx->tuned.end() ? dflt : it->second;
+  }
+  void want_trial(const TuneKey& k, const std::vector<int>& cand) {
+    if (ctx->tuned.count(k)) return;
+    for (const Trial& t : trials)
+      if (t.key == k) return;
+    Trial t;
+    t.key = k;
+    t.cand = cand;
+    for (size_t i = 0; i < cand.size(); ++i) t.seq.push_back((int)i);
+    for (size_t i = cand.size(); i-- > 0;) t.seq.push_back((int)i);
+    t.cost.assign(cand.size(), 0.0);
+    t.n.assign(cand.size(), 0.0);
+    trials.push_back(std::move(t));
+  }
+  // after the sync of a trial rotation: charge its time, decide when done
+  void trial_sample() {
+    if (trials.empty()) return;
+    Trial& t = trials.front();
+    if (t.timed && t.units > 0.0) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, tev[0], tev[1]));
+      const int c = t.seq[t.next];
+      t.cost[c] += ms / t.units;
+      t.n[c] += 1.0;
+      t.next++;
+      stats.tune_rotations++;
+      if (trace_on())
+        std::fprintf(stderr, "[btpm] tune (%d, %lld) candidate %d: %.3f ms for %.0f columns\n",
+                     t.key.first, t.key.second, t.cand[c], ms, t.units);
+    }
+    t.timed = false;
+    if (t.next == t.seq.size()) {
+      size_t best = 0;
+      for (size_t i = 1; i < t.cand.size(); ++i)
+        if (t.cost[i] / t.n[i] < t.cost[best] / t.n[best]) best = i;
+      ctx->tuned[t.key] = t.cand[best];
+      stats.tune_decisions++;
+      if (trace_on())
+        std::fprintf(stderr, "[btpm] tune (%d, %lld) -> %d\n", t.key.first, t.key.second,
+                     t.cand[best]);
+      trials.pop_front();
+    }
+    dirty = true;
+  }
   int n_tiles_s = 0, n_tiles_c = 0, n_ids = 0;
   btpm_batch_stats stats{};
   // launch timing (opts.profile): events around each conduction launch
@@ -458,9 +560,13 @@ struct Sched {
     const double frac = opt.mem_fraction > 0 ? std::min((double)opt.mem_fraction, 1.0) : 0.8;
     headroom = (size_t)((double)fr * (1.0 - frac));
     t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipEventCreate(&tev[0]));
+    HIPCHK(hipEventCreate(&tev[1]));
   }
   ~Sched() {
     drop_graph();
+    for (hipEvent_t e : tev)
+      if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_s) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_c) (void)hipEventDestroy(e);
   }
@@ -590,34 +696,89 @@ struct Sched {
   }
 
   void rebuild_lists() {
-    std::vector<btpm_tile> ts, tc;
-    std::vector<int> ids;
-    for (auto& kv : slots) {
-      SetHost* S = kv.second.second;
-      ids.push_back(S->slot);
-      auto& v = S->d.kind == BTPM_SET_SMOOTH ? ts : tc;
-      // only the tiles the compacted order[] can fill (it only shrinks)
-      S->listed = needed_tiles(S);
-      for (int t = 0; t < S->listed; ++t) v.push_back(btpm_tile{S->slot, t});
-    }
-    n_tiles_s = (int)ts.size();
-    n_tiles_c = (int)tc.size();
-    n_ids = (int)ids.size();
+    // tuning: classes present without a decision get a trial; a trial whose
+    // sets have all left is dropped (it restarts if the class returns)
     std::map<btpm_shape*, std::vector<int>> by_shape;
     for (auto& kv : slots) {
       SetHost* S = kv.second.second;
       if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext)
         by_shape[kv.second.first->shape].push_back(S->slot);
     }
+    if (opt.autotune) {
+      std::set<TuneKey> present;
+      for (auto& kv : slots)
+        if (kv.second.second->d.kind == BTPM_SET_SMOOTH) {
+          present.insert(smooth_key(kv.second.second));
+          want_trial(smooth_key(kv.second.second), {256, 128, 64});
+        }
+      for (auto& kv : by_shape)
+        if (kv.first->use_ell) {
+          present.insert(heat_key(kv.first));
+          want_trial(heat_key(kv.first), {64, 32});
+        }
+      while (!trials.empty() && !present.count(trials.front().key)) trials.pop_front();
+      if (!trials.empty()) trials.front().units = 0.0;
+    }
+    std::map<int, std::vector<btpm_tile>> ts;
+    std::vector<btpm_tile> ts_trial, tc;
+    std::vector<int> ids;
+    for (auto& kv : slots) {
+      SetHost* S = kv.second.second;
+      ids.push_back(S->slot);
+      const int unfrozen = ctx->h_status[S->slot].norder;
+      if (S->d.kind == BTPM_SET_SMOOTH) {
+        const TuneKey k = smooth_key(S);
+        const int w = choice(k, BTPM_SMOOTH_WG);
+        if (S->d.tile_cols != w) {  // the status kernel reads the width too
+          S->d.tile_cols = w;
+          HIPCHK(hipMemcpyAsync(ctx->d_sets + S->slot, &S->d, sizeof(btpm_set_dev),
+                                hipMemcpyHostToDevice, ctx->stream));
+        }
+        S->listed = needed_tiles(S);
+        const bool tr = on_trial(k);
+        if (tr) trials.front().units += unfrozen;
+        for (int t = 0; t < S->listed; ++t) (tr ? ts_trial : ts[w]).push_back(btpm_tile{S->slot, t});
+      } else {
+        // only the tiles the compacted order[] can fill (it only shrinks)
+        S->listed = needed_tiles(S);
+        for (int t = 0; t < S->listed; ++t) tc.push_back(btpm_tile{S->slot, t});
+      }
+    }
+    smooth_lists.clear();
+    n_tiles_s = 0;
+    auto add_list = [&](int w, const std::vector<btpm_tile>& v, bool trial) {
+      if (v.empty()) return;
+      SmoothList l{w, (int)v.size(), trial, std::make_unique<DBuf>()};
+      upload(*l.tiles, v.data(), v.size(), ctx->stream);
+      n_tiles_s += l.n;
+      smooth_lists.push_back(std::move(l));
+    };
+    if (!ts_trial.empty()) add_list(choice(trials.front().key, BTPM_SMOOTH_WG), ts_trial, true);
+    for (auto& kv : ts) add_list(kv.first, kv.second, false);
+    n_tiles_c = (int)tc.size();
+    n_ids = (int)ids.size();
     heat.clear();
     for (auto& kv : by_shape) {
       HeatGroup g{kv.first, (int)kv.second.size(), std::make_unique<DBuf>()};
       upload(*g.slots, kv.second.data(), kv.second.size(), ctx->stream);
+      if (kv.first->use_ell) {
+        const TuneKey k = heat_key(kv.first);
+        g.steps = choice(k, BTPM_SH_STEPS);
+        g.trial = on_trial(k);
+        if (g.trial)
+          for (int sl : kv.second) trials.front().units += ctx->h_status[sl].norder;
+      }
       heat.push_back(std::move(g));
     }
-    upload(d_tiles_s, ts.data(), ts.size(), ctx->stream);
     upload(d_tiles_c, tc.data(), tc.size(), ctx->stream);
     upload(d_ids, ids.data(), ids.size(), ctx->stream);
+  }
+
+  // the events around the launch on trial (only in tuning chunks, which
+  // are never captured)
+  void trial_event(hipStream_t st, int i) {
+    HIPCHK(hipEventRecord(tev[i], st));
+    if (i == 1) trials.front().timed = true;
   }
 
   // one rotation of every resident set, in four phases: facet self-heating
@@ -628,9 +789,14 @@ struct Sched {
       // every smooth set of a shape in one launch
       for (const HeatGroup& g : heat) {
         btpm_shape* sh = g.shape;
-        dim3 grid(grid1(sh->nf, BTPM_SH_ROWS), grid1((size_t)sh->nb * BTPM_BLK, BTPM_SH_STEPS),
+        dim3 grid(grid1(sh->nf, BTPM_SH_ROWS), grid1((size_t)sh->nb * BTPM_BLK, g.steps),
                   g.nsets);
-        if (sh->use_ell)
+        if (g.trial) trial_event(st, 0);
+        if (sh->use_ell && g.steps == 32)
+          hipLaunchKernelGGL(btpm_selfheat_ell_h32, grid, dim3(BTPM_SH_ROWS), 0, st,
+                             sh->wseg.as<int2>(), sh->ell.as<int2>(), sh->nf, sh->nchunk,
+                             g.slots->as<int>(), ctx->d_sets, ctx->d_status);
+        else if (sh->use_ell)
           hipLaunchKernelGGL(btpm_selfheat_ell, grid, dim3(BTPM_SH_ROWS), 0, st,
                              sh->wseg.as<int2>(), sh->ell.as<int2>(), sh->nf, sh->nchunk,
                              g.slots->as<int>(), ctx->d_sets, ctx->d_status);
@@ -638,11 +804,23 @@ struct Sched {
           hipLaunchKernelGGL(btpm_selfheat_tiled, grid, dim3(BTPM_SH_ROWS), 0, st,
                              sh->cptr.as<int>(), sh->sent.as<int2>(), sh->nf, sh->nchunk,
                              g.slots->as<int>(), ctx->d_sets, ctx->d_status);
+        if (g.trial) trial_event(st, 1);
       }
     } else if (ph == 1) {
-      if (n_tiles_s)
-        hipLaunchKernelGGL(btpm_conduct_smooth, dim3(n_tiles_s), dim3(BTPM_SMOOTH_WG), 0, st,
-                           d_tiles_s.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
+      for (const SmoothList& l : smooth_lists) {
+        const btpm_tile* t = l.tiles->as<btpm_tile>();
+        if (l.trial) trial_event(st, 0);
+        if (l.width == 64)
+          hipLaunchKernelGGL(btpm_conduct_smooth<64>, dim3(l.n), dim3(64), 0, st, t,
+                             ctx->d_sets, ctx->d_status);
+        else if (l.width == 128)
+          hipLaunchKernelGGL(btpm_conduct_smooth<128>, dim3(l.n), dim3(128), 0, st, t,
+                             ctx->d_sets, ctx->d_status);
+        else
+          hipLaunchKernelGGL(btpm_conduct_smooth<256>, dim3(l.n), dim3(256), 0, st, t,
+                             ctx->d_sets, ctx->d_status);
+        if (l.trial) trial_event(st, 1);
+      }
     } else if (ph == 2) {
       if (n_tiles_c)
         hipLaunchKernelGGL(btpm_conduct_crater, dim3(n_tiles_c), dim3(BTPM_CRATER_THREADS), 0,
@@ -670,15 +848,16 @@ struct Sched {
   }
 
   // profiled rotation: each conduction phase bracketed by stream events
-  // (around a launch of the phase's own graph when graphs are in use)
-  void profiled_rotation(int i) {
+  // (around a launch of the phase's own graph when graphs are in use and the
+  // chunk is not a tuning trial)
+  void profiled_rotation(int i, bool direct) {
     hipStream_t st = ctx->stream;
     for (int ph = 0; ph < 4; ++ph) {
       hipEvent_t* ev = ph == 1 ? &ev_s[2 * i] : ph == 2 ? &ev_c[2 * i] : nullptr;
       const bool empty = phase_empty(ph);
       if (empty) continue;
       if (ev) HIPCHK(hipEventRecord(ev[0], st));
-      if (opt.use_graph) {
+      if (opt.use_graph && !direct) {
         HIPCHK(hipGraphLaunch(phase_exec[ph], st));
         stats.graph_launches++;
       } else {
@@ -699,12 +878,14 @@ struct Sched {
   }
 
   void run_chunk() {
-    const int R = std::max(1, opt.rotations_per_sync);
+    // a tuning trial is one rotation, enqueued directly and synced
+    const bool tune = tuning();
+    const int R = tune ? 1 : std::max(1, opt.rotations_per_sync);
     if (opt.profile) make_events(R);
-    if (dirty) {
+    if (dirty || tune) {
       drop_graph();
       rebuild_lists();
-      if (opt.use_graph) {
+      if (opt.use_graph && !tune) {
         if (opt.profile) {
           for (int ph = 0; ph < 4; ++ph)
             if (!phase_empty(ph)) capture(&phase_graph[ph], &phase_exec[ph], 1, ph);
@@ -715,7 +896,9 @@ struct Sched {
       dirty = false;
     }
     if (opt.profile) {
-      for (int i = 0; i < R; ++i) profiled_rotation(i);
+      for (int i = 0; i < R; ++i) profiled_rotation(i, tune);
+    } else if (tune) {
+      enqueue_rotation(ctx->stream);
     } else if (opt.use_graph) {
       HIPCHK(hipGraphLaunch(gexec, ctx->stream));
       stats.graph_launches++;
@@ -727,6 +910,7 @@ struct Sched {
                           hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     stats.syncs++;
+    if (tune) trial_sample();
     if (opt.profile) {
       for (int i = 0; i < R; ++i) {
         float ms = 0.0f;
@@ -858,6 +1042,7 @@ void btpm_batch_opts_default(btpm_batch_opts* o) {
   o->max_active_craters = 0;
   o->mem_fraction = 0.8;
   o->profile = 0;
+  o->autotune = 1;
 }
 
 int btpm_ctx_create(int device, btpm_ctx** out) {
@@ -1020,7 +1205,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
                 const int k0 = cp[(size_t)c * nf + f], k1 = cp[(size_t)(c + 1) * nf + f];
                 for (int k = k0; k < k1; ++k) {
                   const size_t o = first + (size_t)(k - k0) * 64 + l;
-                  el[2 * o] = (se[2 * (size_t)k] - c * kChunk) * BTPM_SH_STRIDE;
+                  el[2 * o] = se[2 * (size_t)k] - c * kChunk;  // emitter within the chunk
                   el[2 * o + 1] = se[2 * (size_t)k + 1];
                 }
               }

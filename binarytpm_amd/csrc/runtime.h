@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <atomic>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -124,6 +125,9 @@ struct btpm_ctx {
   std::mutex mu;
   std::atomic<int> waiting{0};  // callers blocked on mu: a server yields to them
   int servers = 0;              // running servers (at most one; guarded by mu)
+  // autotuner decisions of this context (guarded by mu): (tunable, size
+  // class) -> chosen value; kept for every later batch run or server
+  std::map<std::pair<int, long long>, int> tuned;
 };
 
 // Takes ctx->mu for one API call, announcing itself so that a running

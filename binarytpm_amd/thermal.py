@@ -247,10 +247,11 @@ class Solve:
 
 
 def batch_opts(rotations_per_sync=4, use_graph=True, max_active_craters=0,
-               mem_fraction=0.8, profile=False) -> BatchOpts:
+               mem_fraction=0.8, profile=False, autotune=True) -> BatchOpts:
     o = BatchOpts()
     lib().btpm_batch_opts_default(C.byref(o))
     o.profile = int(bool(profile))
+    o.autotune = int(bool(autotune))
     o.rotations_per_sync = int(rotations_per_sync)
     o.use_graph = int(bool(use_graph))
     o.max_active_craters = int(max_active_craters)

@@ -137,6 +137,10 @@ typedef struct {
                                     live free memory, so results of finished
                                     solves count until they are destroyed  */
     int profile;                 /* time every conduction launch with events */
+    int autotune;                /* choose tile widths per size class by timed
+                                    trial rotations (results are identical for
+                                    every choice); decisions are kept in the
+                                    context.  Default 1                        */
 } btpm_batch_opts;
 
 typedef struct {
@@ -157,6 +161,8 @@ typedef struct {
     long long crater_column_steps;
     int max_resident;            /* most solves resident at once           */
     int deferrals;               /* set builds postponed for lack of memory */
+    int tune_rotations;          /* rotations run as timed tuning trials    */
+    int tune_decisions;          /* size classes the autotuner decided      */
 } btpm_batch_stats;
 
 /* ---- library ------------------------------------------------------------ */

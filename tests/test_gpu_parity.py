@@ -304,6 +304,42 @@ def test_crater_sun_table_and_flux_scale(ctx, rough80):
         x.close()
 
 
+def test_autotune_identical_results(rough80):
+    """The autotuner's trial rotations run candidate tile widths on the live
+    solves (smooth conduction 256/128/64 columns per workgroup, self-heating
+    64/32 steps per workgroup); every candidate does the same arithmetic, so a
+    tuned batch equals an untuned one bit for bit.  Decisions are kept in the
+    context: a second batch on it runs no trials."""
+    case, crater = rough80
+    p = th.params(650, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
+                        solar=crater["solar"], multiple=True, scatter_acc=1e-4,
+                        drf=crater["drf"])
+    out = {}
+    for tune in (False, True):
+        with th.Context(0) as c2:  # a fresh context: nothing tuned yet
+            shape = th.Shape(c2, p, case.illum, 0.1, csr=case.csr, scatter=2, crater=cm)
+            solves = [th.Solve(c2, shape, ti, with_craters=True) for ti in (50.0, 400.0)]
+            st = th.batch_run(c2, solves, th.batch_opts(autotune=tune))
+            out[tune] = ([s.surface_temps() for s in solves], [s.crater_temps() for s in solves])
+            print(f"[tune] autotune={tune}: {st.tune_decisions} decisions, "
+                  f"{st.tune_rotations} trial rotations", flush=True)
+            if tune:
+                assert st.tune_decisions == 2 and st.tune_rotations == 10
+                again = [th.Solve(c2, shape, 120.0, with_craters=True)]
+                st2 = th.batch_run(c2, again, th.batch_opts(autotune=True))
+                assert st2.tune_decisions == 0 and st2.tune_rotations == 0
+                assert again[0].info().converged
+                solves += again
+            else:
+                assert st.tune_decisions == 0 and st.tune_rotations == 0
+            for s in solves:
+                s.close()
+            shape.close()
+    for a, b in zip(out[False][0] + out[False][1], out[True][0] + out[True][1]):
+        assert np.array_equal(a, b)
+
+
 def test_stepwise_path_parity(ctx, rough80):
     """The reference-structured path (an FD and a boundary launch plus a host
     sync per step, profile in global memory, per-lane crater loop) reaches
