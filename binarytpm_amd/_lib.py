@@ -67,7 +67,7 @@ class BatchOpts(C.Structure):
     _fields_ = [
         ("rotations_per_sync", C.c_int), ("use_graph", C.c_int),
         ("max_active_craters", C.c_int), ("mem_fraction", C.c_double), ("profile", C.c_int),
-        ("autotune", C.c_int),
+        ("autotune", C.c_int), ("fast_extra", C.c_int),
     ]
 
 
@@ -81,6 +81,7 @@ class BatchStats(C.Structure):
         ("smooth_column_steps", C.c_longlong), ("crater_column_steps", C.c_longlong),
         ("max_resident", C.c_int), ("deferrals", C.c_int),
         ("tune_rotations", C.c_int), ("tune_decisions", C.c_int),
+        ("fast_rotations", C.c_longlong),
     ]
 
 

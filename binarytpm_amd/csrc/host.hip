@@ -536,6 +536,16 @@ struct Sched {
     dirty = true;
   }
   int n_tiles_s = 0, n_tiles_c = 0, n_ids = 0;
+  // Fast lane: sets at most 1/8 the size (columns) of the largest resident
+  // set run opt.fast_extra more rotations after every rotation of all sets,
+  // from lists of their own, so a small solve does not advance in lock-step
+  // with the big sets beside it (sets are independent: results are the same).
+  // Off in profiled runs (their per-launch timings) and in tuning trials.
+  std::vector<SmoothList> fsmooth;
+  std::vector<HeatGroup> fheat;
+  DBuf fd_tiles_c, fd_ids;
+  int fn_tiles_c = 0, fn_ids = 0;
+  bool fast_active = false;
   btpm_batch_stats stats{};
   // launch timing (opts.profile): events around each conduction launch
   std::vector<hipEvent_t> ev_s, ev_c;  // pairs (start, end) per rotation
@@ -772,6 +782,49 @@ struct Sched {
     }
     upload(d_tiles_c, tc.data(), tc.size(), ctx->stream);
     upload(d_ids, ids.data(), ids.size(), ctx->stream);
+    rebuild_fast();
+  }
+
+  void rebuild_fast() {
+    fsmooth.clear();
+    fheat.clear();
+    fn_tiles_c = fn_ids = 0;
+    fast_active = false;
+    if (opt.fast_extra <= 0 || opt.profile || tuning()) return;
+    int max_cols = 0;
+    for (auto& kv : slots) max_cols = std::max(max_cols, kv.second.second->d.ncols);
+    auto fast = [&](const SetHost* S) { return 8LL * S->d.ncols <= max_cols; };
+    std::map<int, std::vector<btpm_tile>> ts;
+    std::vector<btpm_tile> tc;
+    std::vector<int> ids;
+    std::map<btpm_shape*, std::vector<int>> by_shape;
+    for (auto& kv : slots) {
+      const SetHost* S = kv.second.second;
+      if (!fast(S)) continue;
+      ids.push_back(S->slot);
+      auto& v = S->d.kind == BTPM_SET_SMOOTH ? ts[S->d.tile_cols] : tc;
+      for (int t = 0; t < S->listed; ++t) v.push_back(btpm_tile{S->slot, t});
+      if (S->d.kind == BTPM_SET_SMOOTH && S->d.qext)
+        by_shape[kv.second.first->shape].push_back(S->slot);
+    }
+    if (ids.empty()) return;
+    for (auto& kv : ts) {
+      SmoothList l{kv.first, (int)kv.second.size(), false, std::make_unique<DBuf>()};
+      upload(*l.tiles, kv.second.data(), kv.second.size(), ctx->stream);
+      fsmooth.push_back(std::move(l));
+    }
+    for (auto& kv : by_shape) {
+      HeatGroup g{kv.first, (int)kv.second.size(), std::make_unique<DBuf>()};
+      upload(*g.slots, kv.second.data(), kv.second.size(), ctx->stream);
+      for (const HeatGroup& m : heat)
+        if (m.shape == kv.first) g.steps = m.steps;
+      fheat.push_back(std::move(g));
+    }
+    fn_tiles_c = (int)tc.size();
+    fn_ids = (int)ids.size();
+    upload(fd_tiles_c, tc.data(), tc.size(), ctx->stream);
+    upload(fd_ids, ids.data(), ids.size(), ctx->stream);
+    fast_active = true;
   }
 
   // the events around the launch on trial (only in tuning chunks, which
@@ -783,11 +836,11 @@ struct Sched {
 
   // one rotation of every resident set, in four phases: facet self-heating
   // SpMMs, smooth conduction, crater conduction, status
-  void enqueue_phase(hipStream_t st, int ph) {
+  void enqueue_phase(hipStream_t st, int ph, bool fast = false) {
     if (ph == 0) {
       // qext = src + VF * emis: the total absorbed flux of this rotation, for
       // every smooth set of a shape in one launch
-      for (const HeatGroup& g : heat) {
+      for (const HeatGroup& g : fast ? fheat : heat) {
         btpm_shape* sh = g.shape;
         dim3 grid(grid1(sh->nf, BTPM_SH_ROWS), grid1((size_t)sh->nb * BTPM_BLK, g.steps),
                   g.nsets);
@@ -807,7 +860,7 @@ struct Sched {
         if (g.trial) trial_event(st, 1);
       }
     } else if (ph == 1) {
-      for (const SmoothList& l : smooth_lists) {
+      for (const SmoothList& l : fast ? fsmooth : smooth_lists) {
         const btpm_tile* t = l.tiles->as<btpm_tile>();
         if (l.trial) trial_event(st, 0);
         if (l.width == 64)
@@ -822,18 +875,24 @@ struct Sched {
         if (l.trial) trial_event(st, 1);
       }
     } else if (ph == 2) {
-      if (n_tiles_c)
-        hipLaunchKernelGGL(btpm_conduct_crater, dim3(n_tiles_c), dim3(BTPM_CRATER_THREADS), 0,
-                           st, d_tiles_c.as<btpm_tile>(), ctx->d_sets, ctx->d_status);
+      const int n = fast ? fn_tiles_c : n_tiles_c;
+      if (n)
+        hipLaunchKernelGGL(btpm_conduct_crater, dim3(n), dim3(BTPM_CRATER_THREADS), 0, st,
+                           (fast ? fd_tiles_c : d_tiles_c).as<btpm_tile>(), ctx->d_sets,
+                           ctx->d_status);
     } else {
-      if (n_ids)
-        hipLaunchKernelGGL(btpm_status_kernel, dim3(n_ids), dim3(256), 0, st, d_ids.as<int>(),
-                           ctx->d_sets, ctx->d_status);
+      const int n = fast ? fn_ids : n_ids;
+      if (n)
+        hipLaunchKernelGGL(btpm_status_kernel, dim3(n), dim3(256), 0, st,
+                           (fast ? fd_ids : d_ids).as<int>(), ctx->d_sets, ctx->d_status);
     }
   }
 
   void enqueue_rotation(hipStream_t st) {
     for (int ph = 0; ph < 4; ++ph) enqueue_phase(st, ph);
+    if (fast_active)
+      for (int i = 0; i < opt.fast_extra; ++i)
+        for (int ph = 0; ph < 4; ++ph) enqueue_phase(st, ph, true);
   }
 
   bool phase_empty(int ph) const {
@@ -910,6 +969,7 @@ struct Sched {
                           hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     stats.syncs++;
+    if (fast_active && !opt.profile) stats.fast_rotations += R * opt.fast_extra;
     if (tune) trial_sample();
     if (opt.profile) {
       for (int i = 0; i < R; ++i) {
@@ -1043,6 +1103,7 @@ void btpm_batch_opts_default(btpm_batch_opts* o) {
   o->mem_fraction = 0.8;
   o->profile = 0;
   o->autotune = 1;
+  o->fast_extra = 3;
 }
 
 int btpm_ctx_create(int device, btpm_ctx** out) {

@@ -247,11 +247,12 @@ class Solve:
 
 
 def batch_opts(rotations_per_sync=4, use_graph=True, max_active_craters=0,
-               mem_fraction=0.8, profile=False, autotune=True) -> BatchOpts:
+               mem_fraction=0.8, profile=False, autotune=True, fast_extra=3) -> BatchOpts:
     o = BatchOpts()
     lib().btpm_batch_opts_default(C.byref(o))
     o.profile = int(bool(profile))
     o.autotune = int(bool(autotune))
+    o.fast_extra = int(fast_extra)
     o.rotations_per_sync = int(rotations_per_sync)
     o.use_graph = int(bool(use_graph))
     o.max_active_craters = int(max_active_craters)

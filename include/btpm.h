@@ -141,6 +141,11 @@ typedef struct {
                                     trial rotations (results are identical for
                                     every choice); decisions are kept in the
                                     context.  Default 1                        */
+    int fast_extra;              /* extra rotations, after every rotation of
+                                    all sets, for sets at most 1/8 the columns
+                                    of the largest resident set (they finish
+                                    in fewer shared rotations; results do not
+                                    change).  Not in profiled runs.  Default 3 */
 } btpm_batch_opts;
 
 typedef struct {
@@ -163,6 +168,7 @@ typedef struct {
     int deferrals;               /* set builds postponed for lack of memory */
     int tune_rotations;          /* rotations run as timed tuning trials    */
     int tune_decisions;          /* size classes the autotuner decided      */
+    long long fast_rotations;    /* extra fast-lane rotations enqueued      */
 } btpm_batch_stats;
 
 /* ---- library ------------------------------------------------------------ */
