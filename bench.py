@@ -227,7 +227,7 @@ def single_solve_persistent(ctx, shape, plan):
     return 3600.0 * len(plan) / total
 
 
-def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2):
+def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2, rps=4):
     """Serving workload: Poisson arrivals (fixed rate, seeded) of solve
     requests over a pool of resident seeded icosphere-ellipsoid shapes of
     1,280..81,920 facets, each request with TI ~ U[10, 2500], roughness
@@ -258,7 +258,7 @@ def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2
     reqs = [(int(rng.choice(subdivs)), int(rng.integers(seeds_per_size)),
              float(rng.uniform(10.0, 2500.0)), float(rng.uniform(0.0, 1.0)),
              float(rng.uniform(0.9, 3.0))) for _ in range(n_req)]
-    srv = th.Server(ctx, th.batch_opts(rotations_per_sync=4, use_graph=True))
+    srv = th.Server(ctx, th.batch_opts(rotations_per_sync=rps, use_graph=True))
     submitted, results, errors = [], [], []
     lock = threading.Condition()
 
@@ -346,7 +346,10 @@ def main():
     ap.add_argument("--subdiv", type=int, default=4)
     ap.add_argument("--ti", type=int, default=32, help="thermal inertias in the sweep")
     ap.add_argument("--gamma", type=float, default=45.0)
-    ap.add_argument("--rps", type=int, default=4, help="rotations per host sync")
+    ap.add_argument("--rps", type=int, default=4,
+                    help="rotations per host sync of the sweep (0: the scheduler's choice)")
+    ap.add_argument("--serve-rps", type=int, default=0,
+                    help="rotations per host sync of the server (0: the scheduler's choice)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -486,7 +489,7 @@ def main():
             s.close()
         shape.close()
         if not args.no_serve:
-            line["serving"] = serve_phase(ctx, args.serve_n, args.serve_rate)
+            line["serving"] = serve_phase(ctx, args.serve_n, args.serve_rate, rps=args.serve_rps)
     print(json.dumps(line), flush=True)
 
 

@@ -936,10 +936,36 @@ struct Sched {
     HIPCHK(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
   }
 
+  // Rotations per host sync: opt.rotations_per_sync, or with 0 the
+  // scheduler's own choice: the fewest rotations that keep a chunk at least
+  // kChunkMs long (a sync costs tens of microseconds; fewer rotations per
+  // chunk retire finished solves and admit new ones sooner), from the
+  // measured wall time per rotation; it moves only by 2x steps (each move
+  // re-captures the graph).
+  static constexpr double kChunkMs = 2.0;
+  int auto_R = 1;
+  double rot_ms = -1.0;  // smoothed wall time per rotation
+  int rotations_per_chunk() const {
+    return opt.rotations_per_sync > 0 ? opt.rotations_per_sync : auto_R;
+  }
+  void schedule_sample(double chunk_ms, int R) {
+    if (opt.rotations_per_sync > 0 || R <= 0) return;
+    const double t = chunk_ms / R;
+    rot_ms = rot_ms < 0.0 ? t : 0.7 * rot_ms + 0.3 * t;
+    const int want = std::max(1, std::min(16, (int)std::ceil(kChunkMs / std::max(rot_ms, 1e-3))));
+    if (want >= 2 * auto_R || 2 * want <= auto_R) {
+      if (trace_on())
+        std::fprintf(stderr, "[btpm] rotations per sync %d -> %d (%.3f ms per rotation)\n",
+                     auto_R, want, rot_ms);
+      auto_R = want;
+      dirty = true;
+    }
+  }
+
   void run_chunk() {
     // a tuning trial is one rotation, enqueued directly and synced
     const bool tune = tuning();
-    const int R = tune ? 1 : std::max(1, opt.rotations_per_sync);
+    const int R = tune ? 1 : std::max(1, rotations_per_chunk());
     if (opt.profile) make_events(R);
     if (dirty || tune) {
       drop_graph();
@@ -954,6 +980,7 @@ struct Sched {
       }
       dirty = false;
     }
+    const auto c0 = std::chrono::steady_clock::now();  // the rotations alone
     if (opt.profile) {
       for (int i = 0; i < R; ++i) profiled_rotation(i, tune);
     } else if (tune) {
@@ -970,6 +997,9 @@ struct Sched {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     stats.syncs++;
     if (fast_active && !opt.profile) stats.fast_rotations += R * opt.fast_extra;
+    schedule_sample(
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count(),
+        R);
     if (tune) trial_sample();
     if (opt.profile) {
       for (int i = 0; i < R; ++i) {

@@ -129,7 +129,9 @@ typedef struct {
 } btpm_solve_info;
 
 typedef struct {
-    int rotations_per_sync;      /* rotations enqueued between host checks */
+    int rotations_per_sync;      /* rotations enqueued between host checks;
+                                    0: chosen by the scheduler from the
+                                    measured time per rotation             */
     int use_graph;               /* capture each rotation set in a hipGraph */
     int max_active_craters;      /* crater solves resident at once (0: by memory) */
     double mem_fraction;         /* admission keeps (1 - mem_fraction) of
