@@ -386,16 +386,18 @@ def test_batch_equals_alone(ctx, rough80):
             np.testing.assert_array_equal(a_c, b.crater_temps())
 
 
-def test_server_results_match_alone(ctx):
+@pytest.mark.parametrize("rps", [2, 0])
+def test_server_results_match_alone(ctx, rps):
     """Continuous batching: solves submitted while others run complete with
-    the same temperatures as alone."""
+    the same temperatures as alone (fixed rotations per sync, and the
+    scheduler's own choice with rps = 0)."""
     import time
     case = inputs.config0(subdiv=2)
     p = th.params(650, case.rot_period, tacc=0.01)
     shape = th.Shape(ctx, p, case.illum, 0.1)
     tis = [20.0, 700.0, 150.0, 2400.0]
     alone = [th.Solve(ctx, shape, t).run().surface_temps() for t in tis]
-    srv = th.Server(ctx, th.batch_opts(rotations_per_sync=2))
+    srv = th.Server(ctx, th.batch_opts(rotations_per_sync=rps))
     sol = []
     for t in tis:
         s = th.Solve(ctx, shape, t)
