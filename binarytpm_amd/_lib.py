@@ -71,6 +71,15 @@ class BatchOpts(C.Structure):
     ]
 
 
+class Geometry(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("ntau", C.c_int), ("nv_primary", C.c_int),
+        ("equpos", C.POINTER(C.c_double)), ("facets", C.POINTER(C.c_int)),
+        ("rev_primary", C.c_double), ("rev_secondary", C.c_double),
+        ("pol", C.c_double * 3), ("rot", C.c_double * 2), ("au_km", C.c_double),
+    ]
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("wall_s", C.c_double), ("solves", C.c_int), ("converged", C.c_int),
@@ -121,6 +130,8 @@ PROTOTYPES = [
                                        C.POINTER(C.c_int)]),
     ("btpm_server_stop", C.c_int, [_vp, C.POINTER(BatchStats)]),
     ("btpm_crater_contract", C.c_int, [_vp, _fp, _fp, C.c_int, _fp]),
+    ("btpm_geometry_illumination", C.c_int,
+     [_vp, C.POINTER(Geometry), C.POINTER(C.c_int), _fp, _fp, _fp]),
 ]
 
 _LIB = None

@@ -30,6 +30,8 @@
 #define BTPM_SH_ELL_MAX_PAD 4
 // halves per packed shape->crater view-factor tile: [hi, lo][112 sub-facets][32 entries]
 #define BTPM_DRF_TILE (2 * 112 * 32)
+// doubles per facet record of the shadow tests (geometry.hip)
+#define BTPM_GEOM_REC 20
 // largest accepted r = dtau/dz^2 (FTCS is stable for r <= 1/2): the
 // conduction kernels integrate the depth profile in a variable scaled by
 // (1 - 2r)^-k over 8-step blocks, which must stay inside the FP32 range

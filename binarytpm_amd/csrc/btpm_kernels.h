@@ -76,6 +76,16 @@ extern "C" __global__ void btpm_finish_scatter(float* F, const float* p, const f
 extern "C" __global__ void btpm_status_kernel(const int* ids, const btpm_set_dev* sets,
                                               btpm_status_dev* status);
 
+// geometry.hip
+extern "C" __global__ void btpm_geom_facets(const double* equpos, const int* fv, int nf,
+                                            int nv_primary, const double* trig,
+                                            const double* frame, int s0, int nsteps, double* rec,
+                                            int* lit, float* cosang, float* sun);
+extern "C" __global__ void btpm_geom_shadow(const double* rec, int nf, int nsteps, int* lit);
+extern "C" __global__ void btpm_geom_finish(const int* lit, const float* cosang, const float* sun,
+                                            int nf, int nsteps, int s0, int ntau, float solar,
+                                            int* shadow, float* illum, float* sun_out);
+
 // stepwise.hip
 extern "C" __global__ void btpm_step_profile(float* temp, float* prof, int ncols, int to_records);
 extern "C" __global__ void btpm_step_fd(btpm_set_dev S, float* temp);

@@ -167,6 +167,26 @@ def binary_illumination(Vp, Fp, axes_p, Vs, Fs, radius_s, sep, ntau, nrev_p, nre
     return BinaryIllum(Dp, Ds, Mp, Ms)
 
 
+def binary_shape(subdiv_p: int = 2, subdiv_s: int = 1, axes_p=(0.6, 0.5, 0.45),
+                 radius_s: float = 0.3, sep: float = 1.2, dent: float = 0.0):
+    """A binary as one vertex list for the geometry producer: the primary
+    ellipsoid (vertices first; with dent > 0 its radius is modulated by
+    1 - dent * max(0, sin(3 lon) cos(lat))^2, which makes concave troughs that
+    shadow their neighbours) and a spherical secondary offset by sep along x
+    in its own equatorial frame (its orbit).  Returns (equpos [nv][3] km,
+    facets [nf][3], nv_primary)."""
+    Vp, Fp = ellipsoid(subdiv_p, axes_p)
+    if dent > 0.0:
+        lon = np.arctan2(Vp[:, 1], Vp[:, 0])
+        lat = np.arcsin(np.clip(Vp[:, 2] / np.linalg.norm(Vp, axis=1), -1.0, 1.0))
+        Vp = Vp * (1.0 - dent * np.maximum(0.0, np.sin(3.0 * lon) * np.cos(lat)) ** 2)[:, None]
+    Vs, Fs = ellipsoid(subdiv_s, (radius_s, radius_s, radius_s))
+    Vs = Vs + np.array([sep, 0.0, 0.0])
+    equ = np.vstack([Vp, Vs]).astype(np.float64)
+    fv = np.vstack([Fp, Fs + len(Vp)]).astype(np.int32)
+    return equ, fv, len(Vp)
+
+
 # --------------------------------------------------------------------------
 # sparse facet view factors
 # --------------------------------------------------------------------------

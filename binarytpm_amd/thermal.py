@@ -15,7 +15,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, Params, SolveInfo, check, lib)
+from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, Geometry, Params, SolveInfo, check,
+                   lib)
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -298,6 +299,47 @@ class Server:
         if h:
             check(lib().btpm_server_stop(h, C.byref(st)))
         return st
+
+
+AU_KM = 149.6e6  # astronomical unit of the reference, km
+
+
+@dataclass
+class Illumination:
+    shadow: np.ndarray     # [nf][ntau] int32: 1 lit, 0 back-facing, shadowed or eclipsed
+    illum: np.ndarray      # [nf][ntau] float32: direct illumination, W/m^2
+    sun_facet: Optional[np.ndarray]  # [nf][ntau][3] unit Sun direction, facet frame
+    solar: float
+
+
+def geometry_illumination(ctx: Context, equpos, facets, ntau: int, pol, rot, nv_primary=None,
+                          rev_primary=1.0, rev_secondary=1.0, au_km=AU_KM,
+                          sun_facet=True) -> Illumination:
+    """Direct illumination and shadows of a (binary) shape over one cycle on
+    the GPU: vertices equpos [nv][3] (km, each body's equatorial frame, the
+    first nv_primary on the primary), facets [nf][3]; pol = (r AU, ecliptic
+    latitude, longitude) and rot = (pole latitude, longitude) in radians."""
+    eq = np.ascontiguousarray(equpos, dtype=np.float64)
+    fv = np.ascontiguousarray(facets, dtype=np.int32)
+    nv, nf = eq.shape[0], fv.shape[0]
+    g = Geometry()
+    g.nv, g.nf, g.ntau = nv, nf, int(ntau)
+    g.nv_primary = nv if nv_primary is None else int(nv_primary)
+    g.equpos = eq.ctypes.data_as(C.POINTER(C.c_double))
+    g.facets = fv.ctypes.data_as(C.POINTER(C.c_int))
+    g.rev_primary, g.rev_secondary = float(rev_primary), float(rev_secondary)
+    g.pol[:] = [float(x) for x in pol]
+    g.rot[:] = [float(x) for x in rot]
+    g.au_km = float(au_km)
+    shadow = np.empty((nf, ntau), np.int32)
+    illum = np.empty((nf, ntau), np.float32)
+    sun = np.empty((nf, ntau, 3), np.float32) if sun_facet else None
+    solar = C.c_float()
+    check(lib().btpm_geometry_illumination(ctx.handle, C.byref(g),
+                                           shadow.ctypes.data_as(C.POINTER(C.c_int)),
+                                           _ptr(illum), _ptr(sun) if sun is not None else None,
+                                           C.byref(solar)))
+    return Illumination(shadow, illum, sun, float(solar.value))
 
 
 def crater_contract(ctx: Context, vf, e) -> np.ndarray:

@@ -173,6 +173,26 @@ typedef struct {
     long long fast_rotations;    /* extra fast-lane rotations enqueued      */
 } btpm_batch_stats;
 
+/* Illumination geometry and shadows of a (binary) shape over one cycle: the
+ * producer of the direct facet illumination the solves consume
+ * (asteroidgeometry, astshadowtest / astshadow and the direct term of
+ * singlescattering, T175706/thermal.cpp:1272-1471, 1612-1707, 1757; the
+ * reference runs them on the host).  Vertices equpos [nv][3] in km in each
+ * body's equatorial frame (the secondary's include its orbit), the first
+ * nv_primary on the primary; facets [nf][3]; each body turns rev_primary /
+ * rev_secondary times per cycle about its z axis; pol = (r in AU, ecliptic
+ * latitude, longitude) of the system and rot = (pole ecliptic latitude,
+ * longitude) in radians; au_km = 149.6e6 in the reference.                  */
+typedef struct {
+    int nv, nf, ntau, nv_primary;
+    const double *equpos;
+    const int *facets;
+    double rev_primary, rev_secondary;
+    double pol[3];
+    double rot[2];
+    double au_km;
+} btpm_geometry;
+
 /* ---- library ------------------------------------------------------------ */
 int btpm_version(void);
 const char *btpm_last_error(void);
@@ -272,6 +292,14 @@ int btpm_server_stop(btpm_server *srv, btpm_batch_stats *stats);
  * q[n][j] = sum_i VF(j->i) e[n][i] for n < ncol, computed with the same
  * fp16 hi/lo MFMA split; e is [ncol][100], q is [ncol][100].                */
 int btpm_crater_contract(btpm_ctx *ctx, const float *vf, const float *e, int ncol, float *q);
+
+/* Outputs [nf][ntau] (host memory): shadow (1 lit, 0 back-facing or shadowed
+ * by a facet nearer the Sun, including the partner body: eclipses), illum =
+ * solar cos(angle) shadow in W/m^2 (may be NULL), sun_facet [nf][ntau][3] the
+ * unit Sun direction in each facet's frame (x along its first edge, z the
+ * normal; may be NULL), solar = 1367 / r^2 (may be NULL).                   */
+int btpm_geometry_illumination(btpm_ctx *ctx, const btpm_geometry *g, int *shadow, float *illum,
+                               float *sun_facet, float *solar);
 
 #ifdef __cplusplus
 }

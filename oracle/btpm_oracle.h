@@ -175,6 +175,30 @@ int orc_crater_solve(const orc_params *p, int nfac, int ns, const double *vf,
 double orc_newton_sbc(double src, double m1, double t1, double es, double t0,
                       double tol, int cap, int *iters);
 
+/* Illumination geometry and shadows of a (binary) shape over one cycle
+ * (geometry.c; T175706/thermal.cpp:1272-1471, 1612-1707, 1757).  Vertices
+ * [nv][3] in km in each body's equatorial frame (the secondary's include its
+ * orbit), the first nv_primary of the primary; facets fv [nf][3]; each body
+ * turns rev_primary / rev_secondary times per cycle about its z axis; pol =
+ * (r in AU, ecliptic latitude, longitude in rad) of the system, rot = (pole
+ * latitude, longitude in rad), au_km the astronomical unit.
+ * Steps s_begin .. s_begin + s_count - 1 are computed; outputs
+ * [nf][s_count]: shadow (1 lit, 0 back-facing or shadowed), cosang (may
+ * be NULL), illum = solar cos(angle) shadow (may be NULL), sun_facet
+ * [nf][ntau][3] the unit Sun direction in the facet frame (may be NULL);
+ * solar_out = 1367 / r^2.                                                   */
+typedef struct {
+    int nv, nf, ntau, nv_primary;
+    const double *equpos;
+    const int *fv;
+    double rev_primary, rev_secondary;
+    double pol[3], rot[2];
+    double au_km;
+    int s_begin, s_count;    /* steps computed (outputs hold s_count steps) */
+} orc_geometry_in;
+void orc_geometry(const orc_geometry_in *in, int *shadow, double *cosang, float *illum,
+                  float *sun_facet, float *solar_out);
+
 #ifdef __cplusplus
 }
 #endif

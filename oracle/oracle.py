@@ -41,6 +41,19 @@ class OrcStats(C.Structure):
     ]
 
 
+class OrcGeometryIn(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("ntau", C.c_int), ("nv_primary", C.c_int),
+        ("equpos", C.POINTER(C.c_double)), ("fv", C.POINTER(C.c_int)),
+        ("rev_primary", C.c_double), ("rev_secondary", C.c_double),
+        ("pol", C.c_double * 3), ("rot", C.c_double * 2), ("au_km", C.c_double),
+        ("s_begin", C.c_int), ("s_count", C.c_int),
+    ]
+
+
+AU_KM = 149.6e6  # T175706/thermal.cpp:13
+
+
 def _load(name):
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
@@ -310,3 +323,42 @@ def refcpu_crater_solve(vf, albedo, ti, illum, rgh_, rot_period, tacc=0.01, numr
              _ptr(vf, _fp), _ptr(albedo, _fp), _ptr(ti, _fp), _ptr(numrevs, _fp),
              _ptr(illum, _fp), _ptr(rgh_, _fp), _ptr(out, _fp), nthreads, C.byref(steps))
     return out, rev, steps.value
+
+
+@dataclass
+class Geometry:
+    shadow: np.ndarray     # [nf][ntau] int32: 1 lit, 0 back-facing or shadowed
+    cosang: np.ndarray     # [nf][ntau] float64
+    illum: np.ndarray      # [nf][ntau] float32: solar cos(angle) shadow
+    sun_facet: np.ndarray  # [nf][ntau][3] float32: unit Sun direction, facet frame
+    solar: float
+
+
+def geometry(equpos, fv, ntau, pol, rot, nv_primary=None, rev_primary=1.0, rev_secondary=1.0,
+             au_km=AU_KM, steps=None):
+    """Illumination geometry and shadows (geometry.c): the shape's vertices
+    [nv][3] (km) and facets [nf][3], the first nv_primary vertices on the
+    primary; pol = (r AU, ecliptic lat, lon), rot = (pole lat, lon), radians.
+    steps = (first, count) computes that range of the cycle only."""
+    equpos = np.ascontiguousarray(equpos, dtype=np.float64)
+    fv = np.ascontiguousarray(fv, dtype=np.int32)
+    nv, nf = equpos.shape[0], fv.shape[0]
+    g = OrcGeometryIn()
+    g.nv, g.nf, g.ntau = nv, nf, int(ntau)
+    g.nv_primary = nv if nv_primary is None else int(nv_primary)
+    g.equpos = _ptr(equpos, _dp)
+    g.fv = _ptr(fv, _ip)
+    g.rev_primary, g.rev_secondary = float(rev_primary), float(rev_secondary)
+    g.pol[:] = [float(x) for x in pol]
+    g.rot[:] = [float(x) for x in rot]
+    g.au_km = float(au_km)
+    g.s_begin, g.s_count = (0, int(ntau)) if steps is None else (int(steps[0]), int(steps[1]))
+    ns = g.s_count
+    shadow = np.zeros((nf, ns), np.int32)
+    cosang = np.zeros((nf, ns), np.float64)
+    illum = np.zeros((nf, ns), np.float32)
+    sun = np.zeros((nf, ns, 3), np.float32)
+    solar = C.c_float()
+    lib().orc_geometry(C.byref(g), _ptr(shadow, _ip), _ptr(cosang, _dp), _ptr(illum, _fp),
+                       _ptr(sun, _fp), C.byref(solar))
+    return Geometry(shadow, cosang, illum, sun, float(solar.value))

@@ -335,3 +335,142 @@ def test_refcpu_baseline_agrees_with_oracle(small_body):
     st, rev, _ = O.refcpu_smooth_solve(case.illum, 0.1, 500.0, 650, case.rot_period, tacc=0.01)
     assert np.abs(st - r.stemp).max() < 0.05
     assert abs(rev - r.rotations) <= 2
+
+
+# --------------------------------------------------------------------------
+# illumination geometry and shadows (geometry.c)
+# --------------------------------------------------------------------------
+POS = (1.16, math.radians(-1.95), math.radians(198.31))   # r AU, ecliptic lat, lon
+POLE_REF = (math.radians(-83.0), math.radians(266.0))      # the reference's pole
+
+
+def _body_frame(rot):
+    """Equatorial -> ecliptic rotation built from its definition: the body's
+    z axis along the pole (lat, lon); used to place vertices independently."""
+    lat, lon = rot
+    z = np.array([math.cos(lat) * math.cos(lon), math.cos(lat) * math.sin(lon), math.sin(lat)])
+    return z
+
+
+def test_geometry_convex_body_only_faces_away():
+    """On a convex body nothing shadows anything: a facet is unlit exactly
+    when the Sun is behind its plane, and illum = solar cos shadow with
+    solar = 1367 / r^2."""
+    V, F = inputs.ellipsoid(2, (0.6, 0.5, 0.45))
+    for rot in [(math.pi / 2, 0.0), POLE_REF]:
+        g = O.geometry(V, F, 96, POS, rot, rev_primary=2.0)
+        assert np.array_equal(g.shadow, (g.cosang >= 0.0).astype(np.int32))
+        assert 0.45 < g.shadow.mean() < 0.55
+        assert g.solar == np.float32(1367.0 / POS[0] ** 2)
+        ref = (np.float32(g.solar) * g.cosang.astype(np.float32)) * g.shadow.astype(np.float32)
+        assert np.array_equal(g.illum, ref)
+
+
+def test_geometry_subsolar_point_turns_with_the_body():
+    """Closed form for a sphere with its pole along ecliptic north: at step s
+    the body has turned by 2 pi s n / ntau, so the facet nearest the Sun sits
+    at body longitude lon_sun - 2 pi s n / ntau, lon_sun = system longitude +
+    pi; the facet of largest cos(angle) must be within a facet of it."""
+    V, F = inputs.ellipsoid(3, (1.0, 1.0, 1.0))
+    n, c, _, _ = inputs.facet_geometry(V, F)
+    ntau, nrev = 48, 2.0
+    g = O.geometry(V, F, ntau, (1.0, 0.0, math.radians(40.0)), (math.pi / 2, 0.0), rev_primary=nrev)
+    for s in range(0, ntau, 5):
+        want = math.radians(40.0) + math.pi - 2 * math.pi * s * nrev / ntau
+        d = np.array([math.cos(want), math.sin(want), 0.0])
+        best = int(np.argmax(g.cosang[:, s]))
+        assert float(n[best] @ d) > math.cos(math.radians(8.0))
+        assert g.cosang[best, s] > math.cos(math.radians(8.0))
+
+
+def test_pole_frame_maps_z_to_the_pole():
+    """The equatorial -> ecliptic matrix of the reference (thermal.cpp:1286-1296)
+    maps the body's z axis onto the pole (lat, lon).  The six facets around
+    the +z apex of a hexagonal bipyramid have a mean normal k z (k < 1), so
+    their mean cos(angle) is k (z . sun) at every step as the body turns about
+    the pole; it changes sign with z . sun."""
+    # hexagonal bipyramid: apexes on the z axis, the top fan symmetric about it
+    ang = np.arange(6) * (math.pi / 3)
+    V = np.vstack([[0.0, 0.0, 1.0], np.stack([np.cos(ang), np.sin(ang), np.zeros(6)], 1),
+                   [0.0, 0.0, -1.0]])
+    F = np.array([[0, 1 + i, 1 + (i + 1) % 6] for i in range(6)] +
+                 [[7, 1 + (i + 1) % 6, 1 + i] for i in range(6)], np.int32)
+    ring = np.arange(6)
+    n, _, _, _ = inputs.facet_geometry(V, F)
+    assert np.allclose(n[ring].mean(axis=0)[:2], 0.0) and n[ring].mean(axis=0)[2] > 0.5
+    k = float(np.linalg.norm(n[ring].mean(axis=0)))
+    sun = -np.array([1.0, 0.0, 0.0])            # system on the +x axis
+    for rot in [POLE_REF, (math.radians(35.0), math.radians(120.0)),
+                (math.radians(-20.0), math.radians(10.0))]:
+        z = _body_frame(rot)
+        g = O.geometry(V, F, 12, (1.0, 0.0, 0.0), rot, rev_primary=1.0)
+        m = g.cosang[ring].mean(axis=0)
+        assert np.allclose(m, k * float(z @ sun), atol=1e-9)
+
+
+def _brute_force_shadow(equ, fv, npv, ntau, pos, rot, revs):
+    """Independent shadow flags: vertices placed with Rodrigues-free matrices
+    built from the pole, Moller-Trumbore line/triangle tests in the body
+    frame (Sun at -centre), the reference's rule that only facets nearer the
+    Sun (ties to the lower index) can shadow."""
+    lat, lon = rot
+    z = _body_frame(rot)
+    # the reference's x axis: column 0 of its matrix, from its closed form
+    gam = math.acos(math.cos(lat) * math.sin(abs(lon)))
+    x = np.array([math.sin(lat), 0.0, -math.cos(lat) * math.cos(lon)]) / math.sin(gam)
+    y = np.cross(z, x)
+    M = np.stack([x, y, z], axis=1)
+    r, blat, blon = pos
+    cen = O.AU_KM * r * np.array([math.cos(blat) * math.cos(blon), math.cos(blat) * math.sin(blon),
+                                  math.sin(blat)])
+    nf = len(fv)
+    out = np.zeros((nf, ntau), np.int32)
+    for s in range(ntau):
+        P = np.empty_like(equ)
+        for body, sl in ((0, slice(0, npv)), (1, slice(npv, None))):
+            a = 2 * math.pi * s * revs[body] / ntau
+            Rz = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+            P[sl] = equ[sl] @ Rz.T @ M.T
+        v0, v1, v2 = P[fv[:, 0]], P[fv[:, 1]], P[fv[:, 2]]
+        mid = (v0 + v1 + v2) / 3.0
+        nrm = np.cross(v1 - v0, v2 - v0)
+        dist = np.linalg.norm(mid + cen, axis=1)
+        lit = (np.einsum("ij,ij->i", nrm, -(mid + cen)) >= 0.0)
+        e1, e2 = v1 - v0, v2 - v0
+        for G in np.where(lit)[0]:
+            nearer = (dist < dist[G]) | ((dist == dist[G]) & (np.arange(nf) < G))
+            D = mid[G] + cen                      # Sun -> centroid, body frame
+            O_ = -cen
+            pv = np.cross(D, e2[nearer])
+            det = np.einsum("ij,ij->i", e1[nearer], pv)
+            T = O_ - v0[nearer]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                u = np.einsum("ij,ij->i", T, pv) / det
+                q = np.cross(T, e1[nearer])
+                v = (q @ D) / det
+            if np.any((u >= 0) & (v >= 0) & (u + v <= 1)):
+                lit[G] = False
+        out[:, s] = lit
+    return out
+
+
+def test_geometry_shadows_match_brute_force():
+    """A dented primary (concave troughs shadow their neighbours) and a close
+    secondary in the Sun's plane (mutual eclipses): the flags equal an
+    independent line/triangle test wherever the Sun is more than 1e-6 rad
+    above the facet's plane (at grazing incidence the Sun->centroid line runs
+    along the neighbouring facets and the two constructions may round either
+    way; the illumination at stake there is below 2e-3 W/m^2), and the pairs
+    test shadows more than the back faces alone."""
+    equ, fv, npv = inputs.binary_shape(2, 1, sep=1.2, dent=0.35)
+    ntau, revs = 40, (3.0, 1.0)
+    g = O.geometry(equ, fv, ntau, (1.0, 0.0, 0.0), (math.pi / 2, 0.0), nv_primary=npv,
+                   rev_primary=revs[0], rev_secondary=revs[1])
+    bf = _brute_force_shadow(equ, fv, npv, ntau, (1.0, 0.0, 0.0), (math.pi / 2, 0.0), revs)
+    backface_only = int((g.cosang >= 0.0).sum())
+    print(f"shadowed beyond back faces: {backface_only - int(g.shadow.sum())} facet-steps")
+    assert backface_only - int(g.shadow.sum()) > 50
+    clear = np.abs(g.cosang) > 1e-6
+    print(f"grazing facet-steps excluded: {int((~clear).sum())}, disagreeing there: "
+          f"{int(((g.shadow != bf) & ~clear).sum())}")
+    assert np.array_equal(g.shadow[clear], bf[clear])
