@@ -338,6 +338,59 @@ def serve_phase(ctx, n_req, rate, seed=7, subdivs=(3, 4, 5, 6), seeds_per_size=2
     }
 
 
+def geometry_phase(ctx, cpu=True):
+    """Illumination geometry and shadow producer at the config-1 size: the
+    5120-facet primary and a 1280-facet secondary 3 km away, 15:4 cycle over
+    9750 steps, Sun in the orbit plane (eclipses every half orbit).  GPU wall
+    time of the whole call (inputs uploaded, [nf][ntau] outputs copied back);
+    the FP64 oracle (one core) timed on sampled steps and checked bit for bit
+    against the GPU on them."""
+    import math
+
+    from binarytpm_amd import inputs
+    from binarytpm_amd import thermal as th
+    equ, fv, npv = inputs.binary_shape(4, 3, axes_p=(0.6, 0.5, 0.45), radius_s=0.3, sep=3.0)
+    ntau = 9750
+    kw = dict(nv_primary=npv, rev_primary=15.0, rev_secondary=4.0)
+    pos, pole = (1.1, 0.0, 0.0), (math.pi / 2, 0.0)
+    th.geometry_illumination(ctx, equ, fv[:64], 16, pos, pole, sun_facet=False, **kw)  # warm-up
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        g = th.geometry_illumination(ctx, equ, fv, ntau, pos, pole, sun_facet=False, **kw)
+        walls.append(time.perf_counter() - t0)
+    gpu_s = min(walls)
+    nf = len(fv)
+    out = {
+        "workload": (f"{nf} facets (5120 + 1280, 3 km apart) x {ntau} steps of a 15:4 cycle, "
+                     "self-shadowing and mutual eclipses"),
+        "gpu_s": gpu_s,
+        "facet_steps_per_s": nf * ntau / gpu_s,
+        "facet_pairs_per_s": nf * nf / 2 * ntau / gpu_s,
+        "unlit_fraction": float(1.0 - g.shadow.mean()),
+    }
+    log(f"geometry: {nf} x {ntau} in {gpu_s:.3f} s (best of 3)")
+    if cpu:
+        from oracle import oracle as O
+        sample = list(range(0, ntau, ntau // 40))[:40]
+        t0 = time.perf_counter()
+        mism = 0
+        for s in sample:
+            ref = O.geometry(equ, fv, ntau, pos, pole, steps=(s, 1), **kw)
+            mism += int((ref.shadow[:, 0] != g.shadow[:, s]).sum())
+        per_step = (time.perf_counter() - t0) / len(sample)
+        out["cpu_baseline"] = {
+            "value": per_step * ntau, "unit": "s per cycle", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (oracle/geometry.c, one core) on {len(sample)} of {ntau} "
+                      f"steps, {per_step * 1e3:.1f} ms per step, extrapolated to the cycle",
+        }
+        out["vs_cpu"] = per_step * ntau / gpu_s
+        out["shadow_mismatches_on_sample"] = mism
+        log(f"geometry: oracle {per_step * 1e3:.1f} ms per step, GPU {per_step * ntau / gpu_s:.0f}x; "
+            f"{mism} mismatches on {len(sample)} steps")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,6 +406,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-geometry", action="store_true",
+                    help="skip the illumination geometry and shadow producer measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales HBM traffic counters)")
@@ -490,6 +545,8 @@ def main():
         shape.close()
         if not args.no_serve:
             line["serving"] = serve_phase(ctx, args.serve_n, args.serve_rate, rps=args.serve_rps)
+        if not args.no_geometry:
+            line["illumination_producer"] = geometry_phase(ctx, cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 
