@@ -80,6 +80,24 @@ class Geometry(C.Structure):
     ]
 
 
+class VfMesh(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("vtx", C.POINTER(C.c_float)),
+        ("facets", C.POINTER(C.c_int)), ("nsub", C.c_int), ("cull", C.c_int),
+        ("refine_min", C.c_float),
+    ]
+
+
+class DrfIn(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("vtx", C.POINTER(C.c_double)),
+        ("facets", C.POINTER(C.c_int)), ("rowptr", C.POINTER(C.c_int)),
+        ("vf", C.POINTER(C.c_float)), ("dir", C.POINTER(C.c_float)), ("ncv", C.c_int),
+        ("ncf", C.c_int), ("crater_vtx", C.POINTER(C.c_float)),
+        ("crater_facets", C.POINTER(C.c_int)), ("m", C.c_int), ("crater_radius", C.c_float),
+    ]
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("wall_s", C.c_double), ("solves", C.c_int), ("converged", C.c_int),
@@ -132,6 +150,13 @@ PROTOTYPES = [
     ("btpm_crater_contract", C.c_int, [_vp, _fp, _fp, C.c_int, _fp]),
     ("btpm_geometry_illumination", C.c_int,
      [_vp, C.POINTER(Geometry), C.POINTER(C.c_int), _fp, _fp, _fp]),
+    ("btpm_vf_build", C.c_int, [_vp, C.POINTER(VfMesh), C.POINTER(_vp)]),
+    ("btpm_vf_counts", C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                 C.POINTER(C.c_longlong)]),
+    ("btpm_vf_fetch", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _fp, _fp, _fp, _fp,
+                                _fp]),
+    ("btpm_vf_free", None, [_vp]),
+    ("btpm_crater_viewfactors", C.c_int, [_vp, C.POINTER(DrfIn), _fp]),
 ]
 
 _LIB = None

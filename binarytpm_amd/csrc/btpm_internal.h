@@ -32,6 +32,14 @@
 #define BTPM_DRF_TILE (2 * 112 * 32)
 // doubles per facet record of the shadow tests (geometry.hip)
 #define BTPM_GEOM_REC 20
+// floats per facet record of the view-factor producer (viewfactor.hip)
+#define BTPM_VF_REC 28
+// the facing test acosf(c) <= float(pi)/2 of the view-factor producer holds
+// exactly for BTPM_VF_COS_MIN <= c <= 1 under glibc's acosf (checked over
+// every float); NaN fails both
+#define BTPM_VF_COS_MIN (-0x1.bbbd2cp-24f)
+// largest sub-facet split per edge of a refined pair (n*n sub-facets)
+#define BTPM_VF_NSUB_MAX 32
 // largest accepted r = dtau/dz^2 (FTCS is stable for r <= 1/2): the
 // conduction kernels integrate the depth profile in a variable scaled by
 // (1 - 2r)^-k over 8-step blocks, which must stay inside the FP32 range

@@ -86,6 +86,30 @@ extern "C" __global__ void btpm_geom_finish(const int* lit, const float* cosang,
                                             int nf, int nsteps, int s0, int ntau, float solar,
                                             int* shadow, float* illum, float* sun_out);
 
+// viewfactor.hip
+extern "C" __global__ void btpm_vf_prep(const float* vtx, const int* fv, int nf, float* rec);
+extern "C" __global__ void btpm_vf_facing(const float* rec, int nf, const int* rowptr, int* count,
+                                          int* col, float* vf, float* vec);
+extern "C" __global__ void btpm_vf_hidden(const float* rec, const int* rowptr1, const int* col1,
+                                          const int2* work, unsigned char* keep);
+extern "C" __global__ void btpm_vf_kept(const int* rowptr1, const unsigned char* keep, int* count);
+extern "C" __global__ void btpm_vf_compact(const int* rowptr1, const int* col1, const float* vf1,
+                                           const float* vec1, const unsigned char* keep,
+                                           const int* rowptr, int* col, float* vf, float* vec,
+                                           int refine, float refine_min, int* nref,
+                                           int2* reflist);
+extern "C" __global__ void btpm_vf_refine(const float* rec, const int* col, const int2* reflist,
+                                          int nsub, float* vf, float* vec);
+extern "C" __global__ void btpm_vf_finish(const int* rowptr, const float* vf, const float* vec,
+                                          const float* rec, float* dir, float* total,
+                                          float* photon);
+extern "C" __global__ void btpm_drf_frames(const double* vtx, const int* fv, int nf, double* frame);
+extern "C" __global__ void btpm_drf_rows(const int* rowptr, int* row);
+extern "C" __global__ void btpm_drf(const double* frame, const int* row, const float* vfv,
+                                    const float* dir, const float* crater, const float* mul,
+                                    int ncf, int nsamp, float radius, float step, long long n,
+                                    float* drf);
+
 // stepwise.hip
 extern "C" __global__ void btpm_step_profile(float* temp, float* prof, int ncols, int to_records);
 extern "C" __global__ void btpm_step_fd(btpm_set_dev S, float* temp);

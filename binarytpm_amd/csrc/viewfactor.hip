@@ -1,0 +1,757 @@
+// viewfactor.hip -- the view factors of the Thermal stage's self-heating
+// terms, on the GPU:
+//
+//  * facet view factors of a shape (the Shape_Selfheating producer,
+//    SH/main.cpp = 3905/Shape_Selfheating/main.cpp): every ordered facet pair
+//    that faces both ways (:628-735), the occlusion cull (hidden, :187-368),
+//    the n x n sub-facet refinement of pairs with VF >= 0.05
+//    (betterviewfactors, :374-622) and the per-facet totals (:817-835);
+//  * shape->crater view factors (calculate_surface_global_viewfactors,
+//    T3905/main.cpp:1054-1181): per facet-pair entry and crater sub-facet, the
+//    view factor times the crater sub-facet's cosine and its m*m-sample rim
+//    visibility.
+//
+// The producer computes in float and its list membership is decided by signs
+// and comparisons of float expressions, so every expression is evaluated in
+// the reference's operation order with no fused multiply-adds (the pragma
+// below); + - * / and sqrt are IEEE correctly rounded, so the lists, the
+// refined sums and the shape->crater factors equal the float restatement bit
+// for bit.  The one transcendental decision, acosf(c) <= float(pi)/2, is the
+// exact interval BTPM_VF_COS_MIN <= c <= 1; the centroid view factor itself
+// keeps the reference's cosf(acosf(c)), which the device libm evaluates
+// within an ulp or two of the host's.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include "btpm_internal.h"
+#include "btpm_kernels.h"
+
+namespace {
+constexpr int kVR = BTPM_VF_REC;  // floats per facet record
+// record: centroid[3], unnormalised normal[3], |normal|, area, vertices
+// v0 v1 v2 [3][3], edge vectors v1-v0, v0-v2, v2-v1 [3][3], pad[2]
+constexpr int kMid = 0, kNrm = 3, kNlen = 6, kArea = 7, kV0 = 8, kV1 = 11, kV2 = 14, kE1 = 17,
+              kE2 = 20, kE3 = 23;
+constexpr int kHidChunk = 256;    // partner records staged per LDS chunk
+
+__device__ __forceinline__ bool facing_cos(float c) {
+  return c >= BTPM_VF_COS_MIN && c <= 1.0f;
+}
+
+// SH/main.cpp:689-706: separation, its length and the two cosines
+__device__ __forceinline__ void pair_cos(const float* __restrict__ ri, const float* __restrict__ rj,
+                                         float sep[3], float& sd, float& ci, float& cj) {
+  sep[0] = rj[kMid] - ri[kMid];
+  sep[1] = rj[kMid + 1] - ri[kMid + 1];
+  sep[2] = rj[kMid + 2] - ri[kMid + 2];
+  sd = sqrtf((sep[0] * sep[0]) + (sep[1] * sep[1]) + (sep[2] * sep[2]));
+  ci = ((ri[kNrm] * sep[0]) + (ri[kNrm + 1] * sep[1]) + (ri[kNrm + 2] * sep[2])) /
+       (ri[kNlen] * sd);
+  cj = (((-1.0f * rj[kNrm]) * sep[0]) + ((-1.0f * rj[kNrm + 1]) * sep[1]) +
+        ((-1.0f * rj[kNrm + 2]) * sep[2])) / (rj[kNlen] * sd);
+}
+
+// vertex (row, col) of a triangle split n x n (SH/main.cpp:466-479, rows
+// from vertex 2 towards vertex 0, 1-based)
+__device__ __forceinline__ void sub_vertex(const float* __restrict__ r, const float t1[3],
+                                           const float t2[3], int row, int col, float p[3]) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    p[q] = r[kV2 + q] + ((float)(col - 1) * t1[q]) - ((float)(row - 1) * t2[q]);
+}
+
+// centroid of sub-facet l (SH/main.cpp:481-539: row j holds 2j-1 facets from
+// index (j-1)^2; pairs of facets per column k, then the row's last facet)
+__device__ void sub_centroid(const float* __restrict__ r, int n, int l, float m[3]) {
+  float t1[3], t2[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    t1[q] = (r[kV1 + q] - r[kV0 + q]) / (float)n;
+    t2[q] = (r[kV2 + q] - r[kV0 + q]) / (float)n;
+  }
+  int j = (int)sqrtf((float)l);
+  while (j * j > l) --j;
+  while ((j + 1) * (j + 1) <= l) ++j;
+  const int row = j + 1, rr = l - j * j;
+  int ar, ac, br, bc, cr, cc;
+  if (rr == 2 * (row - 1)) {
+    ar = row + 1, ac = row, br = row + 1, bc = row + 1, cr = row, cc = row;
+  } else {
+    const int k = rr / 2 + 1;
+    if ((rr & 1) == 0)
+      ar = row, ac = k + 1, br = row, bc = k, cr = row + 1, cc = k;
+    else
+      ar = row + 1, ac = k, br = row + 1, bc = k + 1, cr = row, cc = k + 1;
+  }
+  float pa[3], pb[3], pc[3];
+  sub_vertex(r, t1, t2, ar, ac, pa);
+  sub_vertex(r, t1, t2, br, bc, pb);
+  sub_vertex(r, t1, t2, cr, cc, pc);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) m[q] = (pa[q] + pb[q] + pc[q]) / 3.0f;
+}
+}  // namespace
+
+// SH/main.cpp:750-808: one facet per lane
+extern "C" __global__ __launch_bounds__(256) void btpm_vf_prep(const float* __restrict__ vtx,
+                                                               const int* __restrict__ fv, int nf,
+                                                               float* __restrict__ rec) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  float p[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float* v = vtx + 3 * (size_t)fv[3 * f + a];
+    p[a][0] = v[0], p[a][1] = v[1], p[a][2] = v[2];
+  }
+  float* r = rec + (size_t)f * kVR;
+  float e0[3], e1[3], n[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    r[kMid + k] = (p[0][k] + p[1][k] + p[2][k]) / 3.0f;
+    e0[k] = p[1][k] - p[0][k];
+    e1[k] = p[2][k] - p[0][k];
+  }
+  n[0] = (e0[1] * e1[2]) - (e0[2] * e1[1]);
+  n[1] = (e0[2] * e1[0]) - (e0[0] * e1[2]);
+  n[2] = (e0[0] * e1[1]) - (e0[1] * e1[0]);
+  const float nl = sqrtf((n[0] * n[0]) + (n[1] * n[1]) + (n[2] * n[2]));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    r[kNrm + k] = n[k];
+    r[kV0 + k] = p[0][k];
+    r[kV1 + k] = p[1][k];
+    r[kV2 + k] = p[2][k];
+    r[kE1 + k] = e0[k];
+    r[kE2 + k] = -1.0f * e1[k];
+    r[kE3 + k] = p[2][k] - p[1][k];
+  }
+  r[kNlen] = nl;
+  r[kArea] = nl / 2.0f;
+  r[26] = 0.0f;
+  r[27] = 0.0f;
+}
+
+// SH/main.cpp:641-735: row I per workgroup, partners J ascending in chunks of
+// 256 lanes.  Count mode (rowptr == nullptr) stores the row's number of
+// facing partners; fill mode stores them in order from rowptr[I] with the
+// centroid view factor (:709, clamped at 0) and the vector A_I VF d/|d|.
+extern "C" __global__ __launch_bounds__(256) void btpm_vf_facing(
+    const float* __restrict__ rec, int nf, const int* __restrict__ rowptr, int* __restrict__ count,
+    int* __restrict__ col, float* __restrict__ vf, float* __restrict__ vec) {
+  __shared__ int wcount[4];
+  const int I = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float* ri = rec + (size_t)I * kVR;
+  const float PI = 4.0f * atanf(1.0f);
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  int base = rowptr ? rowptr[I] : 0;
+  int total = 0;
+  for (int J0 = 0; J0 < nf; J0 += 256) {
+    const int J = J0 + (int)threadIdx.x;
+    float sep[3], sd = 0.0f, ci = 0.0f, cj = 0.0f;
+    bool f = false;
+    if (J < nf && J != I) {
+      pair_cos(ri, rec + (size_t)J * kVR, sep, sd, ci, cj);
+      f = facing_cos(ci) && facing_cos(cj);
+    }
+    const unsigned long long b = __ballot(f);
+    if (lane == 0) wcount[wave] = __popcll(b);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wave; ++w) off += wcount[w];
+    const int chunk = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
+    if (rowptr && f) {
+      const int e = off + __popcll(b & lt);
+      if (e >= rowptr[I + 1]) continue;  // unreachable: the count pass decided the same pairs
+      const float* rj = rec + (size_t)J * kVR;
+      float v = (cosf(acosf(ci)) * cosf(acosf(cj)) * rj[kArea]) / (PI * sd * sd);
+      if (v < 0.0f) v = 0.0f;
+      col[e] = J;
+      vf[e] = v;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) vec[3 * (size_t)e + k] = ri[kArea] * v * (sep[k] / sd);
+    }
+    base += chunk;
+    total += chunk;
+  }
+  if (!rowptr && threadIdx.x == 0) count[I] = total;
+}
+
+// SH/main.cpp:208-360: work item (G, z0) per workgroup, partner z0 + lane of
+// G's facing list per lane; every other partner S of G is staged through LDS
+// and tested against the line from Z's centroid towards G's.  keep = 0 when
+// some S hides Z from G.
+extern "C" __global__ __launch_bounds__(256) void btpm_vf_hidden(const float* __restrict__ rec,
+                                                                 const int* __restrict__ rowptr1,
+                                                                 const int* __restrict__ col1,
+                                                                 const int2* __restrict__ work,
+                                                                 unsigned char* __restrict__ keep) {
+  __shared__ float4 srec[kHidChunk * (kVR / 4)];
+  __shared__ int sidx[kHidChunk];
+  const int2 wk = work[blockIdx.x];
+  const int G = wk.x;
+  const int beg = rowptr1[G], nd = rowptr1[G + 1] - beg;
+  const int zi = wk.y + (int)threadIdx.x;
+  const bool act = zi < nd;
+  const int Z = act ? col1[beg + zi] : G;
+  const float* gr = rec + (size_t)G * kVR;
+  const float* zr = rec + (size_t)Z * kVR;
+  const float zm0 = zr[kMid], zm1 = zr[kMid + 1], zm2 = zr[kMid + 2];
+  const float gv0 = gr[kMid] - zm0, gv1 = gr[kMid + 1] - zm1, gv2 = gr[kMid + 2] - zm2;
+  const float g = sqrtf((gv0 * gv0) + (gv1 * gv1) + (gv2 * gv2));
+  bool vis = act;
+  for (int s0 = 0; s0 < nd; s0 += kHidChunk) {
+    const int cnt = min(kHidChunk, nd - s0);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+      const int S = col1[beg + s0 + threadIdx.x];
+      sidx[threadIdx.x] = S;
+      const float4* src = reinterpret_cast<const float4*>(rec + (size_t)S * kVR);
+#pragma unroll
+      for (int c = 0; c < kVR / 4; ++c) srec[threadIdx.x * (kVR / 4) + c] = src[c];
+    }
+    __syncthreads();
+    for (int r = 0; r < cnt && vis; ++r) {
+      const int Sx = sidx[r];
+      if (Sx == G || Sx == Z) continue;
+      const float* s = reinterpret_cast<const float*>(&srec[r * (kVR / 4)]);
+      const float sv0 = s[kMid] - zm0, sv1 = s[kMid + 1] - zm1, sv2 = s[kMid + 2] - zm2;
+      const float sl = sqrtf((sv0 * sv0) + (sv1 * sv1) + (sv2 * sv2));
+      if (!(g > sl)) continue;
+      const float n0 = s[kNrm], n1 = s[kNrm + 1], n2 = s[kNrm + 2];
+      const float tc1 = (sv0 * n0) + (sv1 * n1) + (sv2 * n2);
+      const float tc2 = (gv0 * n0) + (gv1 * n1) + (gv2 * n2);
+      const float t = tc1 / tc2;
+      if (!(t > 0.0f)) continue;
+      const float x0 = (t * gv0) + zm0, x1 = (t * gv1) + zm1, x2 = (t * gv2) + zm2;
+      // edges from v0 (e1), v2 (e2), v1 (e3), each crossed with x - its origin
+      const float a0 = x0 - s[kV0], a1 = x1 - s[kV0 + 1], a2 = x2 - s[kV0 + 2];
+      const float b0 = x0 - s[kV2], b1 = x1 - s[kV2 + 1], b2 = x2 - s[kV2 + 2];
+      const float c0 = x0 - s[kV1], c1 = x1 - s[kV1 + 1], c2 = x2 - s[kV1 + 2];
+      const float* e1 = s + kE1;
+      const float* e2 = s + kE2;
+      const float* e3 = s + kE3;
+      const float k1 = (((e1[1] * a2) - (e1[2] * a1)) * n0) + (((e1[2] * a0) - (e1[0] * a2)) * n1) +
+                       (((e1[0] * a1) - (e1[1] * a0)) * n2);
+      const float k2 = (((e2[1] * b2) - (e2[2] * b1)) * n0) + (((e2[2] * b0) - (e2[0] * b2)) * n1) +
+                       (((e2[0] * b1) - (e2[1] * b0)) * n2);
+      const float k3 = (((e3[1] * c2) - (e3[2] * c1)) * n0) + (((e3[2] * c0) - (e3[0] * c2)) * n1) +
+                       (((e3[0] * c1) - (e3[1] * c0)) * n2);
+      if (k1 >= 0.0f && k2 >= 0.0f && k3 >= 0.0f) vis = false;
+    }
+  }
+  if (act) keep[beg + zi] = vis ? 1 : 0;
+}
+
+// kept partners per row (one wave per row)
+extern "C" __global__ __launch_bounds__(64) void btpm_vf_kept(const int* __restrict__ rowptr1,
+                                                              const unsigned char* __restrict__ keep,
+                                                              int* __restrict__ count) {
+  const int G = blockIdx.x;
+  const int end = rowptr1[G + 1];
+  int c = 0;
+  for (int e0 = rowptr1[G]; e0 < end; e0 += 64) {
+    const int e = e0 + (int)threadIdx.x;
+    c += __popcll(__ballot(e < end && keep[e] != 0));
+  }
+  if (threadIdx.x == 0) count[G] = c;
+}
+
+// SH/main.cpp:337-347: the kept partners of each row in order (one wave per
+// row); pairs to refine (VF >= refine_min, :443) are appended to reflist as
+// (entry, row), in no particular order.
+extern "C" __global__ __launch_bounds__(64) void btpm_vf_compact(
+    const int* __restrict__ rowptr1, const int* __restrict__ col1, const float* __restrict__ vf1,
+    const float* __restrict__ vec1, const unsigned char* __restrict__ keep,
+    const int* __restrict__ rowptr, int* __restrict__ col, float* __restrict__ vf,
+    float* __restrict__ vec, int refine, float refine_min, int* __restrict__ nref,
+    int2* __restrict__ reflist) {
+  const int G = blockIdx.x;
+  const int lane = threadIdx.x;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int end = rowptr1[G + 1];
+  int o = rowptr[G];
+  for (int e0 = rowptr1[G]; e0 < end; e0 += 64) {
+    const int e = e0 + lane;
+    const bool k = e < end && (keep == nullptr || keep[e] != 0);
+    const unsigned long long b = __ballot(k);
+    const int dst = o + __popcll(b & lt);
+    const float v = k ? vf1[e] : 0.0f;
+    if (k) {
+      col[dst] = col1[e];
+      vf[dst] = v;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) vec[3 * (size_t)dst + q] = vec1[3 * (size_t)e + q];
+    }
+    const bool rf = refine && k && v >= refine_min;
+    const unsigned long long rb = __ballot(rf);
+    if (rb) {
+      int rbase = 0;
+      if (lane == 0) rbase = atomicAdd(nref, __popcll(rb));
+      rbase = __shfl(rbase, 0);
+      if (rf) reflist[rbase + __popcll(rb & lt)] = make_int2(dst, G);
+    }
+    o += __popcll(b);
+  }
+}
+
+// SH/main.cpp:415-615: one refined pair per workgroup.  Lane j owns
+// sub-facet j of facet I and sums over the sub-facets k of J (centroids in
+// LDS) in order; the row sums are then combined in order j = 0.. by lane 0.
+extern "C" __global__ __launch_bounds__(128) void btpm_vf_refine(const float* __restrict__ rec,
+                                                                 const int* __restrict__ col,
+                                                                 const int2* __restrict__ reflist,
+                                                                 int nsub, float* __restrict__ vf,
+                                                                 float* __restrict__ vec) {
+  extern __shared__ float sm[];
+  const int nsf = nsub * nsub;
+  float* m2 = sm;                  // [nsf][3]
+  float* tot = sm + 3 * nsf;       // [nsf]
+  float* totv = sm + 4 * nsf;      // [nsf][3]
+  const int2 pr = reflist[blockIdx.x];
+  const int e = pr.x;
+  const float* ri = rec + (size_t)pr.y * kVR;
+  const float* rj = rec + (size_t)col[e] * kVR;
+  const float PI = 4.0f * atanf(1.0f);
+  for (int l = threadIdx.x; l < nsf; l += blockDim.x) sub_centroid(rj, nsub, l, m2 + 3 * l);
+  __syncthreads();
+  const float a1 = ri[kArea] / (float)nsf, a2 = rj[kArea] / (float)nsf;
+  const float ni0 = ri[kNrm], ni1 = ri[kNrm + 1], ni2 = ri[kNrm + 2], nil = ri[kNlen];
+  const float nj0 = rj[kNrm], nj1 = rj[kNrm + 1], nj2 = rj[kNrm + 2], njl = rj[kNlen];
+  for (int j = threadIdx.x; j < nsf; j += blockDim.x) {
+    float m1[3];
+    sub_centroid(ri, nsub, j, m1);
+    float t = 0.0f, tv0 = 0.0f, tv1 = 0.0f, tv2 = 0.0f;
+    for (int k = 0; k < nsf; ++k) {
+      const float s0 = m2[3 * k] - m1[0], s1 = m2[3 * k + 1] - m1[1], s2 = m2[3 * k + 2] - m1[2];
+      const float sd = sqrtf((s0 * s0) + (s1 * s1) + (s2 * s2));
+      const float ci = ((ni0 * s0) + (ni1 * s1) + (ni2 * s2)) / (nil * sd);
+      const float cj = (((-1.0f * nj0) * s0) + ((-1.0f * nj1) * s1) + ((-1.0f * nj2) * s2)) /
+                       (njl * sd);
+      float v = (ci * cj * a2) / (PI * sd * sd);
+      if (v < 0.0f) v = 0.0f;
+      t = t + v;
+      tv0 = tv0 + (v * (s0 / sd));
+      tv1 = tv1 + (v * (s1 / sd));
+      tv2 = tv2 + (v * (s2 / sd));
+    }
+    tot[j] = t;
+    totv[3 * j] = tv0;
+    totv[3 * j + 1] = tv1;
+    totv[3 * j + 2] = tv2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float eff = 0.0f, v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+    for (int j = 0; j < nsf; ++j) {
+      eff = eff + ((a1 / ri[kArea]) * tot[j]);
+      v0 = v0 + (a1 * totv[3 * j]);
+      v1 = v1 + (a1 * totv[3 * j + 1]);
+      v2 = v2 + (a1 * totv[3 * j + 2]);
+    }
+    vf[e] = eff;
+    vec[3 * (size_t)e] = v0;
+    vec[3 * (size_t)e + 1] = v1;
+    vec[3 * (size_t)e + 2] = v2;
+  }
+}
+
+// SH/main.cpp:822-835 and :874-878: per row (one wave) the total view
+// factor and photon momentum vector, summed in list order, and each entry's
+// vector-list value vec / (VF A_I)
+extern "C" __global__ __launch_bounds__(64) void btpm_vf_finish(
+    const int* __restrict__ rowptr, const float* __restrict__ vf, const float* __restrict__ vec,
+    const float* __restrict__ rec, float* __restrict__ dir, float* __restrict__ total,
+    float* __restrict__ photon) {
+  const int G = blockIdx.x;
+  const int lane = threadIdx.x;
+  const float area = rec[(size_t)G * kVR + kArea];
+  const int beg = rowptr[G], end = rowptr[G + 1];
+  float t = 0.0f, p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+  for (int e0 = beg; e0 < end; e0 += 64) {
+    const int e = e0 + lane;
+    float v = 0.0f, q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
+    if (e < end) {
+      v = vf[e];
+      const float x0 = vec[3 * (size_t)e], x1 = vec[3 * (size_t)e + 1], x2 = vec[3 * (size_t)e + 2];
+      q0 = x0 / area, q1 = x1 / area, q2 = x2 / area;
+      dir[3 * (size_t)e] = x0 / (v * area);
+      dir[3 * (size_t)e + 1] = x1 / (v * area);
+      dir[3 * (size_t)e + 2] = x2 / (v * area);
+    }
+    const int cnt = min(64, end - e0);
+    for (int i = 0; i < cnt; ++i) {
+      t = t + __shfl(v, i);
+      p0 = p0 + __shfl(q0, i);
+      p1 = p1 + __shfl(q1, i);
+      p2 = p2 + __shfl(q2, i);
+    }
+  }
+  if (lane == 0) {
+    total[G] = end > beg ? t : 0.0f;
+    photon[3 * (size_t)G] = p0;
+    photon[3 * (size_t)G + 1] = p1;
+    photon[3 * (size_t)G + 2] = p2;
+  }
+}
+
+// T3905/main.cpp:1059-1097: the facet frame (rows: unit first edge, normal x
+// edge, unit normal) in double, one facet per lane
+extern "C" __global__ __launch_bounds__(256) void btpm_drf_frames(const double* __restrict__ vtx,
+                                                                  const int* __restrict__ fv, int nf,
+                                                                  double* __restrict__ frame) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const double* p0 = vtx + 3 * (size_t)fv[3 * f];
+  const double* p1 = vtx + 3 * (size_t)fv[3 * f + 1];
+  const double* p2 = vtx + 3 * (size_t)fv[3 * f + 2];
+  double e[3], b[3], n[3], y[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    e[k] = p1[k] - p0[k];
+    b[k] = p2[k] - p0[k];
+  }
+  n[0] = (e[1] * b[2]) - (e[2] * b[1]);
+  n[1] = (e[2] * b[0]) - (e[0] * b[2]);
+  n[2] = (e[0] * b[1]) - (e[1] * b[0]);
+  const double nl = sqrt((n[0] * n[0]) + (n[1] * n[1]) + (n[2] * n[2]));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) n[k] = n[k] / nl;
+  const double x = sqrt((e[0] * e[0]) + (e[1] * e[1]) + (e[2] * e[2]));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) e[k] = e[k] / x;
+  y[0] = (n[1] * e[2]) - (n[2] * e[1]);
+  y[1] = (n[2] * e[0]) - (n[0] * e[2]);
+  y[2] = (n[0] * e[1]) - (n[1] * e[0]);
+  double* m = frame + 9 * (size_t)f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    m[k] = e[k];
+    m[3 + k] = y[k];
+    m[6 + k] = n[k];
+  }
+}
+
+// row index of every CSR entry (one wave per row)
+extern "C" __global__ __launch_bounds__(64) void btpm_drf_rows(const int* __restrict__ rowptr,
+                                                               int* __restrict__ row) {
+  const int f = blockIdx.x;
+  for (int e = rowptr[f] + (int)threadIdx.x; e < rowptr[f + 1]; e += 64) row[e] = f;
+}
+
+// T3905/main.cpp:1127-1176: one (entry q, crater sub-facet j) per lane.
+//   crater[j] = (rim-sample origin = vertex 2 [3], trivectors [2][3], unit
+//   normal [3]); mul[p] = the sample's trivector multipliers (:1101-1114)
+extern "C" __global__ __launch_bounds__(256) void btpm_drf(
+    const double* __restrict__ frame, const int* __restrict__ row, const float* __restrict__ vfv,
+    const float* __restrict__ dir, const float* __restrict__ crater, const float* __restrict__ mul,
+    int ncf, int nsamp, float radius, float step, long long n, float* __restrict__ drf) {
+  extern __shared__ float sm[];
+  float* sc = sm;                 // [ncf][12]
+  float* smul = sm + 12 * ncf;    // [nsamp][2]
+  for (int i = threadIdx.x; i < 12 * ncf; i += blockDim.x) sc[i] = crater[i];
+  for (int i = threadIdx.x; i < 2 * nsamp; i += blockDim.x) smul[i] = mul[i];
+  __syncthreads();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long q = t / ncf;
+  const int j = (int)(t - q * ncf);
+  const double* M = frame + 9 * (size_t)row[q];
+  const double d0 = dir[3 * q], d1 = dir[3 * q + 1], d2 = dir[3 * q + 2];
+  const float w0 = (float)((M[0] * d0) + (M[1] * d1) + (M[2] * d2));
+  const float w1 = (float)((M[3] * d0) + (M[4] * d1) + (M[5] * d2));
+  const float w2 = (float)((M[6] * d0) + (M[7] * d1) + (M[8] * d2));
+  const float a = fabsf(w2) / sqrtf((w0 * w0) + (w1 * w1));
+  const float* c = sc + 12 * j;
+  const float o0 = c[0], o1 = c[1], o2 = c[2];
+  const float u0 = c[3], u1 = c[4], u2 = c[5], v0 = c[6], v1 = c[7], v2 = c[8];
+  float vis = 1.0f;
+  const float ang = (w0 * c[9]) + (w1 * c[10]) + (w2 * c[11]);
+  if (ang < 0.0f) vis = 0.0f;
+  const float rr = radius * radius;
+  for (int p = 0; p < nsamp; ++p) {
+    const float ma = smul[2 * p], mb = smul[2 * p + 1];
+    const float mp0 = o0 + (ma * u0) + (mb * v0);
+    const float mp1 = o1 + (ma * u1) + (mb * v1);
+    const float mp2 = o2 + (ma * u2) + (mb * v2);
+    const float tc1 = ((-1.0f * w1) * mp0) + (w0 * mp1);
+    const float sr = sqrtf(rr - (tc1 * tc1));
+    const float tc0 = (w0 * mp0) + (w1 * mp1) - sr;
+    if (mp2 <= (a * tc0) && vis != 0.0f) vis = vis - step;
+    if (vis < 0.0f) vis = 0.0f;
+  }
+  float v = (vfv[q] * vis * ang) / w2;
+  if (v > 1.0f) v = 1.0f;
+  drf[t] = v;
+}
+
+// ===========================================================================
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "runtime.h"
+
+using namespace btpm;
+
+// facet view factors of one mesh, resident on its device until freed
+struct btpm_vf {
+  int device = 0;
+  int nf = 0;
+  long long n_facing = 0, nnz = 0, n_refined = 0;
+  std::vector<int> rowptr;
+  DBuf col, vf, vec, dir, total, photon;
+};
+
+namespace {
+// exclusive scan of per-row counts into a CSR row pointer; false on overflow
+bool scan_rows(const std::vector<int>& cnt, std::vector<int>& rp) {
+  rp.assign(cnt.size() + 1, 0);
+  long long s = 0;
+  for (size_t i = 0; i < cnt.size(); ++i) {
+    s += cnt[i];
+    if (s > INT_MAX) return false;
+    rp[i + 1] = (int)s;
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" int btpm_vf_build(btpm_ctx* ctx, const btpm_vf_mesh* m, btpm_vf** out) {
+  if (!ctx || !m || !out || !m->vtx || !m->facets) return fail(BTPM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (m->nv < 3 || m->nf < 1) return fail(BTPM_ERR_ARG, "bad sizes: nv %d nf %d", m->nv, m->nf);
+  if (m->nsub < 0 || m->nsub > BTPM_VF_NSUB_MAX)
+    return fail(BTPM_ERR_ARG, "nsub %d outside 0..%d", m->nsub, BTPM_VF_NSUB_MAX);
+  if (std::isnan(m->refine_min)) return fail(BTPM_ERR_ARG, "refine_min is NaN");
+  for (int i = 0; i < 3 * m->nf; ++i)
+    if (m->facets[i] < 0 || m->facets[i] >= m->nv)
+      return fail(BTPM_ERR_ARG, "facet %d: vertex %d out of range", i / 3, m->facets[i]);
+  CtxLock lk(ctx);
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int nf = m->nf;
+  btpm_vf* r = new btpm_vf;
+  std::unique_ptr<btpm_vf> guard(r);
+  r->device = ctx->device;
+  r->nf = nf;
+  DBuf d_vtx, d_fv, d_rec;
+  upload(d_vtx, m->vtx, 3 * (size_t)m->nv, st);
+  upload(d_fv, m->facets, 3 * (size_t)nf, st);
+  d_rec.alloc((size_t)nf * BTPM_VF_REC * sizeof(float));
+  hipLaunchKernelGGL(btpm_vf_prep, dim3(grid1(nf, 256)), dim3(256), 0, st, d_vtx.as<float>(),
+                     d_fv.as<int>(), nf, d_rec.as<float>());
+  // facing pairs: counts, row pointers, then the ordered lists
+  DBuf d_cnt;
+  d_cnt.alloc((size_t)nf * sizeof(int));
+  hipLaunchKernelGGL(btpm_vf_facing, dim3(nf), dim3(256), 0, st, d_rec.as<float>(), nf,
+                     (const int*)nullptr, d_cnt.as<int>(), (int*)nullptr, (float*)nullptr,
+                     (float*)nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<int> cnt(nf), rp1;
+  HIPCHK(hipMemcpyAsync(cnt.data(), d_cnt.p, nf * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (!scan_rows(cnt, rp1)) return fail(BTPM_ERR_ARG, "more than 2^31 facing pairs");
+  const long long n1 = rp1[nf];
+  r->n_facing = n1;
+  DBuf d_rp1, d_col1, d_vf1, d_vec1;
+  upload(d_rp1, rp1.data(), rp1.size(), st);
+  d_col1.alloc(std::max<size_t>(1, n1) * sizeof(int));
+  d_vf1.alloc(std::max<size_t>(1, n1) * sizeof(float));
+  d_vec1.alloc(std::max<size_t>(1, n1) * 3 * sizeof(float));
+  hipLaunchKernelGGL(btpm_vf_facing, dim3(nf), dim3(256), 0, st, d_rec.as<float>(), nf,
+                     d_rp1.as<int>(), (int*)nullptr, d_col1.as<int>(), d_vf1.as<float>(),
+                     d_vec1.as<float>());
+  HIPCHK(hipGetLastError());
+  // occlusion cull: (row, first partner) work items, longest rows first
+  DBuf d_keep;
+  std::vector<int> rp;
+  if (m->cull && n1 > 0) {
+    std::vector<int> rows;
+    for (int G = 0; G < nf; ++G)
+      if (cnt[G] > 0) rows.push_back(G);
+    std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+    std::vector<int2> work;
+    for (int G : rows)
+      for (int z0 = 0; z0 < cnt[G]; z0 += 256) work.push_back(make_int2(G, z0));
+    DBuf d_work;
+    upload(d_work, work.data(), work.size(), st);
+    d_keep.alloc((size_t)n1);
+    for (size_t w0 = 0; w0 < work.size(); w0 += 1u << 30) {
+      const size_t nw = std::min<size_t>(work.size() - w0, 1u << 30);
+      hipLaunchKernelGGL(btpm_vf_hidden, dim3((unsigned)nw), dim3(256), 0, st, d_rec.as<float>(),
+                         d_rp1.as<int>(), d_col1.as<int>(), d_work.as<int2>() + w0,
+                         d_keep.as<unsigned char>());
+    }
+    hipLaunchKernelGGL(btpm_vf_kept, dim3(nf), dim3(64), 0, st, d_rp1.as<int>(),
+                       d_keep.as<unsigned char>(), d_cnt.as<int>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cnt.data(), d_cnt.p, nf * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    scan_rows(cnt, rp);
+  } else {
+    rp = rp1;
+  }
+  const long long nnz = rp[nf];
+  r->nnz = nnz;
+  r->rowptr = rp;
+  DBuf d_rp, d_ref, d_nref;
+  upload(d_rp, rp.data(), rp.size(), st);
+  r->col.alloc(std::max<size_t>(1, nnz) * sizeof(int));
+  r->vf.alloc(std::max<size_t>(1, nnz) * sizeof(float));
+  r->vec.alloc(std::max<size_t>(1, nnz) * 3 * sizeof(float));
+  r->dir.alloc(std::max<size_t>(1, nnz) * 3 * sizeof(float));
+  r->total.alloc((size_t)nf * sizeof(float));
+  r->photon.alloc((size_t)nf * 3 * sizeof(float));
+  d_ref.alloc(std::max<size_t>(1, nnz) * sizeof(int2));
+  const int zero = 0;
+  upload(d_nref, &zero, 1, st);
+  hipLaunchKernelGGL(btpm_vf_compact, dim3(nf), dim3(64), 0, st, d_rp1.as<int>(), d_col1.as<int>(),
+                     d_vf1.as<float>(), d_vec1.as<float>(),
+                     d_keep.p ? d_keep.as<unsigned char>() : (const unsigned char*)nullptr,
+                     d_rp.as<int>(), r->col.as<int>(), r->vf.as<float>(), r->vec.as<float>(),
+                     m->nsub > 0 ? 1 : 0, m->refine_min, d_nref.as<int>(), d_ref.as<int2>());
+  HIPCHK(hipGetLastError());
+  int nref = 0;
+  HIPCHK(hipMemcpyAsync(&nref, d_nref.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  r->n_refined = nref;
+  if (nref > 0) {
+    const int nsf = m->nsub * m->nsub;
+    hipLaunchKernelGGL(btpm_vf_refine, dim3(nref), dim3(128), 7 * nsf * sizeof(float), st,
+                       d_rec.as<float>(), r->col.as<int>(), d_ref.as<int2>(), m->nsub,
+                       r->vf.as<float>(), r->vec.as<float>());
+  }
+  hipLaunchKernelGGL(btpm_vf_finish, dim3(nf), dim3(64), 0, st, d_rp.as<int>(), r->vf.as<float>(),
+                     r->vec.as<float>(), d_rec.as<float>(), r->dir.as<float>(),
+                     r->total.as<float>(), r->photon.as<float>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  *out = guard.release();
+  API_END
+}
+
+extern "C" int btpm_vf_counts(const btpm_vf* v, long long* n_facing, long long* nnz,
+                              long long* n_refined) {
+  if (!v) return fail(BTPM_ERR_ARG, "null view-factor handle");
+  if (n_facing) *n_facing = v->n_facing;
+  if (nnz) *nnz = v->nnz;
+  if (n_refined) *n_refined = v->n_refined;
+  return BTPM_OK;
+}
+
+extern "C" int btpm_vf_fetch(const btpm_vf* v, int* rowptr, int* col, float* vf, float* dir,
+                             float* vec, float* total, float* photon) {
+  if (!v) return fail(BTPM_ERR_ARG, "null view-factor handle");
+  API_BEGIN
+  HIPCHK(hipSetDevice(v->device));
+  const size_t n = (size_t)v->nnz, nf = (size_t)v->nf;
+  if (rowptr) std::copy(v->rowptr.begin(), v->rowptr.end(), rowptr);
+  if (col && n) HIPCHK(hipMemcpy(col, v->col.p, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (vf && n) HIPCHK(hipMemcpy(vf, v->vf.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (dir && n) HIPCHK(hipMemcpy(dir, v->dir.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+  if (vec && n) HIPCHK(hipMemcpy(vec, v->vec.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+  if (total) HIPCHK(hipMemcpy(total, v->total.p, nf * sizeof(float), hipMemcpyDeviceToHost));
+  if (photon) HIPCHK(hipMemcpy(photon, v->photon.p, 3 * nf * sizeof(float), hipMemcpyDeviceToHost));
+  API_END
+}
+
+extern "C" void btpm_vf_free(btpm_vf* v) {
+  if (!v) return;
+  hipSetDevice(v->device);
+  delete v;
+}
+
+extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, float* drf) {
+  if (!ctx || !in || !drf || !in->vtx || !in->facets || !in->rowptr || !in->crater_vtx ||
+      !in->crater_facets)
+    return fail(BTPM_ERR_ARG, "null argument");
+  if (in->nv < 3 || in->nf < 1 || in->ncv < 3 || in->ncf < 1 || in->m < 1 || in->m > 64)
+    return fail(BTPM_ERR_ARG, "bad sizes: nv %d nf %d ncv %d ncf %d m %d", in->nv, in->nf, in->ncv,
+                in->ncf, in->m);
+  if (in->rowptr[0] != 0) return fail(BTPM_ERR_ARG, "rowptr[0] must be 0");
+  for (int f = 0; f < in->nf; ++f)
+    if (in->rowptr[f + 1] < in->rowptr[f]) return fail(BTPM_ERR_ARG, "rowptr decreases at %d", f);
+  const long long nnz = in->rowptr[in->nf];
+  if (nnz > 0 && (!in->vf || !in->dir)) return fail(BTPM_ERR_ARG, "null view factors");
+  for (int i = 0; i < 3 * in->nf; ++i)
+    if (in->facets[i] < 0 || in->facets[i] >= in->nv)
+      return fail(BTPM_ERR_ARG, "facet %d: vertex %d out of range", i / 3, in->facets[i]);
+  for (int i = 0; i < 3 * in->ncf; ++i)
+    if (in->crater_facets[i] < 0 || in->crater_facets[i] >= in->ncv)
+      return fail(BTPM_ERR_ARG, "crater facet %d: vertex %d out of range", i / 3,
+                  in->crater_facets[i]);
+  if (nnz == 0) return BTPM_OK;
+  CtxLock lk(ctx);
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int ncf = in->ncf, m = in->m, nsamp = m * m;
+  // crater sub-facets (T3905/main.cpp:819-860): vertex 2, trivectors, unit normal
+  std::vector<float> cr(12 * (size_t)ncf), mul(2 * (size_t)nsamp);
+  for (int j = 0; j < ncf; ++j) {
+    const float* p0 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j];
+    const float* p1 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 1];
+    const float* p2 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 2];
+    float a[3], b[3], n[3];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = p1[k] - p0[k];
+      b[k] = p2[k] - p0[k];
+    }
+    n[0] = (a[1] * b[2]) - (a[2] * b[1]);
+    n[1] = (a[2] * b[0]) - (a[0] * b[2]);
+    n[2] = (a[0] * b[1]) - (a[1] * b[0]);
+    const float l = std::sqrt((n[0] * n[0]) + (n[1] * n[1]) + (n[2] * n[2]));
+    float* c = cr.data() + 12 * (size_t)j;
+    for (int k = 0; k < 3; ++k) {
+      c[k] = p2[k];
+      c[3 + k] = -1.0f * b[k] / (float)m;
+      c[6 + k] = a[k] / (float)m;
+      c[9 + k] = n[k] / l;
+    }
+  }
+  // sample multipliers (T3905/main.cpp:1101-1114)
+  for (int k = 1; k != m + 1; ++k)
+    for (int l = 1; l != k + 1; ++l) {
+      const int p = ((k - 1) * (k - 1)) + (2 * (l - 1));
+      mul[2 * p] = (float)k - (1.0f / 3.0f);
+      mul[2 * p + 1] = (float)l - (2.0f / 3.0f);
+    }
+  for (int k = 2; k != m + 1; ++k)
+    for (int l = 1; l != k; ++l) {
+      const int p = ((k - 1) * (k - 1)) + (2 * (l - 1)) + 1;
+      mul[2 * p] = (float)k - (2.0f / 3.0f);
+      mul[2 * p + 1] = (float)l - (1.0f / 3.0f);
+    }
+  DBuf d_vtx, d_fv, d_rp, d_vf, d_dir, d_cr, d_mul, d_frame, d_row, d_out;
+  upload(d_vtx, in->vtx, 3 * (size_t)in->nv, st);
+  upload(d_fv, in->facets, 3 * (size_t)in->nf, st);
+  upload(d_rp, in->rowptr, (size_t)in->nf + 1, st);
+  upload(d_vf, in->vf, (size_t)nnz, st);
+  upload(d_dir, in->dir, 3 * (size_t)nnz, st);
+  upload(d_cr, cr.data(), cr.size(), st);
+  upload(d_mul, mul.data(), mul.size(), st);
+  d_frame.alloc(9 * (size_t)in->nf * sizeof(double));
+  d_row.alloc((size_t)nnz * sizeof(int));
+  const long long n = nnz * ncf;
+  d_out.alloc((size_t)n * sizeof(float));
+  hipLaunchKernelGGL(btpm_drf_frames, dim3(grid1(in->nf, 256)), dim3(256), 0, st,
+                     d_vtx.as<double>(), d_fv.as<int>(), in->nf, d_frame.as<double>());
+  hipLaunchKernelGGL(btpm_drf_rows, dim3(in->nf), dim3(64), 0, st, d_rp.as<int>(), d_row.as<int>());
+  const size_t shm = (12 * (size_t)ncf + 2 * (size_t)nsamp) * sizeof(float);
+  if (shm > 60 * 1024) return fail(BTPM_ERR_ARG, "crater model too large (%d facets)", ncf);
+  hipLaunchKernelGGL(btpm_drf, dim3(grid1((size_t)n, 256)), dim3(256), shm, st,
+                     d_frame.as<double>(), d_row.as<int>(), d_vf.as<float>(), d_dir.as<float>(),
+                     d_cr.as<float>(), d_mul.as<float>(), ncf, nsamp, in->crater_radius,
+                     1.0f / (float)(m * m), n, d_out.as<float>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(drf, d_out.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  API_END
+}

@@ -187,6 +187,46 @@ def binary_shape(subdiv_p: int = 2, subdiv_s: int = 1, axes_p=(0.6, 0.5, 0.45),
     return equ, fv, len(Vp)
 
 
+def occlusion_scene(nx: int = 6, nblock: int = 16, seed: int = 0, gap: float = 1.0):
+    """A floor (z = 0, facing up) and a ceiling (z = gap, facing down), each
+    an nx x nx grid of triangle pairs over the unit square with vertex
+    heights jittered by 1e-3, and nblock small one-sided triangles in
+    between (half facing up, half down, tilted up to 20 degrees) that hide
+    parts of the floor from the ceiling and back: a scene whose facet view
+    factors need the occlusion cull.  Returns (vertices [nv][3], facets
+    [nf][3])."""
+    rng = np.random.default_rng(seed)
+    V, F = [], []
+
+    def grid(z, up, off):
+        base = len(V)
+        g = (np.arange(nx + 1) / nx)
+        for y in g:
+            for x in g:
+                V.append((x + off[0], y + off[1], z + 1e-3 * rng.standard_normal()))
+        for j in range(nx):
+            for i in range(nx):
+                a = base + j * (nx + 1) + i
+                b, c, d = a + 1, a + nx + 1, a + nx + 2
+                for t in ((a, b, d), (a, d, c)):
+                    F.append(t if up else (t[0], t[2], t[1]))
+
+    grid(0.0, True, (0.0, 0.0))
+    grid(gap, False, tuple(0.1 * rng.random(2)))
+    for k in range(nblock):
+        c = np.array([0.2 + 0.6 * rng.random(), 0.2 + 0.6 * rng.random(),
+                      gap * (0.35 + 0.3 * rng.random())])
+        phi = 2.0 * math.pi * rng.random()
+        tilt = math.radians(20.0) * rng.random()
+        ang = phi + np.array([0.0, 2.0 * math.pi / 3.0, 4.0 * math.pi / 3.0])
+        pts = np.stack([np.cos(ang), np.sin(ang), np.zeros(3)], axis=1) * 0.12
+        pts[:, 2] = np.tan(tilt) * pts[:, 0]
+        base = len(V)
+        V.extend(map(tuple, pts + c))
+        F.append((base, base + 1, base + 2) if k % 2 == 0 else (base, base + 2, base + 1))
+    return np.asarray(V, np.float64), np.asarray(F, np.int32)
+
+
 # --------------------------------------------------------------------------
 # sparse facet view factors
 # --------------------------------------------------------------------------

@@ -15,8 +15,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, Geometry, Params, SolveInfo, check,
-                   lib)
+from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, DrfIn, Geometry, Params, SolveInfo,
+                   VfMesh, check, lib)
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -340,6 +340,81 @@ def geometry_illumination(ctx: Context, equpos, facets, ntau: int, pol, rot, nv_
                                            _ptr(illum), _ptr(sun) if sun is not None else None,
                                            C.byref(solar)))
     return Illumination(shadow, illum, sun, float(solar.value))
+
+
+@dataclass
+class ViewFactors:
+    """Facet view-factor lists of a shape (the Shape_Selfheating outputs)."""
+    rowptr: np.ndarray    # [nf+1] int32
+    col: np.ndarray       # [nnz] int32: facet seen by the row's facet, increasing
+    vf: np.ndarray        # [nnz] float32
+    dir: np.ndarray       # [nnz][3] float32: _selfheating_vector_list.txt value
+    vec: np.ndarray       # [nnz][3] float32: area-weighted view-factor vector
+    total: np.ndarray     # [nf] float32: total view factor
+    photon: np.ndarray    # [nf][3] float32: photon momentum vector (_photovectors.txt)
+    n_facing: int         # facing pairs before the occlusion cull
+    n_refined: int        # entries refined by sub-facet quadrature
+
+    @property
+    def csr(self):
+        return self.rowptr, self.col, self.vf
+
+
+def viewfactors(ctx: Context, vtx, facets, nsub: int = 10, cull: bool = True,
+                refine_min: float = 0.05) -> ViewFactors:
+    """Facet view factors of a mesh on the GPU (Shape_Selfheating): vertices
+    [nv][3] (rounded to float, as the producer reads them), facets [nf][3].
+    nsub = sub-facet split per edge of pairs with VF >= refine_min (0: none);
+    cull = drop pairs hidden by a third facet."""
+    v = np.ascontiguousarray(vtx, dtype=np.float32)
+    fv = np.ascontiguousarray(facets, dtype=np.int32)
+    nf = fv.shape[0]
+    m = VfMesh()
+    m.nv, m.nf = v.shape[0], nf
+    m.vtx, m.facets = _ptr(v), _ptr(fv, _ip)
+    m.nsub, m.cull, m.refine_min = int(nsub), int(bool(cull)), float(refine_min)
+    h = C.c_void_p()
+    check(lib().btpm_vf_build(ctx.handle, C.byref(m), C.byref(h)))
+    try:
+        n1, n, nr = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        check(lib().btpm_vf_counts(h, C.byref(n1), C.byref(n), C.byref(nr)))
+        nnz = int(n.value)
+        rowptr = np.empty(nf + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        vf = np.empty(nnz, np.float32)
+        dirs = np.empty((nnz, 3), np.float32)
+        vec = np.empty((nnz, 3), np.float32)
+        total = np.empty(nf, np.float32)
+        photon = np.empty((nf, 3), np.float32)
+        check(lib().btpm_vf_fetch(h, _ptr(rowptr, _ip), _ptr(col, _ip), _ptr(vf), _ptr(dirs),
+                                  _ptr(vec), _ptr(total), _ptr(photon)))
+    finally:
+        lib().btpm_vf_free(h)
+    return ViewFactors(rowptr, col, vf, dirs, vec, total, photon, int(n1.value), int(nr.value))
+
+
+def crater_viewfactors(ctx: Context, vtx, facets, rowptr, vf, dirs, crater_vtx, crater_facets,
+                       m: int = 10, crater_radius: float = 10.0) -> np.ndarray:
+    """Shape->crater view factors drf [nnz][ncf] on the GPU
+    (calculate_surface_global_viewfactors): the shape in double, its facet
+    view-factor rows with their vector-list directions, the crater model."""
+    v = np.ascontiguousarray(vtx, dtype=np.float64)
+    fv = np.ascontiguousarray(facets, dtype=np.int32)
+    rp = np.ascontiguousarray(rowptr, dtype=np.int32)
+    vfs = np.ascontiguousarray(vf, dtype=np.float32)
+    d = np.ascontiguousarray(dirs, dtype=np.float32)
+    cv = np.ascontiguousarray(crater_vtx, dtype=np.float32)
+    cf = np.ascontiguousarray(crater_facets, dtype=np.int32)
+    q = DrfIn()
+    q.nv, q.nf = v.shape[0], fv.shape[0]
+    q.vtx, q.facets = v.ctypes.data_as(C.POINTER(C.c_double)), _ptr(fv, _ip)
+    q.rowptr, q.vf, q.dir = _ptr(rp, _ip), _ptr(vfs), _ptr(d)
+    q.ncv, q.ncf = cv.shape[0], cf.shape[0]
+    q.crater_vtx, q.crater_facets = _ptr(cv), _ptr(cf, _ip)
+    q.m, q.crater_radius = int(m), float(crater_radius)
+    out = np.empty((int(rp[-1]), q.ncf), np.float32)
+    check(lib().btpm_crater_viewfactors(ctx.handle, C.byref(q), _ptr(out)))
+    return out
 
 
 def crater_contract(ctx: Context, vf, e) -> np.ndarray:

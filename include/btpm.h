@@ -301,6 +301,59 @@ int btpm_crater_contract(btpm_ctx *ctx, const float *vf, const float *e, int nco
 int btpm_geometry_illumination(btpm_ctx *ctx, const btpm_geometry *g, int *shadow, float *illum,
                                float *sun_facet, float *solar);
 
+/* ---- view factors ------------------------------------------------------------
+ * Facet view factors of a shape: replaces the Shape_Selfheating executable
+ * (3905/Shape_Selfheating/main.cpp: initialise :750, selfheating :628, hidden
+ * :187, betterviewfactors :374, the totals of main :817-835), whose outputs
+ * (_selfheating_list.txt, _selfheating_vector_list.txt, _photovectors.txt,
+ * :837-888) the Thermal stage reads.  Float arithmetic as the reference.    */
+typedef struct {
+    int nv, nf;
+    const float *vtx;      /* [nv][3] vertex positions (the producer reads floats) */
+    const int *facets;     /* [nf][3] zero-based vertex indices                     */
+    int nsub;              /* sub-facet split per edge of refined pairs (reference
+                              n = 10, main.cpp:15); 0 = no refinement, at most 32 */
+    int cull;              /* 1: drop pairs hidden by a third facet (hidden())
+                              as the reference; 0: keep every facing pair         */
+    float refine_min;      /* refine pairs whose centroid view factor is at least
+                              this (reference 0.05f, main.cpp:443)                */
+} btpm_vf_mesh;
+typedef struct btpm_vf btpm_vf;   /* the lists, resident on the device */
+/* Builds the lists: row f holds, in increasing facet order, the facets that
+ * f sees (list entry: facet, view factor).                                  */
+int btpm_vf_build(btpm_ctx *ctx, const btpm_vf_mesh *mesh, btpm_vf **out);
+/* facing pairs before the cull, list entries, refined entries              */
+int btpm_vf_counts(const btpm_vf *vf, long long *n_facing, long long *nnz, long long *n_refined);
+/* Copies the lists to host memory (any pointer may be NULL): rowptr [nf+1],
+ * col [nnz], vf [nnz], dir [nnz][3] = the _selfheating_vector_list.txt value
+ * vec / (VF A_f), vec [nnz][3] = the area-weighted VF vector, total [nf] =
+ * the total view factor, photon [nf][3] = _photovectors.txt.               */
+int btpm_vf_fetch(const btpm_vf *vf, int *rowptr, int *col, float *viewfactor, float *dir,
+                  float *vec, float *total, float *photon);
+void btpm_vf_free(btpm_vf *vf);
+
+/* Shape->crater view factors: replaces calculate_surface_global_viewfactors
+ * (3905/Thermal/NominalPole/main.cpp:1054-1181, 175706/Run-04/thermal.cpp:
+ * 1477-1606) that the Thermal stage runs once per geometry before
+ * load_GPU_roughness_globalillumination / _globalheating.                  */
+typedef struct {
+    int nv, nf;
+    const double *vtx;     /* [nv][3] shape vertices (the stage reads doubles)    */
+    const int *facets;     /* [nf][3]                                             */
+    const int *rowptr;     /* [nf+1] facet view-factor rows                        */
+    const float *vf;       /* [nnz] view factors                                   */
+    const float *dir;      /* [nnz][3] vector-list directions                      */
+    int ncv, ncf;
+    const float *crater_vtx;     /* [ncv][3] crater model vertices                */
+    const int *crater_facets;    /* [ncf][3]                                      */
+    int m;                 /* rim-visibility samples per crater facet edge (m*m
+                              samples; reference m = 10), at most 64             */
+    float crater_radius;   /* crater rim radius in model units (reference 10)     */
+} btpm_drf_in;
+/* drf [nnz][ncf] (host memory): per list entry and crater sub-facet j,
+ * VF vis_j cos_j / dir_z capped at 1.                                      */
+int btpm_crater_viewfactors(btpm_ctx *ctx, const btpm_drf_in *in, float *drf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -199,6 +199,54 @@ typedef struct {
 void orc_geometry(const orc_geometry_in *in, int *shadow, double *cosang, float *illum,
                   float *sun_facet, float *solar_out);
 
+/* Facet view-factor producer (viewfactor.c, float): mesh vtx [nv][3] (float,
+ * as the producer reads it), fv [nf][3]; facing pairs, occlusion cull when
+ * cull != 0, n x n sub-facet refinement (n = nsub, reference 10) of pairs
+ * with VF >= refine_min (reference 0.05).  Outputs are malloc'ed lists
+ * released by orc_vf_free: CSR rowptr[nf+1] / col / vf, the area-weighted VF
+ * vector vec[nnz][3], the vector-list value dir = vec / (vf A_I), refined
+ * flags, total VF and photon momentum vector per facet.                    */
+typedef struct {
+    int nv, nf;
+    int nsub;
+    int cull;
+    float refine_min;
+    const float *vtx;
+    const int *fv;
+} orc_vf_in;
+typedef struct {
+    long long n_facing, nnz, n_refined;
+    int *rowptr, *col;
+    float *vf, *vec, *dir;
+    unsigned char *refined;
+    float *total, *photon;
+} orc_vf_out;
+int orc_viewfactors(const orc_vf_in *in, orc_vf_out *out);
+void orc_vf_free(orc_vf_out *out);
+/* the vertex table vpos [(n+1)(n+2)/2][3], facet table tri [n*n][3] and
+ * sub-facet centroids mid [n*n][3] of a triangle v9 = [3][3], split n x n */
+void orc_vf_subdivide(const float *v9, int n, float *vpos, int *tri, float *mid);
+
+/* Shape->crater view factors (viewfactor.c): mesh vtx [nv][3] in double,
+ * fv [nf][3], the facet view-factor CSR (rowptr, vf) and its vector-list
+ * directions dir [nnz][3]; crater mesh cvtx [ncv][3] (float), cfv [ncf][3];
+ * m x m rim-visibility samples per crater facet, crater rim radius.
+ * Output drf [nnz][ncf].                                                   */
+typedef struct {
+    int nv, nf;
+    const double *vtx;
+    const int *fv;
+    const int *rowptr;
+    const float *vf;
+    const float *dir;
+    int ncv, ncf;
+    const float *cvtx;
+    const int *cfv;
+    int m;
+    float crater_radius;
+} orc_drf_in;
+void orc_crater_drf(const orc_drf_in *in, float *drf);
+
 #ifdef __cplusplus
 }
 #endif

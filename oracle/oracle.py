@@ -362,3 +362,108 @@ def geometry(equpos, fv, ntau, pol, rot, nv_primary=None, rev_primary=1.0, rev_s
     lib().orc_geometry(C.byref(g), _ptr(shadow, _ip), _ptr(cosang, _dp), _ptr(illum, _fp),
                        _ptr(sun, _fp), C.byref(solar))
     return Geometry(shadow, cosang, illum, sun, float(solar.value))
+
+
+class OrcVfIn(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("nsub", C.c_int), ("cull", C.c_int),
+        ("refine_min", C.c_float), ("vtx", _fp), ("fv", _ip),
+    ]
+
+
+class OrcVfOut(C.Structure):
+    _fields_ = [
+        ("n_facing", C.c_longlong), ("nnz", C.c_longlong), ("n_refined", C.c_longlong),
+        ("rowptr", _ip), ("col", _ip), ("vf", _fp), ("vec", _fp), ("dir", _fp),
+        ("refined", _up), ("total", _fp), ("photon", _fp),
+    ]
+
+
+class OrcDrfIn(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("vtx", _dp), ("fv", _ip), ("rowptr", _ip),
+        ("vf", _fp), ("dir", _fp), ("ncv", C.c_int), ("ncf", C.c_int), ("cvtx", _fp),
+        ("cfv", _ip), ("m", C.c_int), ("crater_radius", C.c_float),
+    ]
+
+
+@dataclass
+class ViewFactors:
+    rowptr: np.ndarray    # [nf+1] int32
+    col: np.ndarray       # [nnz] int32: facet seen by the row's facet
+    vf: np.ndarray        # [nnz] float32
+    vec: np.ndarray       # [nnz][3] float32: A_I VF d/|d| (refined: sub-facet sum)
+    dir: np.ndarray       # [nnz][3] float32: vec / (VF A_I), the vector-list value
+    refined: np.ndarray   # [nnz] bool
+    total: np.ndarray     # [nf] float32: total view factor
+    photon: np.ndarray    # [nf][3] float32: photon momentum vector
+    n_facing: int         # facing pairs before the occlusion cull
+
+    @property
+    def csr(self):
+        return self.rowptr, self.col, self.vf
+
+
+def viewfactors(vtx, fv, nsub=10, cull=True, refine_min=0.05):
+    """Facet view factors of a mesh (viewfactor.c): vertices [nv][3] rounded
+    to float as the producer reads them, facets [nf][3]."""
+    vtx = np.ascontiguousarray(vtx, dtype=np.float32)
+    fv = np.ascontiguousarray(fv, dtype=np.int32)
+    nf = fv.shape[0]
+    q = OrcVfIn()
+    q.nv, q.nf, q.nsub, q.cull = vtx.shape[0], nf, int(nsub), int(bool(cull))
+    q.refine_min = float(refine_min)
+    q.vtx, q.fv = _ptr(vtx, _fp), _ptr(fv, _ip)
+    o = OrcVfOut()
+    if lib().orc_viewfactors(C.byref(q), C.byref(o)) != 0:
+        raise RuntimeError("orc_viewfactors failed")
+    try:
+        n = int(o.nnz)
+
+        def arr(p, count, dt):
+            return np.ctypeslib.as_array(p, shape=(count,)).astype(dt, copy=True) if count else \
+                np.zeros(0, dt)
+        res = ViewFactors(arr(o.rowptr, nf + 1, np.int32), arr(o.col, n, np.int32),
+                          arr(o.vf, n, np.float32), arr(o.vec, 3 * n, np.float32).reshape(n, 3),
+                          arr(o.dir, 3 * n, np.float32).reshape(n, 3),
+                          arr(o.refined, n, np.uint8).astype(bool), arr(o.total, nf, np.float32),
+                          arr(o.photon, 3 * nf, np.float32).reshape(nf, 3), int(o.n_facing))
+    finally:
+        lib().orc_vf_free(C.byref(o))
+    return res
+
+
+def vf_subdivide(v9, n):
+    """(vertex table, facet table, sub-facet centroids) of a triangle [3][3]
+    split n x n, in the producer's order (viewfactor.c)."""
+    v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(9)
+    nv = (n + 1) * (n + 2) // 2
+    vpos = np.zeros((nv, 3), np.float32)
+    tri = np.zeros((n * n, 3), np.int32)
+    mid = np.zeros((n * n, 3), np.float32)
+    lib().orc_vf_subdivide(_ptr(v9, _fp), C.c_int(n), _ptr(vpos, _fp), _ptr(tri, _ip),
+                           _ptr(mid, _fp))
+    return vpos, tri, mid
+
+
+def crater_drf(vtx, fv, rowptr, vf, dirs, cvtx, cfv, m=10, crater_radius=10.0):
+    """Shape->crater view factors drf [nnz][ncf] (viewfactor.c): the mesh in
+    double, the facet view-factor CSR rows and values with their vector-list
+    directions, the crater mesh in float."""
+    vtx = np.ascontiguousarray(vtx, dtype=np.float64)
+    fv = np.ascontiguousarray(fv, dtype=np.int32)
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    vf = np.ascontiguousarray(vf, dtype=np.float32)
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32)
+    cvtx = np.ascontiguousarray(cvtx, dtype=np.float32)
+    cfv = np.ascontiguousarray(cfv, dtype=np.int32)
+    q = OrcDrfIn()
+    q.nv, q.nf = vtx.shape[0], fv.shape[0]
+    q.vtx, q.fv, q.rowptr = _ptr(vtx, _dp), _ptr(fv, _ip), _ptr(rowptr, _ip)
+    q.vf, q.dir = _ptr(vf, _fp), _ptr(dirs, _fp)
+    q.ncv, q.ncf = cvtx.shape[0], cfv.shape[0]
+    q.cvtx, q.cfv = _ptr(cvtx, _fp), _ptr(cfv, _ip)
+    q.m, q.crater_radius = int(m), float(crater_radius)
+    out = np.zeros((int(rowptr[-1]), q.ncf), np.float32)
+    lib().orc_crater_drf(C.byref(q), _ptr(out, _fp))
+    return out

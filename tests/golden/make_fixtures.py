@@ -1,6 +1,7 @@
 """Writes the small data fixtures the CPU tests use, from data files that ship
-in the reference tree (data only: meshes, view-factor lists, the mesh
-statistics file).  Run in a checkout that has the reference at REF:
+in the reference tree (data only: meshes, view-factor lists, photon momentum
+vectors, the mesh statistics file).  Run in a checkout that has the reference
+at REF:
 
     python tests/golden/make_fixtures.py [REF]
 
@@ -24,8 +25,9 @@ D = os.path.join(REF, "3905", "Thermal", "NominalPole")
 def main():
     V, F = formats.read_shape(os.path.join(D, "new_crater2.txt"), 61, 100)
     csr = formats.read_selfheating_list(os.path.join(D, "new_crater2_selfheating_list.txt"), 100)
+    photon = np.loadtxt(os.path.join(D, "new_crater2_photovectors.txt"), dtype=np.float64)
     np.savez_compressed(os.path.join(HERE, "new_crater2.npz"), V=V, F=F, rowptr=csr[0],
-                        colidx=csr[1], vf=csr[2])
+                        colidx=csr[1], vf=csr[2], photon=photon)
     V, F = formats.read_shape(os.path.join(D, "doppler_binary.txt"), 3204, 6400)
     np.savez_compressed(os.path.join(HERE, "doppler_binary.npz"), V=V, F=F.astype(np.int32))
     props = {}

@@ -1,0 +1,350 @@
+"""Pins of the view-factor restatement (oracle/viewfactor.c) on the CPU.
+
+The facet view-factor producer (3905/Shape_Selfheating/main.cpp) is checked
+against the one output of it the reference ships (the crater model's
+view-factor list and photon momentum vectors), against its facing and
+occlusion predicates recomputed independently in FP64 (a different
+ray-triangle algorithm), and against what the mathematics fixes for the
+sub-facet quadrature (tiling of the split, reciprocity, convergence to the
+integral, the n = 1 case, the closed-cavity sum rule).  The shape->crater
+view factors (3905/Thermal/NominalPole/main.cpp:1054-1181) are checked at
+normal incidence (closed form) and against an independent FP64 rim test.
+"""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from binarytpm_amd import inputs
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "binarytpm_amd",
+                    "csrc")
+
+
+def _cos_min():
+    """BTPM_VF_COS_MIN as the kernels compile it (hex float literal)."""
+    txt = open(os.path.join(CSRC, "btpm_internal.h")).read()
+    lit = re.search(r"#define BTPM_VF_COS_MIN \((-0x[0-9a-fp.+-]+)f\)", txt).group(1)
+    return np.float32(float.fromhex(lit))
+
+
+def _fp64_geometry(V, F):
+    V = np.asarray(V, np.float32).astype(np.float64)
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    mid = (p0 + p1 + p2) / 3.0
+    n = np.cross(p1 - p0, p2 - p0)
+    nl = np.linalg.norm(n, axis=1)
+    return mid, n / nl[:, None], nl / 2.0, (p0, p1, p2)
+
+
+# ---------------------------------------------------------------------------
+# the shipped crater list
+# ---------------------------------------------------------------------------
+
+def test_shipped_crater_list_reproduced():
+    """new_crater2_selfheating_list.txt / _photovectors.txt (shipped with
+    every Thermal run) are the producer's output for new_crater2.txt with
+    every pair refined at n = 10: the restatement reproduces the membership
+    exactly and the view factors to the precision the files and the mesh
+    are printed with (6 significant digits)."""
+    d = np.load(os.path.join(GOLD, "new_crater2.npz"))
+    r = O.viewfactors(d["V"], d["F"], nsub=10, refine_min=0.0)
+    assert np.array_equal(r.rowptr, d["rowptr"]) and np.array_equal(r.col, d["colidx"])
+    assert r.refined.all() and r.n_facing == len(r.col)   # bowl: nothing hidden
+    ref = d["vf"].astype(np.float64)
+    rel = np.abs(r.vf - ref) / ref
+    assert np.median(rel) < 2e-6 and rel.max() < 1e-4
+    assert np.abs(r.photon - d["photon"]).max() < 2e-6
+    # the centroid formula alone (no refinement) does not reproduce the file
+    r0 = O.viewfactors(d["V"], d["F"], nsub=0)
+    assert np.abs(r0.vf - ref).max() / ref.max() > 0.01
+    assert np.abs(r0.photon - d["photon"]).max() > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# facing test
+# ---------------------------------------------------------------------------
+
+def test_facing_threshold_matches_libm_acosf():
+    """acosf(c) <= float(pi)/2 (the reference's test) holds exactly for
+    BTPM_VF_COS_MIN <= c <= 1, the interval the kernels use: checked on
+    every float within 4096 ulps of both ends and on random floats (an
+    exhaustive scan of all 2^32 floats against this libm agreed)."""
+    m = ctypes.CDLL("libm.so.6")
+    m.acosf.restype = ctypes.c_float
+    m.acosf.argtypes = [ctypes.c_float]
+    m.atanf.restype = ctypes.c_float
+    m.atanf.argtypes = [ctypes.c_float]
+    P = np.float32(np.float32(4.0) * np.float32(m.atanf(1.0)) / np.float32(2.0))
+    c0 = _cos_min()
+    cand = []
+    for edge in (c0, np.float32(1.0)):
+        bits = int(edge.view(np.int32))
+        cand.extend(np.arange(bits - 4096, bits + 4097, dtype=np.int32).view(np.float32))
+    rng = np.random.default_rng(0)
+    cand.extend(rng.uniform(-1.5, 1.5, 4000).astype(np.float32))
+    cand.extend((rng.standard_normal(2000) * 1e-6).astype(np.float32))
+    for x in cand:
+        want = m.acosf(float(x)) <= P
+        got = bool(c0 <= x <= np.float32(1.0))
+        assert want == got, (float(x), want, got)
+
+
+def test_facing_pairs_match_fp64():
+    """The facing lists (no cull) equal the FP64 sign test on every pair
+    whose two cosines are clear of zero, and their centroid view factors are
+    cos_i cos_j A_j / (pi d^2) to float accuracy."""
+    V, F = inputs.occlusion_scene(seed=3)
+    r = O.viewfactors(V, F, nsub=0, cull=False)
+    mid, nrm, area, _ = _fp64_geometry(V, F)
+    nf = len(F)
+    sep = mid[None, :, :] - mid[:, None, :]
+    d = np.linalg.norm(sep, axis=-1)
+    np.fill_diagonal(d, 1.0)
+    ci = np.einsum("ik,ijk->ij", nrm, sep) / d
+    cj = -np.einsum("jk,ijk->ij", nrm, sep) / d
+    listed = np.zeros((nf, nf), bool)
+    for i in range(nf):
+        listed[i, r.col[r.rowptr[i]:r.rowptr[i + 1]]] = True
+    clear = (np.abs(ci) > 1e-5) & (np.abs(cj) > 1e-5)
+    np.fill_diagonal(clear, False)
+    want = (ci > 0) & (cj > 0)
+    assert np.array_equal(listed[clear], want[clear])
+    assert want[clear].sum() > 5000 and (~want[clear]).sum() > 5000
+    rows = np.repeat(np.arange(nf), np.diff(r.rowptr))
+    vf64 = np.clip(ci * cj, 0.0, None)[rows, r.col] * area[r.col] / (math.pi * d[rows, r.col] ** 2)
+    big = vf64 > 1e-6
+    assert np.allclose(r.vf[big], vf64[big], rtol=2e-4)
+
+
+# ---------------------------------------------------------------------------
+# occlusion cull
+# ---------------------------------------------------------------------------
+
+def _moller_trumbore(orig, dirn, p0, p1, p2):
+    """FP64 ray-triangle: ray parameter u (hit point orig + u dirn), the
+    barycentric coordinates (b1, b2) and the determinant, per triangle."""
+    e1, e2 = p1 - p0, p2 - p0
+    pv = np.cross(dirn, e2)
+    det = np.einsum("ik,ik->i", e1, pv)
+    inv = 1.0 / np.where(det == 0.0, 1e-300, det)
+    tv = orig - p0
+    b1 = np.einsum("ik,ik->i", tv, pv) * inv
+    qv = np.cross(tv, e1)
+    b2 = np.einsum("ik,ik->i", dirn, qv) * inv
+    u = np.einsum("ik,ik->i", e2, qv) * inv
+    return u, b1, b2, det
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_hidden_pairs_match_ray_triangle(seed):
+    """A pair (G, Z) is dropped exactly when the ray from Z's centroid
+    towards G's meets (at any positive distance) the triangle of some other
+    facet S that G faces and whose centroid is nearer Z than G's: checked
+    with Moller-Trumbore in FP64 on every pair no S decides within a margin
+    (grazing rays, hits on an edge, equal distances)."""
+    V, F = inputs.occlusion_scene(seed=seed)
+    r0 = O.viewfactors(V, F, nsub=0, cull=False)
+    r1 = O.viewfactors(V, F, nsub=0, cull=True)
+    mid, nrm, area, (p0, p1, p2) = _fp64_geometry(V, F)
+    culled = checked = 0
+    for G in range(len(F)):
+        part = r0.col[r0.rowptr[G]:r0.rowptr[G + 1]]
+        kept = set(r1.col[r1.rowptr[G]:r1.rowptr[G + 1]].tolist())
+        for Z in part:
+            S = part[part != Z]
+            if len(S) == 0:
+                continue
+            dirn = np.broadcast_to(mid[G] - mid[Z], (len(S), 3))
+            orig = np.broadcast_to(mid[Z], (len(S), 3))
+            u, b1, b2, det = _moller_trumbore(orig, dirn, p0[S], p1[S], p2[S])
+            g = np.linalg.norm(mid[G] - mid[Z])
+            s = np.linalg.norm(mid[S] - mid[Z], axis=1)
+            edge = np.minimum(np.minimum(b1, b2), 1.0 - b1 - b2)
+            scale = np.linalg.norm(p1[S] - p0[S], axis=1) * np.linalg.norm(dirn, axis=1)
+            ambiguous = ((np.abs(det) < 1e-6 * scale) | (np.abs(edge) < 1e-5) |
+                         (np.abs(u) < 1e-6) | (np.abs(g - s) < 1e-6 * g))
+            hit = (s < g) & (u > 0) & (edge > 0)
+            if hit.any() and not (ambiguous & ~hit).any():
+                want = False
+            elif not hit.any() and not ambiguous.any():
+                want = True
+            else:
+                continue
+            checked += 1
+            culled += not want
+            assert (Z in kept) == want, (G, Z)
+    assert checked > 10000 and culled > 2000
+
+
+# ---------------------------------------------------------------------------
+# sub-facet quadrature
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 3, 10])
+def test_subdivision_tiles_the_triangle(n):
+    """The n x n split: (n+1)(n+2)/2 vertices on the triangle's lattice,
+    n^2 distinct sub-triangles of equal area summing to the triangle's, and
+    sub-facet centroids inside the triangle averaging to its centroid."""
+    rng = np.random.default_rng(n)
+    tri = rng.standard_normal((3, 3)).astype(np.float32)
+    vpos, ftab, mid = O.vf_subdivide(tri, n)
+    t = tri.astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(t[1] - t[0], t[2] - t[0]))
+    vp = vpos.astype(np.float64)
+    a = 0.5 * np.linalg.norm(np.cross(vp[ftab[:, 1]] - vp[ftab[:, 0]],
+                                      vp[ftab[:, 2]] - vp[ftab[:, 0]]), axis=1)
+    assert len({tuple(sorted(f)) for f in ftab.tolist()}) == n * n
+    assert np.allclose(a, area / (n * n), rtol=1e-4)
+    assert abs(a.sum() - area) < 1e-4 * area
+    # barycentric coordinates of the centroids
+    M = np.stack([t[1] - t[0], t[2] - t[0]], axis=1)
+    bc, *_ = np.linalg.lstsq(M, (mid.astype(np.float64) - t[0]).T, rcond=None)
+    assert (bc > -1e-5).all() and (bc.sum(axis=0) < 1 + 1e-5).all()
+    assert np.allclose(mid.astype(np.float64).mean(axis=0), t.mean(axis=0), atol=1e-5)
+    # vertices lie on the lattice (i/n, j/n)
+    bv, *_ = np.linalg.lstsq(M, (vp - t[0]).T, rcond=None)
+    assert np.allclose(bv * n, np.round(bv * n), atol=1e-3)
+
+
+def test_refined_pairs_are_reciprocal_and_converge():
+    """Refined view factors obey reciprocity A_I F_IJ = A_J F_JI (the
+    quadrature is symmetric in the two facets); against the integral (FP64
+    quadrature at n = 64) the n = 10 values are within 1.5% and closer than
+    the centroid formula; n = 1 is the centroid formula."""
+    V, F = inputs.occlusion_scene(seed=2)
+    r = O.viewfactors(V, F, nsub=10, refine_min=0.0)
+    mid, nrm, area, (p0, p1, p2) = _fp64_geometry(V, F)
+    rows = np.repeat(np.arange(len(F)), np.diff(r.rowptr))
+    key = {(int(i), int(j)): k for k, (i, j) in enumerate(zip(rows, r.col))}
+    pairs = [(i, j) for (i, j) in key if i < j and (j, i) in key and r.vf[key[(i, j)]] > 1e-3]
+    assert len(pairs) > 200
+    for i, j in pairs:
+        a, b = area[i] * r.vf[key[(i, j)]], area[j] * r.vf[key[(j, i)]]
+        assert abs(a - b) <= 2e-5 * max(a, b)
+    r1 = O.viewfactors(V, F, nsub=1, refine_min=0.0)
+    r0 = O.viewfactors(V, F, nsub=0)
+    # (the centroid formula goes through cos(acos(c)), which loses relative
+    # accuracy for c near 0: an absolute floor covers those pairs)
+    assert np.allclose(r1.vf, r0.vf, rtol=1e-5, atol=1e-6 * r0.vf.max())
+
+    def integral(i, j, n=64):
+        ci = []
+        for f in (i, j):
+            _, _, m = O.vf_subdivide(np.stack([p0[f], p1[f], p2[f]]), n)
+            ci.append(m.astype(np.float64))
+        sep = ci[1][None, :, :] - ci[0][:, None, :]
+        d2 = (sep ** 2).sum(-1)
+        c1 = sep @ nrm[i] / np.sqrt(d2)
+        c2 = -(sep @ nrm[j]) / np.sqrt(d2)
+        return (np.clip(c1 * c2, 0, None) / (math.pi * d2)).mean() * area[j]
+
+    near = sorted(pairs, key=lambda p: -r0.vf[key[p]])[:12]
+    for i, j in near:
+        ex = integral(i, j)
+        e10 = abs(r.vf[key[(i, j)]] - ex) / ex
+        e1 = abs(r0.vf[key[(i, j)]] - ex) / ex
+        assert e10 < 0.015 and e10 <= e1 + 1e-6, (i, j, e10, e1)
+
+
+def test_closed_cavity_total_view_factor_is_one():
+    """Inside a closed sphere (icosphere, normals inward) every facet sees
+    all the others and nothing hides: the total view factor of each facet
+    is 1 up to the quadrature error."""
+    Vs, Fs = inputs.icosphere(2)
+    Fin = Fs[:, [0, 2, 1]]
+    r = O.viewfactors(Vs, Fin, nsub=10, refine_min=0.0)
+    assert r.n_facing == len(r.col) == len(Fs) * (len(Fs) - 1)
+    assert np.allclose(r.total, 1.0, atol=0.01)
+    # the centroid formula alone is visibly worse for the nearest neighbours
+    r0 = O.viewfactors(Vs, Fin, nsub=0)
+    assert np.abs(r0.total - 1.0).max() > np.abs(r.total - 1.0).max()
+
+
+# ---------------------------------------------------------------------------
+# shape -> crater view factors
+# ---------------------------------------------------------------------------
+
+def _crater():
+    d = np.load(os.path.join(GOLD, "new_crater2.npz"))
+    return d["V"].astype(np.float32), d["F"].astype(np.int32)
+
+
+def _similar_split_centroids(tri, m):
+    """Centroids of the m^2 triangles similar to tri that tile it: weights
+    (i + 1/3)/m over i0+i1+i2 = m-1 (upright) and (i + 2/3)/m over
+    i0+i1+i2 = m-2 (inverted) on the three vertices."""
+    w = []
+    for s, off in ((m - 1, 1.0 / 3.0), (m - 2, 2.0 / 3.0)):
+        for i0 in range(s + 1):
+            for i1 in range(s + 1 - i0):
+                w.append((i0 + off, i1 + off, s - i0 - i1 + off))
+    return np.asarray(w, np.float64) / m @ np.asarray(tri, np.float64)
+
+
+def _single_facet_case(w):
+    """One shape facet in the xy plane (frame = identity) with one list
+    entry of VF 0.5 whose direction, in the facet frame, is w."""
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [5, 5, 5]], np.float64)
+    F = np.array([[0, 1, 2]], np.int32)
+    return V, F, np.array([0, 1], np.int32), np.array([0.5], np.float32), \
+        np.asarray(w, np.float32).reshape(1, 3)
+
+
+def test_drf_at_normal_incidence_is_cosine():
+    """Direction along the facet normal: no rim shadow, so each crater
+    sub-facet gets VF cos(n_j, z) (0 for sub-facets facing away)."""
+    cv, cf = _crater()
+    V, F, rp, vf, w = _single_facet_case([0.0, 0.0, 1.0])
+    drf = O.crater_drf(V, F, rp, vf, w, cv, cf)[0]
+    c = cv.astype(np.float64)
+    n = np.cross(c[cf[:, 1]] - c[cf[:, 0]], c[cf[:, 2]] - c[cf[:, 0]])
+    nz = n[:, 2] / np.linalg.norm(n, axis=1)
+    assert np.allclose(drf, np.where(nz >= 0, 0.5 * nz, 0.0), atol=1e-6)
+    assert (nz > 0).sum() > 50
+
+
+@pytest.mark.parametrize("theta_deg", [60.0, 80.0])
+def test_drf_rim_visibility_matches_fp64(theta_deg):
+    """The m x m rim-visibility count per crater sub-facet, recomputed in
+    FP64 from sample points generated independently (the centroids of the
+    m^2 triangles similar to the sub-facet that tile it, from barycentric
+    lattice coordinates) with the rim test of the reference
+    (T3905/main.cpp:1162-1166, point shadowed when its height is below
+    |w_z|/|w_xy| times its distance to the rim along w_xy); samples within
+    1e-4 of the decision are excluded."""
+    cv, cf = _crater()
+    th = math.radians(theta_deg)
+    for phi in (0.3, 1.9, 4.0):
+        w64 = np.array([math.sin(th) * math.cos(phi), math.sin(th) * math.sin(phi), math.cos(th)])
+        V, F, rp, vf, w = _single_facet_case(w64)
+        drf = O.crater_drf(V, F, rp, vf, w, cv, cf)[0].astype(np.float64)
+        wf = w[0].astype(np.float64)
+        c = cv.astype(np.float64)
+        n = np.cross(c[cf[:, 1]] - c[cf[:, 0]], c[cf[:, 2]] - c[cf[:, 0]])
+        n /= np.linalg.norm(n, axis=1)[:, None]
+        ang = n @ wf
+        a = abs(wf[2]) / math.hypot(wf[0], wf[1])
+        tested = 0
+        for j in range(len(cf)):
+            if ang[j] <= 1e-4:
+                assert drf[j] == 0.0
+                continue
+            p = _similar_split_centroids(c[cf[j]], 10)
+            tc1 = -wf[1] * p[:, 0] + wf[0] * p[:, 1]
+            with np.errstate(invalid="ignore"):
+                tc0 = wf[0] * p[:, 0] + wf[1] * p[:, 1] - np.sqrt(100.0 - tc1 * tc1)
+            marg = p[:, 2] - a * tc0
+            if np.any(np.abs(marg[np.isfinite(marg)]) < 1e-4):
+                continue
+            shadowed = np.isfinite(marg) & (marg <= 0)
+            vis = 1.0 - shadowed.sum() / 100.0
+            want = min(1.0, 0.5 * vis * ang[j] / wf[2])
+            assert abs(drf[j] - want) < 1e-5 + 1e-4 * want, (phi, j, drf[j], want)
+            tested += 1
+        assert tested > 40
