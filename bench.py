@@ -391,6 +391,94 @@ def geometry_phase(ctx, cpu=True):
     return out
 
 
+def viewfactor_phase(ctx, cpu=True):
+    """Facet view-factor producer (Shape_Selfheating) on its own input, the
+    6400-facet Doppler binary mesh of the reference (a committed fixture)
+    with n = 10, then the shape->crater view factors of the resulting lists
+    against the shipped 100-facet crater model.  GPU wall times of the whole
+    calls (inputs uploaded, lists / [nnz][100] factors copied back), best of
+    3; the float restatement (OpenMP over rows) timed on the whole shape for
+    the lists and on sampled rows for the crater factors, and checked against
+    the GPU."""
+    from binarytpm_amd import thermal as th
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    d = np.load(os.path.join(gold, "doppler_binary.npz"))
+    V, F = d["V"], d["F"]
+    c = np.load(os.path.join(gold, "new_crater2.npz"))
+    cv, cf = c["V"].astype(np.float32), c["F"].astype(np.int32)
+    th.viewfactors(ctx, V, F[:256])   # warm-up
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        g = th.viewfactors(ctx, V, F)
+        walls.append(time.perf_counter() - t0)
+    gpu_s = min(walls)
+    nf = len(F)
+    # the cull tests every (G, Z, S) triple of a row's facing partners; counted
+    # here from the kept lists (the cull removes 4 of 4.1 M pairs on this mesh)
+    triples = float((np.diff(g.rowptr).astype(np.float64) ** 2).sum())
+    out = {
+        "workload": (f"Doppler binary mesh, {nf} facets (two 3200-facet bodies), n = 10 "
+                     "sub-facet split, occlusion cull"),
+        "gpu_s": gpu_s,
+        "facing_pairs": int(g.n_facing),
+        "entries": int(len(g.col)),
+        "refined": int(g.n_refined),
+        "pair_tests_per_s": nf * (nf - 1) / gpu_s,
+        "cull_triples": triples,
+        "cull_triples_per_s": triples / gpu_s,
+    }
+    log(f"viewfactors: {nf} facets, {g.n_facing} facing, {len(g.col)} kept in {gpu_s:.3f} s")
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        drf = th.crater_viewfactors(ctx, V, F, g.rowptr, g.vf, g.dir, cv, cf)
+        walls.append(time.perf_counter() - t0)
+    drf_s = min(walls)
+    out["crater_factors"] = {
+        "workload": f"{len(g.col)} list entries x {len(cf)} crater sub-facets x 100 rim samples",
+        "gpu_s": drf_s,
+        "factors_per_s": drf.size / drf_s,
+        "output_gb": drf.nbytes / 1e9,
+    }
+    log(f"crater factors: {drf.shape[0]} x {drf.shape[1]} in {drf_s:.3f} s")
+    if cpu:
+        from oracle import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = int(os.environ["OMP_NUM_THREADS"])
+        t0 = time.perf_counter()
+        o = O.viewfactors(V, F)
+        cpu_s = time.perf_counter() - t0
+        same = bool(np.array_equal(o.rowptr, g.rowptr) and np.array_equal(o.col, g.col))
+        out["cpu_baseline"] = {
+            "value": cpu_s, "unit": "s per shape", "cores": threads, "kind": "port",
+            "sample": "the whole shape: float restatement (oracle/viewfactor.c), OpenMP over rows",
+        }
+        out["vs_cpu"] = cpu_s / gpu_s
+        out["lists_identical"] = same
+        rows = np.flatnonzero(np.diff(g.rowptr) > 0)[::25]
+        cnt = np.diff(g.rowptr)[rows]
+        rp = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+        sel = np.concatenate([np.arange(g.rowptr[f], g.rowptr[f + 1]) for f in rows])
+        t0 = time.perf_counter()
+        od = O.crater_drf(V, F[rows], rp, g.vf[sel], g.dir[sel], cv, cf)
+        per_entry = (time.perf_counter() - t0) / len(sel)
+        out["crater_factors"]["cpu_baseline"] = {
+            "value": per_entry * len(g.col), "unit": "s per shape", "cores": threads,
+            "kind": "port",
+            "sample": f"float restatement on {len(rows)} rows ({len(sel)} entries), "
+                      "extrapolated to every entry",
+        }
+        out["crater_factors"]["vs_cpu"] = per_entry * len(g.col) / drf_s
+        out["crater_factors"]["mismatches_on_sample"] = int((od != drf[sel]).sum())
+        log(f"viewfactors: restatement {cpu_s:.2f} s on {threads} threads ({cpu_s / gpu_s:.0f}x), "
+            f"lists identical {same}; crater factors {per_entry * len(g.col):.1f} s "
+            f"extrapolated ({per_entry * len(g.col) / drf_s:.0f}x), "
+            f"{out['crater_factors']['mismatches_on_sample']} mismatches on the sample")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,6 +496,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-geometry", action="store_true",
                     help="skip the illumination geometry and shadow producer measurement")
+    ap.add_argument("--no-viewfactors", action="store_true",
+                    help="skip the view-factor producer measurement")
+    ap.add_argument("--producers-only", action="store_true",
+                    help="measure only the illumination and view-factor producers (one JSON "
+                         "line with their results; for profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales HBM traffic counters)")
@@ -430,6 +523,12 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
         del src, dst
+    if args.producers_only:
+        with th.Context(0) as ctx:
+            line = {"illumination_producer": geometry_phase(ctx, cpu=not args.no_cpu),
+                    "viewfactor_producer": viewfactor_phase(ctx, cpu=not args.no_cpu)}
+        print(json.dumps(line), flush=True)
+        return
     case, crater = build_workload(args.subdiv, args.gamma)
     plan = sweep_plan(args.ti)
     with th.Context(0) as ctx:
@@ -547,6 +646,8 @@ def main():
             line["serving"] = serve_phase(ctx, args.serve_n, args.serve_rate, rps=args.serve_rps)
         if not args.no_geometry:
             line["illumination_producer"] = geometry_phase(ctx, cpu=not args.no_cpu)
+        if not args.no_viewfactors:
+            line["viewfactor_producer"] = viewfactor_phase(ctx, cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

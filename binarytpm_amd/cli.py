@@ -12,7 +12,9 @@ Inputs:  name.txt, name_illumination.dat (double[NV][3], int[NF][3],
 float[NF][NTAU], the reference's binary_writeglobalillumination), optional
 name_selfheating_list.txt; with --crater: new_crater2.txt,
 new_crater2_selfheating_list.txt and name_roughness_illumination.dat
-(float[NVc][3], int[NFc][3], float[NF][NFc][NTAU]).
+(float[NVc][3], int[NFc][3], float[NF][NFc][NTAU]); with both, also
+name_selfheating_vector_list.txt, from which the shape->crater view factors
+of the crater self-heating are computed on the GPU.
 Outputs: name_temperature(TI).txt and .dat, name_roughness_temperature(TI).dat.
 """
 from __future__ import annotations
@@ -55,6 +57,12 @@ def run(args) -> int:
     if args.numrevs is not None:
         numrevs = np.loadtxt(args.numrevs, dtype=np.float32).reshape(-1)
     with th.Context(args.device) as ctx:
+        if crater is not None and csr is not None:
+            # shape->crater view factors from the facet lists and their
+            # directions (calculate_surface_global_viewfactors,
+            # T3905/main.cpp:1054-1181), on the GPU
+            dirs = formats.read_selfheating_vectors(base + "_selfheating_vector_list.txt", csr[0])
+            crater.drf = th.crater_viewfactors(ctx, V, F, csr[0], csr[2], dirs, Vc, Fc)
         shape = th.Shape(ctx, p, illum, args.albedo, numrevs=numrevs, csr=csr, scatter=0,
                          crater=crater)
         solves = [th.Solve(ctx, shape, ti, with_craters=crater is not None) for ti in args.ti]

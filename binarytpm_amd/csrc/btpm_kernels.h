@@ -106,9 +106,9 @@ extern "C" __global__ void btpm_vf_finish(const int* rowptr, const float* vf, co
 extern "C" __global__ void btpm_drf_frames(const double* vtx, const int* fv, int nf, double* frame);
 extern "C" __global__ void btpm_drf_rows(const int* rowptr, int* row);
 extern "C" __global__ void btpm_drf(const double* frame, const int* row, const float* vfv,
-                                    const float* dir, const float* crater, const float* mul,
-                                    int ncf, int nsamp, float radius, float step, long long n,
-                                    float* drf);
+                                    const float* dir, const float4* csub, const float4* samp,
+                                    const float* vtab, int ncf, int nsamp, float radius,
+                                    long long nnz, float* drf);
 
 // stepwise.hip
 extern "C" __global__ void btpm_step_profile(float* temp, float* prof, int ncols, int to_records);

@@ -1,0 +1,84 @@
+// probe.hip -- calibration kernel: the VALU issue ceiling of the device for
+// plain (unpacked) FP32 instructions, the unit in which the compute-bound
+// kernels of the library are measured.  Every lane runs 8 independent
+// dependency chains of one instruction kind, so issue, not latency, bounds
+// the loop; the kernel does no memory traffic besides one store per lane.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include "btpm_internal.h"
+
+namespace {
+constexpr int kChains = 8;
+}
+
+// op 0: v_fma_f32, op 1: v_add_f32, op 2: v_mul_f32, op 3: v_sqrt_f32
+// (the hardware square root, the quarter-rate unit); kChains instructions
+// of that kind per iteration and lane
+template <int OP>
+__global__ __launch_bounds__(256) void btpm_probe_valu_kernel(float* __restrict__ out, int iters,
+                                                              float a, float b) {
+  float x[kChains];
+#pragma unroll
+  for (int k = 0; k < kChains; ++k) x[k] = (float)(threadIdx.x + k) * 1e-3f;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < kChains; ++k) {
+      if (OP == 0) x[k] = __builtin_fmaf(x[k], a, b);
+      if (OP == 1) x[k] = x[k] + a;
+      if (OP == 2) x[k] = x[k] * a;
+      if (OP == 3) x[k] = __builtin_amdgcn_sqrtf(x[k]);
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kChains; ++k) s += x[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+#include "runtime.h"
+
+using namespace btpm;
+
+extern "C" int btpm_probe_valu(btpm_ctx* ctx, int op, int iters, double* wave_instr_per_s) {
+  if (!ctx || !wave_instr_per_s) return fail(BTPM_ERR_ARG, "null argument");
+  if (op < 0 || op > 3 || iters < 1 || iters > (1 << 20))
+    return fail(BTPM_ERR_ARG, "bad probe: op %d iters %d", op, iters);
+  CtxLock lk(ctx);
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int blocks = cus * 32;   // 8 waves per SIMD
+  DBuf d_out;
+  d_out.alloc((size_t)blocks * 256 * sizeof(float));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto launch = [&](int n) {
+    switch (op) {
+      case 0: hipLaunchKernelGGL(btpm_probe_valu_kernel<0>, dim3(blocks), dim3(256), 0, st,
+                                 d_out.as<float>(), n, 1.0001f, 1e-7f); break;
+      case 1: hipLaunchKernelGGL(btpm_probe_valu_kernel<1>, dim3(blocks), dim3(256), 0, st,
+                                 d_out.as<float>(), n, 1e-7f, 0.0f); break;
+      case 2: hipLaunchKernelGGL(btpm_probe_valu_kernel<2>, dim3(blocks), dim3(256), 0, st,
+                                 d_out.as<float>(), n, 0.99999f, 0.0f); break;
+      default: hipLaunchKernelGGL(btpm_probe_valu_kernel<3>, dim3(blocks), dim3(256), 0, st,
+                                  d_out.as<float>(), n, 0.0f, 0.0f); break;
+    }
+  };
+  launch(16);  // warm-up
+  HIPCHK(hipEventRecord(e0, st));
+  launch(iters);
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  const double waves = (double)blocks * 4.0;
+  *wave_instr_per_s = waves * (double)iters * kChains / (ms * 1e-3);
+  API_END
+}

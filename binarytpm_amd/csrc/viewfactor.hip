@@ -34,6 +34,7 @@ constexpr int kVR = BTPM_VF_REC;  // floats per facet record
 constexpr int kMid = 0, kNrm = 3, kNlen = 6, kArea = 7, kV0 = 8, kV1 = 11, kV2 = 14, kE1 = 17,
               kE2 = 20, kE3 = 23;
 constexpr int kHidChunk = 256;    // partner records staged per LDS chunk
+constexpr float kGap = 1.0f - 0x1p-20f;
 
 __device__ __forceinline__ bool facing_cos(float c) {
   return c >= BTPM_VF_COS_MIN && c <= 1.0f;
@@ -201,7 +202,8 @@ extern "C" __global__ __launch_bounds__(256) void btpm_vf_hidden(const float* __
   const float* zr = rec + (size_t)Z * kVR;
   const float zm0 = zr[kMid], zm1 = zr[kMid + 1], zm2 = zr[kMid + 2];
   const float gv0 = gr[kMid] - zm0, gv1 = gr[kMid + 1] - zm1, gv2 = gr[kMid + 2] - zm2;
-  const float g = sqrtf((gv0 * gv0) + (gv1 * gv1) + (gv2 * gv2));
+  const float g2 = (gv0 * gv0) + (gv1 * gv1) + (gv2 * gv2);
+  const float g = sqrtf(g2);
   bool vis = act;
   for (int s0 = 0; s0 < nd; s0 += kHidChunk) {
     const int cnt = min(kHidChunk, nd - s0);
@@ -219,28 +221,37 @@ extern "C" __global__ __launch_bounds__(256) void btpm_vf_hidden(const float* __
       if (Sx == G || Sx == Z) continue;
       const float* s = reinterpret_cast<const float*>(&srec[r * (kVR / 4)]);
       const float sv0 = s[kMid] - zm0, sv1 = s[kMid + 1] - zm1, sv2 = s[kMid + 2] - zm2;
-      const float sl = sqrtf((sv0 * sv0) + (sv1 * sv1) + (sv2 * sv2));
-      if (!(g > sl)) continue;
+      // the reference's g > s with s = sqrtf(s2): sqrtf is monotone, so
+      // s2 >= g2 rules it out, and s2 below g2 (1 - 2^-20) (normal range)
+      // decides it without the root (two correctly rounded roots cannot meet
+      // across that gap); in between, the root itself
+      const float s2 = (sv0 * sv0) + (sv1 * sv1) + (sv2 * sv2);
+      if (!(s2 < g2)) continue;
+      if (!(s2 < g2 * kGap && g2 > 1e-30f) && !(g > sqrtf(s2))) continue;
       const float n0 = s[kNrm], n1 = s[kNrm + 1], n2 = s[kNrm + 2];
       const float tc1 = (sv0 * n0) + (sv1 * n1) + (sv2 * n2);
       const float tc2 = (gv0 * n0) + (gv1 * n1) + (gv2 * n2);
       const float t = tc1 / tc2;
       if (!(t > 0.0f)) continue;
       const float x0 = (t * gv0) + zm0, x1 = (t * gv1) + zm1, x2 = (t * gv2) + zm2;
-      // edges from v0 (e1), v2 (e2), v1 (e3), each crossed with x - its origin
-      const float a0 = x0 - s[kV0], a1 = x1 - s[kV0 + 1], a2 = x2 - s[kV0 + 2];
-      const float b0 = x0 - s[kV2], b1 = x1 - s[kV2 + 1], b2 = x2 - s[kV2 + 2];
-      const float c0 = x0 - s[kV1], c1 = x1 - s[kV1 + 1], c2 = x2 - s[kV1 + 2];
+      // edges from v0 (e1), v2 (e2), v1 (e3), each crossed with x - its
+      // origin and dotted with the normal: inside when all three are >= 0
+      // (tested one at a time: most points fail the first)
       const float* e1 = s + kE1;
-      const float* e2 = s + kE2;
-      const float* e3 = s + kE3;
+      const float a0 = x0 - s[kV0], a1 = x1 - s[kV0 + 1], a2 = x2 - s[kV0 + 2];
       const float k1 = (((e1[1] * a2) - (e1[2] * a1)) * n0) + (((e1[2] * a0) - (e1[0] * a2)) * n1) +
                        (((e1[0] * a1) - (e1[1] * a0)) * n2);
+      if (!(k1 >= 0.0f)) continue;
+      const float* e2 = s + kE2;
+      const float b0 = x0 - s[kV2], b1 = x1 - s[kV2 + 1], b2 = x2 - s[kV2 + 2];
       const float k2 = (((e2[1] * b2) - (e2[2] * b1)) * n0) + (((e2[2] * b0) - (e2[0] * b2)) * n1) +
                        (((e2[0] * b1) - (e2[1] * b0)) * n2);
+      if (!(k2 >= 0.0f)) continue;
+      const float* e3 = s + kE3;
+      const float c0 = x0 - s[kV1], c1 = x1 - s[kV1 + 1], c2 = x2 - s[kV1 + 2];
       const float k3 = (((e3[1] * c2) - (e3[2] * c1)) * n0) + (((e3[2] * c0) - (e3[0] * c2)) * n1) +
                        (((e3[0] * c1) - (e3[1] * c0)) * n2);
-      if (k1 >= 0.0f && k2 >= 0.0f && k3 >= 0.0f) vis = false;
+      if (k3 >= 0.0f) vis = false;
     }
   }
   if (act) keep[beg + zi] = vis ? 1 : 0;
@@ -442,50 +453,80 @@ extern "C" __global__ __launch_bounds__(64) void btpm_drf_rows(const int* __rest
   for (int e = rowptr[f] + (int)threadIdx.x; e < rowptr[f + 1]; e += 64) row[e] = f;
 }
 
-// T3905/main.cpp:1127-1176: one (entry q, crater sub-facet j) per lane.
-//   crater[j] = (rim-sample origin = vertex 2 [3], trivectors [2][3], unit
-//   normal [3]); mul[p] = the sample's trivector multipliers (:1101-1114)
+// T3905/main.cpp:1127-1176: a workgroup per tile of 64 list entries (lane =
+// entry), wave w taking crater sub-facets j = w, w+4, ...; the per-entry
+// direction in the facet frame is computed once per lane.
+//   csub[j]       = unit normal of crater sub-facet j (xyz, pad)
+//   samp[j][p]    = rim-visibility sample p of sub-facet j (xyz, pad): the
+//                   reference's vertex 2 + mul_p0 trivector0 + mul_p1 trivector1
+//                   (:1159-1161), the same float operations, done on the host
+//   vtab[c]       = the visibility after c shadowed samples (the reference's
+//                   running 1 - c/(m*m) with its clamp at 0, :1165-1166)
+// A sample is shadowed when its height is at most |w_z|/|w_xy| times its
+// distance to the rim along w (:1162-1165); the count of shadowed samples
+// decides the visibility, so the loop counts them.  Results go through an LDS
+// tile so that each entry's row of ncf factors is stored contiguously.
 extern "C" __global__ __launch_bounds__(256) void btpm_drf(
     const double* __restrict__ frame, const int* __restrict__ row, const float* __restrict__ vfv,
-    const float* __restrict__ dir, const float* __restrict__ crater, const float* __restrict__ mul,
-    int ncf, int nsamp, float radius, float step, long long n, float* __restrict__ drf) {
+    const float* __restrict__ dir, const float4* __restrict__ csub,
+    const float4* __restrict__ samp, const float* __restrict__ vtab, int ncf, int nsamp,
+    float radius, long long nnz, float* __restrict__ drf) {
   extern __shared__ float sm[];
-  float* sc = sm;                 // [ncf][12]
-  float* smul = sm + 12 * ncf;    // [nsamp][2]
-  for (int i = threadIdx.x; i < 12 * ncf; i += blockDim.x) sc[i] = crater[i];
-  for (int i = threadIdx.x; i < 2 * nsamp; i += blockDim.x) smul[i] = mul[i];
-  __syncthreads();
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const long long q = t / ncf;
-  const int j = (int)(t - q * ncf);
-  const double* M = frame + 9 * (size_t)row[q];
-  const double d0 = dir[3 * q], d1 = dir[3 * q + 1], d2 = dir[3 * q + 2];
-  const float w0 = (float)((M[0] * d0) + (M[1] * d1) + (M[2] * d2));
-  const float w1 = (float)((M[3] * d0) + (M[4] * d1) + (M[5] * d2));
-  const float w2 = (float)((M[6] * d0) + (M[7] * d1) + (M[8] * d2));
-  const float a = fabsf(w2) / sqrtf((w0 * w0) + (w1 * w1));
-  const float* c = sc + 12 * j;
-  const float o0 = c[0], o1 = c[1], o2 = c[2];
-  const float u0 = c[3], u1 = c[4], u2 = c[5], v0 = c[6], v1 = c[7], v2 = c[8];
-  float vis = 1.0f;
-  const float ang = (w0 * c[9]) + (w1 * c[10]) + (w2 * c[11]);
-  if (ang < 0.0f) vis = 0.0f;
-  const float rr = radius * radius;
-  for (int p = 0; p < nsamp; ++p) {
-    const float ma = smul[2 * p], mb = smul[2 * p + 1];
-    const float mp0 = o0 + (ma * u0) + (mb * v0);
-    const float mp1 = o1 + (ma * u1) + (mb * v1);
-    const float mp2 = o2 + (ma * u2) + (mb * v2);
-    const float tc1 = ((-1.0f * w1) * mp0) + (w0 * mp1);
-    const float sr = sqrtf(rr - (tc1 * tc1));
-    const float tc0 = (w0 * mp0) + (w1 * mp1) - sr;
-    if (mp2 <= (a * tc0) && vis != 0.0f) vis = vis - step;
-    if (vis < 0.0f) vis = 0.0f;
+  float* tile = sm;                        // [64][ncf + 1]
+  float* svt = sm + 64 * (ncf + 1);        // [nsamp + 1]
+  for (int i = threadIdx.x; i <= nsamp; i += blockDim.x) svt[i] = vtab[i];
+  const long long q0 = (long long)blockIdx.x * 64;
+  const int lane = threadIdx.x & 63;
+  // the wave's crater sub-facets are uniform across its lanes: their normals
+  // and samples come through scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const long long q = q0 + lane;
+  const bool act = q < nnz;
+  float w0 = 0.0f, w1 = 0.0f, w2 = 1.0f, a = 0.0f, vf = 0.0f;
+  if (act) {
+    const double* M = frame + 9 * (size_t)row[q];
+    const double d0 = dir[3 * q], d1 = dir[3 * q + 1], d2 = dir[3 * q + 2];
+    w0 = (float)((M[0] * d0) + (M[1] * d1) + (M[2] * d2));
+    w1 = (float)((M[3] * d0) + (M[4] * d1) + (M[5] * d2));
+    w2 = (float)((M[6] * d0) + (M[7] * d1) + (M[8] * d2));
+    a = fabsf(w2) / sqrtf((w0 * w0) + (w1 * w1));
+    vf = vfv[q];
   }
-  float v = (vfv[q] * vis * ang) / w2;
-  if (v > 1.0f) v = 1.0f;
-  drf[t] = v;
+  const float nw1 = -1.0f * w1;
+  const float rr = radius * radius;
+  __syncthreads();
+  for (int j = wave; j < ncf; j += 4) {
+    const float4 nj = csub[j];
+    const float ang = (w0 * nj.x) + (w1 * nj.y) + (w2 * nj.z);
+    float vis = 0.0f;
+    if (act && !(ang < 0.0f)) {
+      const float4* sp = samp + (size_t)__builtin_amdgcn_readfirstlane(j) * nsamp;
+      auto shadowed = [&](const float4 m) -> int {
+        const float tc1 = (nw1 * m.x) + (w0 * m.y);
+        const float sr = sqrtf(rr - (tc1 * tc1));
+        const float tc0 = (w0 * m.x) + (w1 * m.y) - sr;
+        return (m.z <= (a * tc0)) ? 1 : 0;
+      };
+      int hits = 0, p = 0;
+      for (; p + 4 <= nsamp; p += 4) {
+        const int pu = __builtin_amdgcn_readfirstlane(p);
+        const float4 m0 = sp[pu], m1 = sp[pu + 1], m2 = sp[pu + 2], m3 = sp[pu + 3];
+        hits += shadowed(m0) + shadowed(m1) + shadowed(m2) + shadowed(m3);
+      }
+      for (; p < nsamp; ++p) hits += shadowed(sp[__builtin_amdgcn_readfirstlane(p)]);
+      vis = svt[hits];
+    }
+    float r = (vf * vis * ang) / w2;
+    if (r > 1.0f) r = 1.0f;
+    tile[lane * (ncf + 1) + j] = r;
+  }
+  __syncthreads();
+  const long long rows = min(64ll, nnz - q0);
+  float* out = drf + q0 * ncf;
+  for (int i = threadIdx.x; i < rows * ncf; i += blockDim.x) {
+    const int e = i / ncf, j = i - e * ncf;
+    out[i] = tile[e * (ncf + 1) + j];
+  }
 }
 
 // ===========================================================================
@@ -664,7 +705,7 @@ extern "C" int btpm_vf_fetch(const btpm_vf* v, int* rowptr, int* col, float* vf,
 
 extern "C" void btpm_vf_free(btpm_vf* v) {
   if (!v) return;
-  hipSetDevice(v->device);
+  (void)hipSetDevice(v->device);
   delete v;
 }
 
@@ -672,7 +713,8 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
   if (!ctx || !in || !drf || !in->vtx || !in->facets || !in->rowptr || !in->crater_vtx ||
       !in->crater_facets)
     return fail(BTPM_ERR_ARG, "null argument");
-  if (in->nv < 3 || in->nf < 1 || in->ncv < 3 || in->ncf < 1 || in->m < 1 || in->m > 64)
+  if (in->nv < 3 || in->nf < 1 || in->ncv < 3 || in->ncf < 1 || in->ncf > 128 || in->m < 1 ||
+      in->m > 64)
     return fail(BTPM_ERR_ARG, "bad sizes: nv %d nf %d ncv %d ncf %d m %d", in->nv, in->nf, in->ncv,
                 in->ncf, in->m);
   if (in->rowptr[0] != 0) return fail(BTPM_ERR_ARG, "rowptr[0] must be 0");
@@ -693,30 +735,10 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int ncf = in->ncf, m = in->m, nsamp = m * m;
-  // crater sub-facets (T3905/main.cpp:819-860): vertex 2, trivectors, unit normal
-  std::vector<float> cr(12 * (size_t)ncf), mul(2 * (size_t)nsamp);
-  for (int j = 0; j < ncf; ++j) {
-    const float* p0 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j];
-    const float* p1 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 1];
-    const float* p2 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 2];
-    float a[3], b[3], n[3];
-    for (int k = 0; k < 3; ++k) {
-      a[k] = p1[k] - p0[k];
-      b[k] = p2[k] - p0[k];
-    }
-    n[0] = (a[1] * b[2]) - (a[2] * b[1]);
-    n[1] = (a[2] * b[0]) - (a[0] * b[2]);
-    n[2] = (a[0] * b[1]) - (a[1] * b[0]);
-    const float l = std::sqrt((n[0] * n[0]) + (n[1] * n[1]) + (n[2] * n[2]));
-    float* c = cr.data() + 12 * (size_t)j;
-    for (int k = 0; k < 3; ++k) {
-      c[k] = p2[k];
-      c[3 + k] = -1.0f * b[k] / (float)m;
-      c[6 + k] = a[k] / (float)m;
-      c[9 + k] = n[k] / l;
-    }
-  }
-  // sample multipliers (T3905/main.cpp:1101-1114)
+  // crater sub-facets (T3905/main.cpp:819-860): unit normal, and the
+  // rim-visibility samples vertex 2 + mul0 trivector0 + mul1 trivector1
+  // with the multipliers of :1101-1114
+  std::vector<float> mul(2 * (size_t)nsamp);
   for (int k = 1; k != m + 1; ++k)
     for (int l = 1; l != k + 1; ++l) {
       const int p = ((k - 1) * (k - 1)) + (2 * (l - 1));
@@ -729,14 +751,53 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
       mul[2 * p] = (float)k - (2.0f / 3.0f);
       mul[2 * p + 1] = (float)l - (1.0f / 3.0f);
     }
-  DBuf d_vtx, d_fv, d_rp, d_vf, d_dir, d_cr, d_mul, d_frame, d_row, d_out;
+  std::vector<float4> csub(ncf), samp((size_t)ncf * nsamp);
+  for (int j = 0; j < ncf; ++j) {
+    const float* p0 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j];
+    const float* p1 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 1];
+    const float* p2 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 2];
+    float a[3], b[3], n[3], t0[3], t1[3];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = p1[k] - p0[k];
+      b[k] = p2[k] - p0[k];
+    }
+    n[0] = (a[1] * b[2]) - (a[2] * b[1]);
+    n[1] = (a[2] * b[0]) - (a[0] * b[2]);
+    n[2] = (a[0] * b[1]) - (a[1] * b[0]);
+    const float l = std::sqrt((n[0] * n[0]) + (n[1] * n[1]) + (n[2] * n[2]));
+    csub[j] = make_float4(n[0] / l, n[1] / l, n[2] / l, 0.0f);
+    for (int k = 0; k < 3; ++k) {
+      t0[k] = -1.0f * b[k] / (float)m;
+      t1[k] = a[k] / (float)m;
+    }
+    for (int p = 0; p < nsamp; ++p) {
+      float mp[3];
+      for (int k = 0; k < 3; ++k)
+        mp[k] = p2[k] + (mul[2 * p] * t0[k]) + (mul[2 * p + 1] * t1[k]);
+      samp[(size_t)j * nsamp + p] = make_float4(mp[0], mp[1], mp[2], 0.0f);
+    }
+  }
+  // visibility after c shadowed samples (T3905/main.cpp:1165-1166)
+  std::vector<float> vtab(nsamp + 1);
+  {
+    const float step = 1.0f / (float)(m * m);
+    float vis = 1.0f;
+    vtab[0] = vis;
+    for (int c = 1; c <= nsamp; ++c) {
+      if (vis != 0.0f) vis = vis - step;
+      if (vis < 0.0f) vis = 0.0f;
+      vtab[c] = vis;
+    }
+  }
+  DBuf d_vtx, d_fv, d_rp, d_vf, d_dir, d_cs, d_samp, d_vt, d_frame, d_row, d_out;
   upload(d_vtx, in->vtx, 3 * (size_t)in->nv, st);
   upload(d_fv, in->facets, 3 * (size_t)in->nf, st);
   upload(d_rp, in->rowptr, (size_t)in->nf + 1, st);
   upload(d_vf, in->vf, (size_t)nnz, st);
   upload(d_dir, in->dir, 3 * (size_t)nnz, st);
-  upload(d_cr, cr.data(), cr.size(), st);
-  upload(d_mul, mul.data(), mul.size(), st);
+  upload(d_cs, csub.data(), csub.size(), st);
+  upload(d_samp, samp.data(), samp.size(), st);
+  upload(d_vt, vtab.data(), vtab.size(), st);
   d_frame.alloc(9 * (size_t)in->nf * sizeof(double));
   d_row.alloc((size_t)nnz * sizeof(int));
   const long long n = nnz * ncf;
@@ -744,12 +805,11 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
   hipLaunchKernelGGL(btpm_drf_frames, dim3(grid1(in->nf, 256)), dim3(256), 0, st,
                      d_vtx.as<double>(), d_fv.as<int>(), in->nf, d_frame.as<double>());
   hipLaunchKernelGGL(btpm_drf_rows, dim3(in->nf), dim3(64), 0, st, d_rp.as<int>(), d_row.as<int>());
-  const size_t shm = (12 * (size_t)ncf + 2 * (size_t)nsamp) * sizeof(float);
-  if (shm > 60 * 1024) return fail(BTPM_ERR_ARG, "crater model too large (%d facets)", ncf);
-  hipLaunchKernelGGL(btpm_drf, dim3(grid1((size_t)n, 256)), dim3(256), shm, st,
+  const size_t shm = (64 * ((size_t)ncf + 1) + (size_t)nsamp + 1) * sizeof(float);
+  hipLaunchKernelGGL(btpm_drf, dim3(grid1((size_t)nnz, 64)), dim3(256), shm, st,
                      d_frame.as<double>(), d_row.as<int>(), d_vf.as<float>(), d_dir.as<float>(),
-                     d_cr.as<float>(), d_mul.as<float>(), ncf, nsamp, in->crater_radius,
-                     1.0f / (float)(m * m), n, d_out.as<float>());
+                     d_cs.as<float4>(), d_samp.as<float4>(), d_vt.as<float>(), ncf, nsamp,
+                     in->crater_radius, nnz, d_out.as<float>());
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(drf, d_out.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -6,6 +6,10 @@ Formats (whitespace-separated text unless noted):
   - view-factor list `<name>_selfheating_list.txt`: per facet
     "n idx_1 .. idx_n vf_1 .. vf_n" (readselfheating, T3905/main.cpp:418-434);
   - `<name>_selfheating_vector_list.txt`: per facet 3 floats per dependent;
+  - `<name>_photovectors.txt`: per facet "x y z" (the photon momentum
+    vector); `<name>_selfheating_map.txt` / `_total_viewfactor_map.txt`: vertex
+    lines, then "v0 v1 v2 value" per facet (Shape_Selfheating's writers,
+    3905/Shape_Selfheating/main.cpp:837-888: every value followed by a tab);
   - temperature text `<shape>_temperature(<TI>).txt`: vertex lines, facet
     lines, then NF*NTAU lines of surface T, facet-major
     (writeglobaltemperature, T3905/main.cpp:2290-2308);
@@ -90,13 +94,58 @@ def read_selfheating_list(path, nf: int):
     return rowptr, colidx, vf
 
 
+def _g(x) -> str:
+    """A float as an ostream prints it by default (6 significant digits)."""
+    return f"{float(x):g}"
+
+
 def write_selfheating_list(path, csr):
+    """`_selfheating_list.txt` as Shape_Selfheating writes it
+    (3905/Shape_Selfheating/main.cpp:857-869): per facet the count, the
+    partner indices, then the view factors, each followed by a tab."""
     rowptr, colidx, vf = csr
     with open(path, "w") as fh:
         for f in range(len(rowptr) - 1):
             a, b = rowptr[f], rowptr[f + 1]
-            fh.write("\t".join([str(b - a)] + [str(int(x)) for x in colidx[a:b]]
-                               + [f"{float(x):g}" for x in vf[a:b]]) + "\n")
+            fh.write("".join([f"{b - a}\t"] + [f"{int(x)}\t" for x in colidx[a:b]]
+                             + [f"{_g(x)}\t" for x in vf[a:b]]) + "\n")
+
+
+def write_selfheating_vectors(path, rowptr, dirs):
+    """`_selfheating_vector_list.txt` (main.cpp:870-880): per facet the unit
+    direction of each partner, 3 values each followed by a tab."""
+    with open(path, "w") as fh:
+        for f in range(len(rowptr) - 1):
+            a, b = rowptr[f], rowptr[f + 1]
+            fh.write("".join(f"{_g(x)}\t" for x in np.asarray(dirs[a:b]).reshape(-1)) + "\n")
+
+
+def read_selfheating_vectors(path, rowptr):
+    """Directions [nnz][3] of a `_selfheating_vector_list.txt` for the rows
+    of a view-factor list (readselfheating_vectors, T3905/main.cpp:436-446)."""
+    t = np.array(_tokens(path), dtype=np.float64)
+    n = int(rowptr[-1])
+    if len(t) < 3 * n:
+        raise ValueError(f"{path}: expected {3 * n} values, found {len(t)}")
+    return t[:3 * n].reshape(n, 3).astype(np.float32)
+
+
+def write_photovectors(path, photon):
+    """`_photovectors.txt` (main.cpp:882-886): "x\ty\tz" per facet."""
+    with open(path, "w") as fh:
+        for p in np.asarray(photon):
+            fh.write(f"{_g(p[0])}\t{_g(p[1])}\t{_g(p[2])}\n")
+
+
+def write_facet_map(path, V, F, values):
+    """`_selfheating_map.txt` / `_total_viewfactor_map.txt` (main.cpp:
+    837-856): vertex lines "x\ty\tz", then "v0\tv1\tv2\tvalue" per facet."""
+    with open(path, "w") as fh:
+        for v in np.asarray(V):
+            fh.write(f"{_g(v[0])}\t{_g(v[1])}\t{_g(v[2])}\n")
+        for f, x in zip(np.asarray(F), np.asarray(values)):
+            val = str(int(x)) if np.issubdtype(np.asarray(values).dtype, np.integer) else _g(x)
+            fh.write(f"{int(f[0])}\t{int(f[1])}\t{int(f[2])}\t{val}\n")
 
 
 def crater_vf_matrix(csr, nsf: int = 100):

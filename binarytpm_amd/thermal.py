@@ -417,6 +417,15 @@ def crater_viewfactors(ctx: Context, vtx, facets, rowptr, vf, dirs, crater_vtx, 
     return out
 
 
+def probe_valu(ctx: Context, op: str = "fma", iters: int = 8192) -> float:
+    """VALU issue ceiling (wave instructions per second) of one unpacked FP32
+    instruction kind: "fma", "add", "mul" or "sqrt"."""
+    code = {"fma": 0, "add": 1, "mul": 2, "sqrt": 3}[op]
+    r = C.c_double()
+    check(lib().btpm_probe_valu(ctx.handle, code, int(iters), C.byref(r)))
+    return float(r.value)
+
+
 def crater_contract(ctx: Context, vf, e) -> np.ndarray:
     """q[n][j] = sum_i VF(j->i) e[n][i] through the MFMA split product."""
     vf = _f32(vf, (NS * NS,))

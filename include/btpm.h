@@ -292,6 +292,12 @@ int btpm_server_stop(btpm_server *srv, btpm_batch_stats *stats);
  * q[n][j] = sum_i VF(j->i) e[n][i] for n < ncol, computed with the same
  * fp16 hi/lo MFMA split; e is [ncol][100], q is [ncol][100].                */
 int btpm_crater_contract(btpm_ctx *ctx, const float *vf, const float *e, int ncol, float *q);
+/* VALU issue ceiling of the device: a calibration kernel of 8 independent
+ * chains per lane of one unpacked FP32 instruction kind (op 0 fma, 1 add,
+ * 2 mul, 3 hardware sqrt), iters iterations, 8 waves per SIMD; returns the
+ * wave instructions issued per second (the unit of the compute-bound
+ * kernels' counter summaries).  No counterpart in the reference.           */
+int btpm_probe_valu(btpm_ctx *ctx, int op, int iters, double *wave_instr_per_s);
 
 /* Outputs [nf][ntau] (host memory): shadow (1 lit, 0 back-facing or shadowed
  * by a facet nearer the Sun, including the partner body: eclipses), illum =
@@ -343,7 +349,7 @@ typedef struct {
     const int *rowptr;     /* [nf+1] facet view-factor rows                        */
     const float *vf;       /* [nnz] view factors                                   */
     const float *dir;      /* [nnz][3] vector-list directions                      */
-    int ncv, ncf;
+    int ncv, ncf;                /* ncf at most 128                                 */
     const float *crater_vtx;     /* [ncv][3] crater model vertices                */
     const int *crater_facets;    /* [ncf][3]                                      */
     int m;                 /* rim-visibility samples per crater facet edge (m*m

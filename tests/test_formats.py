@@ -67,3 +67,26 @@ def test_crater_list_transposition():
     # row j of the list holds (i, VF(j->i)); storage M[i][j] = VF(j->i)
     assert M[1, 0] == 0.25 and M[2, 0] == 0.5 and M[0, 1] == 0.125
     assert M.trace() == 0
+
+
+def test_producer_files_layout(tmp_path):
+    """Shape_Selfheating's writers put a tab after every value
+    (3905/Shape_Selfheating/main.cpp:837-888); the readers take them back."""
+    rp = np.array([0, 2, 2, 3], np.int32)
+    ci = np.array([1, 2, 0], np.int32)
+    vf = np.array([0.125, 1e-7, 0.5], np.float32)
+    dirs = np.array([[1, 0, 0], [0, 0.6, 0.8], [0, 0, -1]], np.float32)
+    p = str(tmp_path / "s")
+    formats.write_selfheating_list(p + "_selfheating_list.txt", (rp, ci, vf))
+    assert open(p + "_selfheating_list.txt").read() == "2\t1\t2\t0.125\t1e-07\t\n0\t\n1\t0\t0.5\t\n"
+    formats.write_selfheating_vectors(p + "_v.txt", rp, dirs)
+    assert open(p + "_v.txt").read().splitlines()[1] == ""
+    np.testing.assert_array_equal(formats.read_selfheating_vectors(p + "_v.txt", rp), dirs)
+    formats.write_photovectors(p + "_photovectors.txt", dirs)
+    assert open(p + "_photovectors.txt").read().splitlines()[1] == "0\t0.6\t0.8"
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1.5, 0]], np.float32)
+    F = np.array([[0, 1, 2], [0, 2, 1], [1, 2, 0]])
+    formats.write_facet_map(p + "_map.txt", V, F, np.diff(rp))
+    assert open(p + "_map.txt").read().splitlines()[2:4] == ["0\t1.5\t0", "0\t1\t2\t2"]
+    formats.write_facet_map(p + "_tot.txt", V, F, np.array([0.5, 0, 0.25], np.float32))
+    assert open(p + "_tot.txt").read().splitlines()[-1] == "1\t2\t0\t0.25"

@@ -190,3 +190,89 @@ def test_drf_doppler_binary(ctx):
     print(f"[parity] drf doppler_binary: {g.shape[0]} x {g.shape[1]} in {tg:.2f} s "
           f"(incl. transfers), 64 rows checked, mismatches {mism}", flush=True)
     assert mism == 0
+
+
+def _write_shape(path, V, F):
+    with open(path, "w") as fh:
+        for v in V:
+            fh.write(f"{v[0]:.9g}\t{v[1]:.9g}\t{v[2]:.9g}\n")
+        for f in F:
+            fh.write(f"{f[0]}\t{f[1]}\t{f[2]}\n")
+
+
+def test_selfheating_cli_reproduces_shipped_crater_files(tmp_path):
+    """btpm_selfheating on the shipped crater mesh (every pair refined, as
+    its shipped list) writes the reference's list and photon vectors."""
+    from binarytpm_amd import formats, vfcli
+    d = np.load(os.path.join(GOLD, "new_crater2.npz"))
+    base = str(tmp_path / "new_crater2")
+    _write_shape(base + ".txt", d["V"], d["F"])
+    assert vfcli.main(["--shape", base, "--nv", "61", "--nf", "100", "--refine-min", "0"]) == 0
+    rp, ci, vf = formats.read_selfheating_list(base + "_selfheating_list.txt", 100)
+    assert np.array_equal(rp, d["rowptr"]) and np.array_equal(ci, d["colidx"])
+    assert (np.abs(vf - d["vf"]) / d["vf"]).max() < 1e-4
+    ph = np.loadtxt(base + "_photovectors.txt")
+    assert np.abs(ph - d["photon"]).max() < 2e-6
+    dirs = formats.read_selfheating_vectors(base + "_selfheating_vector_list.txt", rp)
+    # refined pairs: the VF-weighted mean of the sub-pair unit directions
+    nrm = np.linalg.norm(dirs, axis=1)
+    assert (nrm <= 1.0 + 1e-5).all() and (nrm > 0.85).all()
+    tot = np.loadtxt(base + "_total_viewfactor_map.txt", skiprows=61)
+    assert tot.shape == (100, 4) and np.allclose(tot[:, 3], np.bincount(
+        np.repeat(np.arange(100), np.diff(rp)), weights=vf, minlength=100), rtol=1e-4)
+
+
+def test_thermal_cli_uses_producer_files(ctx, tmp_path):
+    """File-level chain: btpm_selfheating writes the lists of a dented
+    binary, btpm_thermal reads them with the crater model, computes the
+    shape->crater view factors from the vector list on the GPU and solves;
+    its temperature files equal the same chain through the library API."""
+    import math
+
+    from binarytpm_amd import cli, formats, vfcli
+    equ, fv, npv = inputs.binary_shape(2, 0, sep=1.4, dent=0.35)
+    nf, ntau, period = len(fv), 650, 4.3 * 3600.0
+    base = str(tmp_path / "body")
+    _write_shape(base + ".txt", equ, fv)
+    assert vfcli.main(["--shape", base, "--nv", str(len(equ)), "--nf", str(nf)]) == 0
+    ill = th.geometry_illumination(ctx, equ, fv, ntau, (1.1, 0.0, 0.0),
+                                   (math.pi / 2, 0.0), nv_primary=npv, sun_facet=False).illum
+    formats.write_temperature_dat(base + "_illumination.dat", equ, fv, ill)
+    d = np.load(os.path.join(GOLD, "new_crater2.npz"))
+    crater = str(tmp_path / "new_crater2")
+    _write_shape(crater + ".txt", d["V"], d["F"])
+    formats.write_selfheating_list(crater + "_selfheating_list.txt",
+                                   (d["rowptr"], d["colidx"], d["vf"]))
+    cil = np.repeat(ill[:, None, :], 100, axis=1) * np.linspace(0.5, 1.0, 100)[None, :, None]
+    formats.write_roughness_dat(base + "_roughness_illumination.dat", d["V"], d["F"],
+                                np.ascontiguousarray(np.transpose(cil, (0, 2, 1))))
+    rc = cli.main(["--shape", base, "--nv", str(len(equ)), "--nf", str(nf), "--ntau", str(ntau),
+                   "--period", str(period), "--ti", "100", "--tacc", "0.01", "--selfheating",
+                   "--crater", crater, "--out", str(tmp_path)])
+    assert rc == 0
+    _, _, T_cli = formats.read_temperature_dat(base + "_temperature(100).dat", len(equ), nf, ntau)
+    Tc_cli = formats.read_roughness_dat(base + "_roughness_temperature(100).dat", 61, 100, nf,
+                                        ntau)[2]
+    # the same chain through the API
+    csr = formats.read_selfheating_list(base + "_selfheating_list.txt", nf)
+    dirs = formats.read_selfheating_vectors(base + "_selfheating_vector_list.txt", csr[0])
+    assert csr[0][-1] > 100
+    Vc, Fc = formats.read_shape(crater + ".txt", 61, 100)
+    Vb, Fb = formats.read_shape(base + ".txt", len(equ), nf)   # the vertices as the CLI read them
+    drf = th.crater_viewfactors(ctx, Vb, Fb, csr[0], csr[2], dirs, Vc, Fc)
+    assert (drf > 0).sum() > 1000
+    _, _, cill = formats.read_roughness_dat(base + "_roughness_illumination.dat", 61, 100, nf, ntau)
+    cm = th.CraterModel(vf=formats.crater_vf_matrix(
+        formats.read_selfheating_list(crater + "_selfheating_list.txt", 100)),
+        illum=np.ascontiguousarray(cill), drf=drf)
+    p = th.params(ntau, period, tacc=0.01)
+    _, _, ill_back = formats.read_temperature_dat(base + "_illumination.dat", len(equ), nf, ntau)
+    shape = th.Shape(ctx, p, ill_back, 0.1, csr=csr, scatter=0, crater=cm)
+    s = th.Solve(ctx, shape, 100.0, with_craters=True)
+    th.batch_run(ctx, [s])
+    try:
+        assert np.array_equal(s.surface_temps(), T_cli)
+        assert np.array_equal(s.crater_temps(), Tc_cli)
+    finally:
+        s.close()
+        shape.close()
