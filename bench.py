@@ -391,6 +391,14 @@ def geometry_phase(ctx, cpu=True):
     return out
 
 
+def valu_ceiling(ctx):
+    """Wave instructions per second the device issues for unpacked 32-bit
+    VALU instructions of one kind (calibration kernel btpm_probe_valu): the
+    ceiling against which the compute-bound kernels' issue rates are read."""
+    from binarytpm_amd import thermal as th
+    return {op: th.probe_valu(ctx, op, 16384) for op in ("fma", "add", "mul", "iadd", "sqrt")}
+
+
 def viewfactor_phase(ctx, cpu=True):
     """Facet view-factor producer (Shape_Selfheating) on its own input, the
     6400-facet Doppler binary mesh of the reference (a committed fixture)
@@ -525,7 +533,8 @@ def main():
         del src, dst
     if args.producers_only:
         with th.Context(0) as ctx:
-            line = {"illumination_producer": geometry_phase(ctx, cpu=not args.no_cpu),
+            line = {"valu_ceiling_wave_instr_per_s": valu_ceiling(ctx),
+                    "illumination_producer": geometry_phase(ctx, cpu=not args.no_cpu),
                     "viewfactor_producer": viewfactor_phase(ctx, cpu=not args.no_cpu)}
         print(json.dumps(line), flush=True)
         return
@@ -648,6 +657,7 @@ def main():
             line["illumination_producer"] = geometry_phase(ctx, cpu=not args.no_cpu)
         if not args.no_viewfactors:
             line["viewfactor_producer"] = viewfactor_phase(ctx, cpu=not args.no_cpu)
+        line["roofline"]["valu_ceiling_wave_instr_per_s"] = valu_ceiling(ctx)
     print(json.dumps(line), flush=True)
 
 

@@ -14,8 +14,8 @@ constexpr int kChains = 8;
 }
 
 // op 0: v_fma_f32, op 1: v_add_f32, op 2: v_mul_f32, op 3: v_sqrt_f32
-// (the hardware square root, the quarter-rate unit); kChains instructions
-// of that kind per iteration and lane
+// (the hardware square root), op 4: v_add_u32 (32-bit integer add);
+// kChains instructions of that kind per iteration and lane
 template <int OP>
 __global__ __launch_bounds__(256) void btpm_probe_valu_kernel(float* __restrict__ out, int iters,
                                                               float a, float b) {
@@ -29,6 +29,7 @@ __global__ __launch_bounds__(256) void btpm_probe_valu_kernel(float* __restrict_
       if (OP == 1) x[k] = x[k] + a;
       if (OP == 2) x[k] = x[k] * a;
       if (OP == 3) x[k] = __builtin_amdgcn_sqrtf(x[k]);
+      if (OP == 4) x[k] = __uint_as_float(__float_as_uint(x[k]) + __float_as_uint(a));
     }
   }
   float s = 0.0f;
@@ -43,7 +44,7 @@ using namespace btpm;
 
 extern "C" int btpm_probe_valu(btpm_ctx* ctx, int op, int iters, double* wave_instr_per_s) {
   if (!ctx || !wave_instr_per_s) return fail(BTPM_ERR_ARG, "null argument");
-  if (op < 0 || op > 3 || iters < 1 || iters > (1 << 20))
+  if (op < 0 || op > 4 || iters < 1 || iters > (1 << 20))
     return fail(BTPM_ERR_ARG, "bad probe: op %d iters %d", op, iters);
   CtxLock lk(ctx);
   API_BEGIN
@@ -65,8 +66,10 @@ extern "C" int btpm_probe_valu(btpm_ctx* ctx, int op, int iters, double* wave_in
                                  d_out.as<float>(), n, 1e-7f, 0.0f); break;
       case 2: hipLaunchKernelGGL(btpm_probe_valu_kernel<2>, dim3(blocks), dim3(256), 0, st,
                                  d_out.as<float>(), n, 0.99999f, 0.0f); break;
-      default: hipLaunchKernelGGL(btpm_probe_valu_kernel<3>, dim3(blocks), dim3(256), 0, st,
-                                  d_out.as<float>(), n, 0.0f, 0.0f); break;
+      case 3: hipLaunchKernelGGL(btpm_probe_valu_kernel<3>, dim3(blocks), dim3(256), 0, st,
+                                 d_out.as<float>(), n, 0.0f, 0.0f); break;
+      default: hipLaunchKernelGGL(btpm_probe_valu_kernel<4>, dim3(blocks), dim3(256), 0, st,
+                                  d_out.as<float>(), n, 1.0f, 0.0f); break;
     }
   };
   launch(16);  // warm-up

@@ -419,8 +419,8 @@ def crater_viewfactors(ctx: Context, vtx, facets, rowptr, vf, dirs, crater_vtx, 
 
 def probe_valu(ctx: Context, op: str = "fma", iters: int = 8192) -> float:
     """VALU issue ceiling (wave instructions per second) of one unpacked FP32
-    instruction kind: "fma", "add", "mul" or "sqrt"."""
-    code = {"fma": 0, "add": 1, "mul": 2, "sqrt": 3}[op]
+    instruction kind: "fma", "add", "mul", "sqrt" or "iadd" (32-bit integer)."""
+    code = {"fma": 0, "add": 1, "mul": 2, "sqrt": 3, "iadd": 4}[op]
     r = C.c_double()
     check(lib().btpm_probe_valu(ctx.handle, code, int(iters), C.byref(r)))
     return float(r.value)

@@ -293,8 +293,9 @@ int btpm_server_stop(btpm_server *srv, btpm_batch_stats *stats);
  * fp16 hi/lo MFMA split; e is [ncol][100], q is [ncol][100].                */
 int btpm_crater_contract(btpm_ctx *ctx, const float *vf, const float *e, int ncol, float *q);
 /* VALU issue ceiling of the device: a calibration kernel of 8 independent
- * chains per lane of one unpacked FP32 instruction kind (op 0 fma, 1 add,
- * 2 mul, 3 hardware sqrt), iters iterations, 8 waves per SIMD; returns the
+ * chains per lane of one unpacked 32-bit instruction kind (op 0 fma, 1 add,
+ * 2 mul, 3 hardware sqrt, 4 integer add), iters iterations, 8 waves per
+ * SIMD; returns the
  * wave instructions issued per second (the unit of the compute-bound
  * kernels' counter summaries).  No counterpart in the reference.           */
 int btpm_probe_valu(btpm_ctx *ctx, int op, int iters, double *wave_instr_per_s);
