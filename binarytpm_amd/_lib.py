@@ -98,6 +98,21 @@ class DrfIn(C.Structure):
     ]
 
 
+class ObserveIn(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("ntau", C.c_int), ("nv_primary", C.c_int),
+        ("equpos", C.POINTER(C.c_double)), ("facets", C.POINTER(C.c_int)),
+        ("rev_primary", C.c_double), ("rev_secondary", C.c_double),
+        ("hel", C.c_double * 3), ("obs", C.c_double * 3), ("rot", C.c_double * 2),
+        ("au_m", C.c_double), ("visible", C.POINTER(C.c_int)), ("stemp", C.POINTER(C.c_float)),
+        ("rough", C.c_int), ("ncv", C.c_int), ("ncf", C.c_int),
+        ("crater_vtx", C.POINTER(C.c_float)), ("crater_facets", C.POINTER(C.c_int)),
+        ("crater_visible", C.POINTER(C.c_float)), ("crater_temp", C.POINTER(C.c_float)),
+        ("wavelength_um", C.POINTER(C.c_double)), ("colour", C.POINTER(C.c_double)),
+        ("tmin", C.c_int), ("tcount", C.c_int), ("eps", C.c_double),
+    ]
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("wall_s", C.c_double), ("solves", C.c_int), ("converged", C.c_int),
@@ -158,6 +173,8 @@ PROTOTYPES = [
                                 _fp]),
     ("btpm_vf_free", None, [_vp]),
     ("btpm_crater_viewfactors", C.c_int, [_vp, C.POINTER(DrfIn), _fp]),
+    ("btpm_observe", C.c_int, [_vp, C.POINTER(ObserveIn), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)]),
 ]
 
 _LIB = None

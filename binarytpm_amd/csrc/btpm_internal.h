@@ -40,6 +40,10 @@
 #define BTPM_VF_COS_MIN (-0x1.bbbd2cp-24f)
 // largest sub-facet split per edge of a refined pair (n*n sub-facets)
 #define BTPM_VF_NSUB_MAX 32
+// the Observe stage's test acos(c) > pi/2 (double pi = 4 atan 1) holds
+// exactly for -1 <= c <= BTPM_OBS_COS_BEHIND under glibc's acos (checked by
+// bisection and on 2e6 doubles each side of the boundary)
+#define BTPM_OBS_COS_BEHIND (-0x1.cb3b399d747f5p-55)
 // largest accepted r = dtau/dz^2 (FTCS is stable for r <= 1/2): the
 // conduction kernels integrate the depth profile in a variable scaled by
 // (1 - 2r)^-k over 8-step blocks, which must stay inside the FP32 range

@@ -110,6 +110,20 @@ extern "C" __global__ void btpm_drf(const double* frame, const int* row, const f
                                     const float* vtab, int ncf, int nsamp, float radius,
                                     long long nnz, float* drf);
 
+// observe.hip
+extern "C" __global__ void btpm_obs_geom(const double* equpos, const int* fv, int nf,
+                                         int nv_primary, const double* trig, const double* frame,
+                                         int s0, int nsteps, int ntau, const int* visible,
+                                         double* rec, double* area);
+extern "C" __global__ void btpm_obs_flux(const double* rec, const double* area,
+                                         const float* stemp, const float* cvis,
+                                         const float* ctemp, const double4* csub, int ncf,
+                                         double stot, int rough, const double* pl, double eps,
+                                         int nf, int ntau, int s0, int nsteps, double* out);
+extern "C" __global__ void btpm_obs_sum(const double* terms, int nf, int nsteps, const double* bn,
+                                        const double* colour, int tmin, int tcount, double* smooth,
+                                        double* rough);
+
 // stepwise.hip
 extern "C" __global__ void btpm_step_profile(float* temp, float* prof, int ncols, int to_records);
 extern "C" __global__ void btpm_step_fd(btpm_set_dev S, float* temp);

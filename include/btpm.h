@@ -361,6 +361,37 @@ typedef struct {
  * VF vis_j cos_j / dir_z capped at 1.                                      */
 int btpm_crater_viewfactors(btpm_ctx *ctx, const btpm_drf_in *in, float *drf);
 
+/* ---- observed fluxes ---------------------------------------------------------
+ * The Observe stage's flux synthesis (175706/Run-04/observe_generator.cpp:
+ * asteroidgeometry :530-698, surfacegeometry :465-529, observe :699-836),
+ * downstream of the Thermal stage: per time step the flux of the smooth and
+ * of the rough model at the observer in each band, after the colour
+ * correction of the best-fitting colour temperature.  FP64.                 */
+typedef struct {
+    int nv, nf, ntau, nv_primary;
+    const double *equpos;        /* [nv][3] each body's frame (shape units)          */
+    const int *facets;           /* [nf][3]                                          */
+    double rev_primary, rev_secondary;   /* revolutions per cycle                   */
+    double hel[3];               /* system from the Sun: r (AU), ecl. lat, lon (rad) */
+    double obs[3];               /* system from the observer: r (AU), lat, lon       */
+    double rot[2];               /* pole latitude, longitude (rad)                   */
+    double au_m;                 /* astronomical unit in metres (reference 149.6e9)  */
+    const int *visible;          /* [nf][ntau] facet visibility (Visibility stage)   */
+    const float *stemp;          /* [nf][ntau] smooth surface temperatures           */
+    int rough;                   /* 1: crater (rough) fluxes as well                 */
+    int ncv, ncf;
+    const float *crater_vtx;     /* [ncv][3]                                         */
+    const int *crater_facets;    /* [ncf][3]                                         */
+    const float *crater_visible; /* [nf][ntau][ncf] sub-facet visible fractions      */
+    const float *crater_temp;    /* [nf][ntau][ncf] crater temperatures              */
+    const double *wavelength_um; /* [3] band wavelengths (reference WISE W2 W3 W4)   */
+    const double *colour;        /* [3][tcount] colour corrections                   */
+    int tmin, tcount;            /* colour temperatures tmin .. tmin+tcount-1 K      */
+    double eps;                  /* emissivity                                       */
+} btpm_observe_in;
+/* smooth, rough: [ntau][3] (host memory)                                    */
+int btpm_observe(btpm_ctx *ctx, const btpm_observe_in *in, double *smooth, double *rough);
+
 #ifdef __cplusplus
 }
 #endif

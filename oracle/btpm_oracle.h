@@ -247,6 +247,41 @@ typedef struct {
 } orc_drf_in;
 void orc_crater_drf(const orc_drf_in *in, float *drf);
 
+/* Observe flux synthesis (observe.c, FP64): the shape (equpos [nv][3] in
+ * each body's frame, km; the first nv_primary on the primary; fv [nf][3]),
+ * revolutions per cycle of each body, hel / obs = (r AU, ecliptic latitude,
+ * longitude) of the system from the Sun and from the observer, rot = pole
+ * (latitude, longitude), au_m the astronomical unit in metres; visible
+ * [nf][ntau] (the Visibility stage's facet flags), stemp [nf][ntau]; with
+ * rough != 0 the crater mesh cvtx [ncv][3] / cfv [ncf][3], cvis and ctemp
+ * [nf][ntau][ncf]; nwl = 3 wavelengths in microns, colour [nwl][tcount]
+ * corrections for colour temperatures tmin .. tmin+tcount-1 K, eps.  Steps
+ * s_begin .. s_begin+s_count-1; outputs [s_count][nwl] smooth and rough
+ * fluxes after colour correction.                                          */
+typedef struct {
+    int nv, nf, ntau, nv_primary;
+    const double *equpos;
+    const int *fv;
+    double rev_primary, rev_secondary;
+    double hel[3], obs[3], rot[2];
+    double au_m;
+    const int *visible;
+    const float *stemp;
+    int rough;
+    int ncv, ncf;
+    const float *cvtx;
+    const int *cfv;
+    const float *cvis;
+    const float *ctemp;
+    int nwl;
+    const double *wavelength_um;
+    const double *colour;
+    int tmin, tcount;
+    double eps;
+    int s_begin, s_count;
+} orc_observe_in;
+void orc_observe(const orc_observe_in *in, double *smooth_out, double *rough_out);
+
 #ifdef __cplusplus
 }
 #endif

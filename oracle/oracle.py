@@ -467,3 +467,58 @@ def crater_drf(vtx, fv, rowptr, vf, dirs, cvtx, cfv, m=10, crater_radius=10.0):
     out = np.zeros((int(rowptr[-1]), q.ncf), np.float32)
     lib().orc_crater_drf(C.byref(q), _ptr(out, _fp))
     return out
+
+
+class OrcObserveIn(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("nf", C.c_int), ("ntau", C.c_int), ("nv_primary", C.c_int),
+        ("equpos", _dp), ("fv", _ip), ("rev_primary", C.c_double), ("rev_secondary", C.c_double),
+        ("hel", C.c_double * 3), ("obs", C.c_double * 3), ("rot", C.c_double * 2),
+        ("au_m", C.c_double), ("visible", _ip), ("stemp", _fp), ("rough", C.c_int),
+        ("ncv", C.c_int), ("ncf", C.c_int), ("cvtx", _fp), ("cfv", _ip), ("cvis", _fp),
+        ("ctemp", _fp), ("nwl", C.c_int), ("wavelength_um", _dp), ("colour", _dp),
+        ("tmin", C.c_int), ("tcount", C.c_int), ("eps", C.c_double), ("s_begin", C.c_int),
+        ("s_count", C.c_int),
+    ]
+
+
+AU_M = 149.6e9                                # observe_generator.cpp:9
+WISE_UM = (4.6028, 11.0984, 22.6405)          # observe_generator.cpp:43
+
+
+def observe(equpos, fv, ntau, hel, obs, rot, visible, stemp, colour, nv_primary=None,
+            rev_primary=1.0, rev_secondary=1.0, crater=None, eps=0.95, wavelengths=WISE_UM,
+            au_m=AU_M, steps=None):
+    """Observed fluxes [s_count][3] (smooth, rough) of observe.c.  crater =
+    (cvtx [ncv][3], cfv [ncf][3], cvis [nf][ntau][ncf], ctemp [nf][ntau][ncf])
+    or None (no roughness); colour [3][tcount] for 100 K upwards."""
+    eq = _d(equpos)
+    f = _i(fv)
+    vis = _i(visible)
+    st = _f(stemp)
+    col = _d(colour)
+    wl = _d(wavelengths)
+    q = OrcObserveIn()
+    q.nv, q.nf, q.ntau = eq.shape[0], f.shape[0], int(ntau)
+    q.nv_primary = eq.shape[0] if nv_primary is None else int(nv_primary)
+    q.equpos, q.fv = _ptr(eq, _dp), _ptr(f, _ip)
+    q.rev_primary, q.rev_secondary = float(rev_primary), float(rev_secondary)
+    q.hel[:] = [float(x) for x in hel]
+    q.obs[:] = [float(x) for x in obs]
+    q.rot[:] = [float(x) for x in rot]
+    q.au_m = float(au_m)
+    q.visible, q.stemp = _ptr(vis, _ip), _ptr(st, _fp)
+    keep = []
+    if crater is not None:
+        cv, cf, cvis, ct = (_f(crater[0]), _i(crater[1]), _f(crater[2]), _f(crater[3]))
+        keep += [cv, cf, cvis, ct]
+        q.rough, q.ncv, q.ncf = 1, cv.shape[0], cf.shape[0]
+        q.cvtx, q.cfv, q.cvis, q.ctemp = (_ptr(cv, _fp), _ptr(cf, _ip), _ptr(cvis, _fp),
+                                          _ptr(ct, _fp))
+    q.nwl, q.wavelength_um, q.colour = len(wl), _ptr(wl, _dp), _ptr(col, _dp)
+    q.tmin, q.tcount, q.eps = 100, col.shape[1], float(eps)
+    q.s_begin, q.s_count = (0, int(ntau)) if steps is None else (int(steps[0]), int(steps[1]))
+    sm = np.zeros((q.s_count, len(wl)), np.float64)
+    ro = np.zeros((q.s_count, len(wl)), np.float64)
+    lib().orc_observe(C.byref(q), _ptr(sm, _dp), _ptr(ro, _dp))
+    return sm, ro

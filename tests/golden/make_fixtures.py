@@ -1,12 +1,13 @@
 """Writes the small data fixtures the CPU tests use, from data files that ship
 in the reference tree (data only: meshes, view-factor lists, photon momentum
-vectors, the mesh statistics file).  Run in a checkout that has the reference
+vectors, the mesh statistics file, colour corrections).  Run in a checkout that has the reference
 at REF:
 
     python tests/golden/make_fixtures.py [REF]
 
 Outputs (committed): tests/golden/new_crater2.npz,
-tests/golden/doppler_binary.npz, tests/golden/doppler_binary_properties.json
+tests/golden/doppler_binary.npz, tests/golden/doppler_binary_properties.json,
+tests/golden/fg3_observe.npz (WISE colour corrections and the 1996 FG3 mesh)
 """
 import json
 import os
@@ -30,6 +31,14 @@ def main():
                         colidx=csr[1], vf=csr[2], photon=photon)
     V, F = formats.read_shape(os.path.join(D, "doppler_binary.txt"), 3204, 6400)
     np.savez_compressed(os.path.join(HERE, "doppler_binary.npz"), V=V, F=F.astype(np.int32))
+    R = os.path.join(REF, "175706", "Run-04")
+    # 466 lines of three bands; the reference reads 467 (observe_generator.cpp:
+    # 865-869), so its last row is the 0 a failed stream extraction stores
+    cc = np.loadtxt(os.path.join(R, "wise_colour_corrections.txt"), dtype=np.float64)
+    cc = np.vstack([cc, np.zeros((467 - len(cc), 3))]).T.copy()
+    V, F = formats.read_shape(os.path.join(R, "1996FG3_binary_lowres.txt"), 2052, 4096)
+    np.savez_compressed(os.path.join(HERE, "fg3_observe.npz"), colour=cc, V=V,
+                        F=F.astype(np.int32))
     props = {}
     with open(os.path.join(D, "doppler_binary_properties.txt")) as fh:
         for line in fh:
