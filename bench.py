@@ -487,6 +487,73 @@ def viewfactor_phase(ctx, cpu=True):
     return out
 
 
+def observe_phase(ctx, cpu=True):
+    """Observe flux synthesis at the reference's configuration
+    (175706/Run-04/observe_generator.cpp:18-35): the 4096-facet 1996 FG3 mesh
+    (a committed fixture), 9:2 revolutions over 5850 steps, the rough model
+    with the shipped 100-facet crater.  The inputs come from this library:
+    facet visibility from the geometry producer with the observer as the
+    source, instantaneous-equilibrium temperatures from its Sun illumination
+    (A = 0.01), crater temperatures and visible fractions from a fixed
+    sub-facet pattern.  GPU wall time of the call (9.6 GB of crater inputs
+    uploaded, included); the FP64 restatement on sampled steps."""
+    import math
+
+    from binarytpm_amd import thermal as th
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    d = np.load(os.path.join(gold, "fg3_observe.npz"))
+    c = np.load(os.path.join(gold, "new_crater2.npz"))
+    V, F, cc = d["V"], d["F"], d["colour"]
+    nf, ntau = len(F), 5850
+    hel = (1.160088, -1.9521 / 180.0 * math.pi, 198.3142 / 180.0 * math.pi)
+    obs = (0.583905, -3.882500 / 180.0 * math.pi, 257.492321 / 180.0 * math.pi)
+    rot = (-83.0 / 180.0 * math.pi, 266.0 / 180.0 * math.pi)
+    kw = dict(nv_primary=1026, rev_primary=9.0, rev_secondary=2.0)
+    sun = th.geometry_illumination(ctx, V, F, ntau, hel, rot, sun_facet=False, **kw)
+    seen = th.geometry_illumination(ctx, V, F, ntau, obs, rot, sun_facet=False, **kw)
+    t_eq = np.power(np.maximum(0.99 * sun.illum, 0.0) / (0.95 * 5.67e-8), 0.25)
+    st = np.maximum(t_eq, 100.0).astype(np.float32)
+    pat = (0.85 + 0.3 * (np.arange(100) % 7) / 6.0).astype(np.float32)
+    ct = (st[:, :, None] * pat[None, None, :]).astype(np.float32)
+    cvis = (seen.shadow[:, :, None] * (0.2 + 0.8 * (np.arange(100) % 5) / 4.0)).astype(np.float32)
+    crater = (c["V"], c["F"], cvis, ct)
+    th.observe(ctx, V, F[:64], 16, hel, obs, rot, seen.shadow[:64, :16], st[:64, :16], cc,
+               crater=(c["V"], c["F"], cvis[:64, :16], ct[:64, :16]), **kw)   # warm-up
+    t0 = time.perf_counter()
+    sm, ro = th.observe(ctx, V, F, ntau, hel, obs, rot, seen.shadow, st, cc, crater=crater, **kw)
+    gpu_s = time.perf_counter() - t0
+    out = {
+        "workload": (f"1996 FG3 mesh, {nf} facets x {ntau} steps (9:2 revolutions) x 100 crater "
+                     "sub-facets, 3 WISE bands, the reference's Sun / observer / pole"),
+        "gpu_s": gpu_s,
+        "subfacet_steps_per_s": nf * ntau * 100 / gpu_s,
+        "visible_fraction": float(seen.shadow.mean()),
+        "rough_flux_mean": [float(x) for x in ro.mean(axis=0)],
+    }
+    log(f"observe: {nf} x {ntau} x 100 in {gpu_s:.2f} s (crater inputs {2 * ct.nbytes / 1e9:.1f} GB)")
+    if cpu:
+        from oracle import oracle as O
+        sample = list(range(0, ntau, ntau // 24))[:24]
+        t0 = time.perf_counter()
+        worst = 0.0
+        for s in sample:
+            o_sm, o_ro = O.observe(V, F, ntau, hel, obs, rot, seen.shadow, st, cc, crater=crater,
+                                   steps=(s, 1), **kw)
+            for a, b in ((sm[s], o_sm[0]), (ro[s], o_ro[0])):
+                worst = max(worst, float((np.abs(a - b) / np.abs(b)).max()))
+        per_step = (time.perf_counter() - t0) / len(sample)
+        out["cpu_baseline"] = {
+            "value": per_step * ntau, "unit": "s per cycle", "cores": 1, "kind": "port",
+            "sample": f"FP64 restatement (oracle/observe.c, one core) on {len(sample)} of {ntau} "
+                      f"steps, {per_step * 1e3:.1f} ms per step, extrapolated to the cycle",
+        }
+        out["vs_cpu"] = per_step * ntau / gpu_s
+        out["max_rel_diff_on_sample"] = worst
+        log(f"observe: restatement {per_step * 1e3:.1f} ms per step ({per_step * ntau / gpu_s:.0f}x), "
+            f"max rel diff {worst:.1e} on {len(sample)} steps")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -506,6 +573,8 @@ def main():
                     help="skip the illumination geometry and shadow producer measurement")
     ap.add_argument("--no-viewfactors", action="store_true",
                     help="skip the view-factor producer measurement")
+    ap.add_argument("--no-observe", action="store_true",
+                    help="skip the Observe flux synthesis measurement")
     ap.add_argument("--producers-only", action="store_true",
                     help="measure only the illumination and view-factor producers (one JSON "
                          "line with their results; for profiling)")
@@ -535,7 +604,8 @@ def main():
         with th.Context(0) as ctx:
             line = {"valu_ceiling_wave_instr_per_s": valu_ceiling(ctx),
                     "illumination_producer": geometry_phase(ctx, cpu=not args.no_cpu),
-                    "viewfactor_producer": viewfactor_phase(ctx, cpu=not args.no_cpu)}
+                    "viewfactor_producer": viewfactor_phase(ctx, cpu=not args.no_cpu),
+                    "observe_fluxes": observe_phase(ctx, cpu=not args.no_cpu)}
         print(json.dumps(line), flush=True)
         return
     case, crater = build_workload(args.subdiv, args.gamma)
@@ -657,6 +727,8 @@ def main():
             line["illumination_producer"] = geometry_phase(ctx, cpu=not args.no_cpu)
         if not args.no_viewfactors:
             line["viewfactor_producer"] = viewfactor_phase(ctx, cpu=not args.no_cpu)
+        if not args.no_observe:
+            line["observe_fluxes"] = observe_phase(ctx, cpu=not args.no_cpu)
         line["roofline"]["valu_ceiling_wave_instr_per_s"] = valu_ceiling(ctx)
     print(json.dumps(line), flush=True)
 

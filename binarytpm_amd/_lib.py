@@ -173,6 +173,14 @@ PROTOTYPES = [
                                 _fp]),
     ("btpm_vf_free", None, [_vp]),
     ("btpm_crater_viewfactors", C.c_int, [_vp, C.POINTER(DrfIn), _fp]),
+    ("btpm_write_temperature_txt", C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.c_int,
+                                             C.POINTER(C.c_int), C.c_int, _fp, C.c_int, C.c_int]),
+    ("btpm_write_crater_dat", C.c_int, [C.c_char_p, _fp, C.c_int, C.POINTER(C.c_int), C.c_int,
+                                        _fp, C.c_int, C.c_int]),
+    ("btpm_solve_write_crater_dat", C.c_int, [_vp, C.c_char_p, _fp, C.c_int, C.POINTER(C.c_int),
+                                              C.c_int]),
+    ("btpm_stitch_crater_parts", C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_int, C.c_int,
+                                           C.c_int, C.c_int]),
     ("btpm_observe", C.c_int, [_vp, C.POINTER(ObserveIn), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]),
 ]

@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["conduct.hip", "aux.hip", "stepwise.hip", "geometry.hip", "viewfactor.hip", "observe.hip", "probe.hip", "host.hip", "compat.hip"]
+SOURCES = ["conduct.hip", "aux.hip", "stepwise.hip", "geometry.hip", "viewfactor.hip", "observe.hip", "io.hip", "probe.hip", "host.hip", "compat.hip"]
 LIB = os.path.join(HERE, "libbtpm.so")
 
 HIPCC_FLAGS = [

@@ -71,11 +71,10 @@ def run(args) -> int:
             T = s.surface_temps()
             lab = _ti_label(ti)
             out = os.path.join(args.out, os.path.basename(base))
-            formats.write_temperature_txt(f"{out}_temperature({lab}).txt", V, F, T)
+            th.write_temperature_txt(f"{out}_temperature({lab}).txt", V, F, T)
             formats.write_temperature_dat(f"{out}_temperature({lab}).dat", V, F, T)
             if crater is not None:
-                formats.write_roughness_dat(f"{out}_roughness_temperature({lab}).dat", Vc, Fc,
-                                            s.crater_temps())
+                s.write_crater_dat(f"{out}_roughness_temperature({lab}).dat", Vc, Fc)
             inf = s.info()
             print(f"TI {lab}: {inf.rotations} rotations (converged {inf.converged})"
                   + (f", crater {inf.crater_rotations} (converged {inf.crater_converged})"

@@ -82,6 +82,22 @@ extern "C" __global__ void btpm_unpack_crater(const float* __restrict__ in, floa
   }
 }
 
+// facets [c0/100, (c0+nc)/100) of the device layout (step blocks x columns x
+// 8 steps) in the crater temperature file's order [column][step]
+extern "C" __global__ void btpm_unpack_crater_rows(const float* __restrict__ in,
+                                                   float* __restrict__ out, int ncols, int c0,
+                                                   int nc, int ntau, int nb) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nc * nb) return;
+  const int b = (int)(i / nc);
+  const int c = (int)(i - (size_t)b * nc);
+  const float* p = in + ((size_t)b * ncols + c0 + c) * BTPM_BLK;
+  float* o = out + (size_t)c * ntau + (size_t)b * BTPM_BLK;
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k)
+    if (b * BTPM_BLK + k < ntau) o[k] = p[k];
+}
+
 // ---------------------------------------------------------------------------
 // initial temperatures
 // ---------------------------------------------------------------------------

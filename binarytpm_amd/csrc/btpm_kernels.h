@@ -86,6 +86,9 @@ extern "C" __global__ void btpm_geom_finish(const int* lit, const float* cosang,
                                             int nf, int nsteps, int s0, int ntau, float solar,
                                             int* shadow, float* illum, float* sun_out);
 
+extern "C" __global__ void btpm_unpack_crater_rows(const float* in, float* out, int ncols, int c0,
+                                                   int nc, int ntau, int nb);
+
 // viewfactor.hip
 extern "C" __global__ void btpm_vf_prep(const float* vtx, const int* fv, int nf, float* rec);
 extern "C" __global__ void btpm_vf_facing(const float* rec, int nf, const int* rowptr, int* count,

@@ -1674,6 +1674,72 @@ int btpm_solve_crater_temps(btpm_solve* s, float* out) {
   return fetch_series(s, s->crater.get(), out, 1);
 }
 
+// binary_writesurfacetemperature (T3905/main.cpp:2314-2341) straight from the
+// device: facet chunks unpacked into the file's [facet][sub-facet][step]
+// order, copied through two pinned buffers while the previous chunk is
+// written; host memory stays at two chunks of at most 128 MB.
+int btpm_solve_write_crater_dat(btpm_solve* s, const char* path, const float* cvtx, int ncv,
+                                const int* cfv, int ncf) {
+  if (!s || !path || !cvtx || !cfv) return fail(BTPM_ERR_ARG, "null argument");
+  if (ncf != BTPM_NS) return fail(BTPM_ERR_ARG, "the crater model has %d facets", BTPM_NS);
+  if (s->phase != 3 || !s->crater) return fail(BTPM_ERR_STATE, "no finished crater solve");
+  btpm_ctx* c = s->ctx;
+  CtxLock lk(c);
+  FILE* out = std::fopen(path, "wb");
+  if (!out) return fail(BTPM_ERR_ARG, "cannot open %s for writing", path);
+  int rc = BTPM_OK;
+  float* host[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    SetHost* S = s->crater.get();
+    const int nt = s->shape->p.ntau, nb = S->d.nb, nf = s->shape->nf, ncols = S->d.ncols;
+    const size_t per_facet = (size_t)BTPM_NS * nt;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)nf, (128u << 20) /
+                                                                          (per_facet * 4)));
+    DBuf dev[2];
+    for (int k = 0; k < 2; ++k) {
+      dev[k].alloc((size_t)chunk * per_facet * 4);
+      HIPCHK(hipHostMalloc((void**)&host[k], (size_t)chunk * per_facet * 4, hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&done[k], hipEventDisableTiming));
+    }
+    if (std::fwrite(cvtx, sizeof(float) * 3, (size_t)ncv, out) != (size_t)ncv ||
+        std::fwrite(cfv, sizeof(int) * 3, (size_t)ncf, out) != (size_t)ncf)
+      throw HipError{fail(BTPM_ERR_ARG, "write to %s failed", path)};
+    const int nchunks = (nf + chunk - 1) / chunk;
+    auto issue = [&](int k) {
+      const int f0 = k * chunk, nfc = std::min(chunk, nf - f0);
+      const int nc = nfc * BTPM_NS;
+      hipLaunchKernelGGL(btpm_unpack_crater_rows, dim3(grid1((size_t)nc * nb, 256)), dim3(256), 0,
+                         c->stream, S->T.as<float>(), dev[k & 1].as<float>(), ncols,
+                         f0 * BTPM_NS, nc, nt, nb);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(host[k & 1], dev[k & 1].p, (size_t)nc * nt * 4, hipMemcpyDeviceToHost,
+                            c->stream));
+      HIPCHK(hipEventRecord(done[k & 1], c->stream));
+    };
+    if (nchunks > 0) issue(0);
+    for (int k = 0; k < nchunks; ++k) {
+      HIPCHK(hipEventSynchronize(done[k & 1]));
+      if (k + 1 < nchunks) issue(k + 1);   // overlaps the write below
+      const size_t n = (size_t)std::min(chunk, nf - k * chunk) * per_facet;
+      if (std::fwrite(host[k & 1], sizeof(float), n, out) != n)
+        throw HipError{fail(BTPM_ERR_ARG, "write to %s failed", path)};
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (const HipError& e) {
+    rc = e.code;
+  } catch (const std::bad_alloc&) {
+    rc = fail(BTPM_ERR_NOMEM, "host allocation failed");
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (host[k]) (void)hipHostFree(host[k]);
+    if (done[k]) (void)hipEventDestroy(done[k]);
+  }
+  if (std::fclose(out) != 0 && rc == BTPM_OK) rc = fail(BTPM_ERR_ARG, "closing %s failed", path);
+  return rc;
+}
+
 static int fetch_profile(btpm_solve* s, SetHost* S, float* out, int crater) {
   btpm_ctx* c = s->ctx;
   CtxLock lk(c);

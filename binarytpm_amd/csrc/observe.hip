@@ -16,8 +16,8 @@
 //                  x area x visibility x cos(view angle) / d^2 in three bands),
 //                  summed over the wave, scaled by A_facet / S_crater; the
 //                  smooth facet term; both written per facet, band and step.
-//   btpm_obs_sum   one step per lane: the facet terms summed in facet order,
-//                  the colour temperature fit (100..566 K, band ratios; the
+//   btpm_obs_sum   64 steps per workgroup, 16 facet ranges summed in order
+//                  and combined in order; the colour temperature fit (100..566 K, band ratios; the
 //                  smooth fit keeps the reference's comma-operator terms,
 //                  :782, i.e. fits band 1 alone) and the colour corrections.
 //
@@ -25,7 +25,7 @@
 // rotation trigonometry, pole matrix, positions, Planck prefactors and the
 // colour-fit table come from the host (the same libm as the restatement), so
 // what differs from it is the device's exp / acos / cos (ulps) and the
-// crater sums' order across lanes: fluxes agree to ~1e-13 relative.
+// order of the crater and facet sums: fluxes agree to ~1e-15 relative.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -171,22 +171,37 @@ extern "C" __global__ __launch_bounds__(64) void btpm_obs_flux(
 }
 
 // bn[t][3] = the band intensities of a blackbody at tmin + t, divided by
-// their mean; colour[3][tcount]
-extern "C" __global__ __launch_bounds__(256) void btpm_obs_sum(
+// their mean; colour[3][tcount].  A workgroup per 64 steps (lane = step);
+// its 16 waves sum 16 consecutive facet ranges in facet order, the ranges'
+// sums are then combined in order.
+extern "C" __global__ __launch_bounds__(1024) void btpm_obs_sum(
     const double* __restrict__ terms, int nf, int nsteps, const double* __restrict__ bn,
     const double* __restrict__ colour, int tmin, int tcount, double* __restrict__ smooth,
     double* __restrict__ rough) {
-  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= nsteps) return;
-  double sm[3] = {0.0, 0.0, 0.0}, ro[3] = {0.0, 0.0, 0.0};
-  for (int f = 0; f < nf; ++f) {
-    const double* t = terms + (size_t)f * 6 * nsteps + sl;
+  __shared__ double part[16][6][64];
+  const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+  const int sl = blockIdx.x * 64 + lane;
+  const int per = (nf + 15) / 16;
+  const int f0 = seg * per, f1 = min(nf, f0 + per);
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (sl < nsteps) {
+    for (int f = f0; f < f1; ++f) {
+      const double* t = terms + (size_t)f * 6 * nsteps + sl;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      sm[i] = sm[i] + t[(size_t)i * nsteps];
-      ro[i] = ro[i] + t[(size_t)(3 + i) * nsteps];
+      for (int k = 0; k < 6; ++k) acc[k] = acc[k] + t[(size_t)k * nsteps];
     }
   }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) part[seg][k][lane] = acc[k];
+  __syncthreads();
+  if (seg != 0 || sl >= nsteps) return;
+  double sm[3] = {0.0, 0.0, 0.0}, ro[3] = {0.0, 0.0, 0.0};
+  for (int g = 0; g < 16; ++g)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      sm[i] = sm[i] + part[g][i][lane];
+      ro[i] = ro[i] + part[g][3 + i][lane];
+    }
   double cs_s = 100.0, cs_r = 100.0;
   int bt_s = tmin, bt_r = tmin;
   for (int k = 0; k < tcount; ++k) {
@@ -367,7 +382,7 @@ extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* sm
                        rough_on ? d_ct.as<float>() : (const float*)nullptr, d_cs.as<double4>(),
                        ncf, stot, rough_on, d_pl.as<double>(), in->eps, nf, ntau, s0, ns,
                        d_terms.as<double>());
-    hipLaunchKernelGGL(btpm_obs_sum, dim3(grid1(ns, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(btpm_obs_sum, dim3(grid1(ns, 64)), dim3(1024), 0, st,
                        d_terms.as<double>(), nf, ns, d_bn.as<double>(), d_col.as<double>(),
                        in->tmin, in->tcount, d_sm.as<double>() + 3 * (size_t)s0,
                        d_ro.as<double>() + 3 * (size_t)s0);

@@ -10,6 +10,7 @@ Arrays use the reference layouts: facet series [nf][ntau], crater series
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -231,6 +232,14 @@ class Solve:
         check(lib().btpm_solve_crater_temps(self._h, _ptr(out)))
         return out
 
+    def write_crater_dat(self, path, Vc, Fc):
+        """The crater temperature file (binary_writesurfacetemperature) straight
+        from device memory, streamed in facet chunks."""
+        cv = _f32(Vc)
+        cf = np.ascontiguousarray(Fc, dtype=np.int32)
+        check(lib().btpm_solve_write_crater_dat(self._h, os.fsencode(path), _ptr(cv), cv.shape[0],
+                                                _ptr(cf, _ip), cf.shape[0]))
+
     def profile(self) -> np.ndarray:
         out = np.empty((self.shape.nf, NX + 1), dtype=np.float32)
         check(lib().btpm_solve_profile(self._h, _ptr(out)))
@@ -415,6 +424,36 @@ def crater_viewfactors(ctx: Context, vtx, facets, rowptr, vf, dirs, crater_vtx, 
     out = np.empty((int(rp[-1]), q.ncf), np.float32)
     check(lib().btpm_crater_viewfactors(ctx.handle, C.byref(q), _ptr(out)))
     return out
+
+
+def write_temperature_txt(path, V, F, stemp, threads: int = 0):
+    """writeglobaltemperature's text file, formatted natively on `threads`
+    threads (0: all cores); byte-identical to formats.write_temperature_txt."""
+    v = np.ascontiguousarray(V, dtype=np.float64)
+    f = np.ascontiguousarray(F, dtype=np.int32)
+    t = _f32(stemp)
+    nf, ntau = t.shape
+    check(lib().btpm_write_temperature_txt(os.fsencode(path), v.ctypes.data_as(C.POINTER(C.c_double)),
+                                           v.shape[0], _ptr(f, _ip), f.shape[0], _ptr(t), ntau,
+                                           int(threads)))
+
+
+def write_crater_dat(path, Vc, Fc, temps):
+    """binary_writesurfacetemperature's file from host temperatures
+    [nf][ntau][ncf] (transposed a facet at a time)."""
+    cv = _f32(Vc)
+    cf = np.ascontiguousarray(Fc, dtype=np.int32)
+    t = _f32(temps)
+    nf, ntau, ncf = t.shape
+    check(lib().btpm_write_crater_dat(os.fsencode(path), _ptr(cv), cv.shape[0], _ptr(cf, _ip), ncf,
+                                      _ptr(t), nf, ntau))
+
+
+def stitch_crater_parts(out_path, parts, ncv: int, ncf: int, ntau: int):
+    """stitch_temperatures: join part files of consecutive facet ranges."""
+    arr = (C.c_char_p * len(parts))(*[os.fsencode(p) for p in parts])
+    check(lib().btpm_stitch_crater_parts(os.fsencode(out_path), arr, len(parts), int(ncv), int(ncf),
+                                         int(ntau)))
 
 
 AU_M = 149.6e9                          # observe_generator.cpp:9 (metres)

@@ -361,6 +361,28 @@ typedef struct {
  * VF vis_j cos_j / dir_z capped at 1.                                      */
 int btpm_crater_viewfactors(btpm_ctx *ctx, const btpm_drf_in *in, float *drf);
 
+/* ---- output files at scale ---------------------------------------------------
+ * The stage's writers (T3905/main.cpp:2290-2341; 175706/Run-04/thermal.cpp:
+ * 104-244, 3291-3381).  They return BTPM_ERR_ARG on an I/O failure.
+ * Text temperatures, writeglobaltemperature's layout: vertex lines, facet
+ * lines, then nf*ntau values one per line, facet-major, as the default
+ * ostream prints them (%g); formatted on `threads` threads (0: all cores). */
+int btpm_write_temperature_txt(const char *path, const double *vtx, int nv, const int *facets,
+                               int nf, const float *stemp, int ntau, int threads);
+/* binary_writesurfacetemperature: float[ncv][3], int[ncf][3], then
+ * float[nf][ncf][ntau], from host temps [nf][ntau][ncf]                     */
+int btpm_write_crater_dat(const char *path, const float *cvtx, int ncv, const int *cfv, int ncf,
+                          const float *temps, int nf, int ntau);
+/* the same file straight from a finished solve's device memory, streamed in
+ * facet chunks through pinned buffers (no [nf][ntau][100] host copy)         */
+int btpm_solve_write_crater_dat(btpm_solve *s, const char *path, const float *cvtx, int ncv,
+                                const int *cfv, int ncf);
+/* stitch_temperatures: part files (each a header plus whole facets of
+ * ncf*ntau floats, consecutive facet ranges in order) joined into one file
+ * with the last part's header, streamed                                      */
+int btpm_stitch_crater_parts(const char *out_path, const char *const *parts, int nparts, int ncv,
+                             int ncf, int ntau);
+
 /* ---- observed fluxes ---------------------------------------------------------
  * The Observe stage's flux synthesis (175706/Run-04/observe_generator.cpp:
  * asteroidgeometry :530-698, surfacegeometry :465-529, observe :699-836),
