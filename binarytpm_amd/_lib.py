@@ -77,6 +77,7 @@ class Geometry(C.Structure):
         ("equpos", C.POINTER(C.c_double)), ("facets", C.POINTER(C.c_int)),
         ("rev_primary", C.c_double), ("rev_secondary", C.c_double),
         ("pol", C.c_double * 3), ("rot", C.c_double * 2), ("au_km", C.c_double),
+        ("sun_scale", C.c_double),
     ]
 
 
@@ -110,6 +111,15 @@ class ObserveIn(C.Structure):
         ("crater_visible", C.POINTER(C.c_float)), ("crater_temp", C.POINTER(C.c_float)),
         ("wavelength_um", C.POINTER(C.c_double)), ("colour", C.POINTER(C.c_double)),
         ("tmin", C.c_int), ("tcount", C.c_int), ("eps", C.c_double),
+    ]
+
+
+class CraterSunIn(C.Structure):
+    _fields_ = [
+        ("nf", C.c_int), ("ntau", C.c_int), ("shadow", C.POINTER(C.c_int)),
+        ("sun", C.POINTER(C.c_float)), ("ncv", C.c_int), ("ncf", C.c_int),
+        ("crater_vtx", C.POINTER(C.c_float)), ("crater_facets", C.POINTER(C.c_int)),
+        ("m", C.c_int), ("crater_radius", C.c_float),
     ]
 
 
@@ -173,6 +183,7 @@ PROTOTYPES = [
                                 _fp]),
     ("btpm_vf_free", None, [_vp]),
     ("btpm_crater_viewfactors", C.c_int, [_vp, C.POINTER(DrfIn), _fp]),
+    ("btpm_crater_sunlight", C.c_int, [_vp, C.POINTER(CraterSunIn), _fp, _fp]),
     ("btpm_write_temperature_txt", C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.c_int,
                                              C.POINTER(C.c_int), C.c_int, _fp, C.c_int, C.c_int]),
     ("btpm_write_crater_dat", C.c_int, [C.c_char_p, _fp, C.c_int, C.POINTER(C.c_int), C.c_int,

@@ -80,7 +80,7 @@ extern "C" __global__ void btpm_status_kernel(const int* ids, const btpm_set_dev
 extern "C" __global__ void btpm_geom_facets(const double* equpos, const int* fv, int nf,
                                             int nv_primary, const double* trig,
                                             const double* frame, int s0, int nsteps, double* rec,
-                                            int* lit, float* cosang, float* sun);
+                                            int* lit, float* cosang, float* sun, double sun_scale);
 extern "C" __global__ void btpm_geom_shadow(const double* rec, int nf, int nsteps, int* lit);
 extern "C" __global__ void btpm_geom_finish(const int* lit, const float* cosang, const float* sun,
                                             int nf, int nsteps, int s0, int ntau, float solar,
@@ -106,6 +106,10 @@ extern "C" __global__ void btpm_vf_refine(const float* rec, const int* col, cons
 extern "C" __global__ void btpm_vf_finish(const int* rowptr, const float* vf, const float* vec,
                                           const float* rec, float* dir, float* total,
                                           float* photon);
+extern "C" __global__ void btpm_crater_sun(const int* lit_in, const float* sun, const float4* csub,
+                                           const float4* samp, const float* vtab, int ncf,
+                                           int nsamp, float radius, long long n, float* shadow,
+                                           float* cosang);
 extern "C" __global__ void btpm_drf_frames(const double* vtx, const int* fv, int nf, double* frame);
 extern "C" __global__ void btpm_drf_rows(const int* rowptr, int* row);
 extern "C" __global__ void btpm_drf(const double* frame, const int* row, const float* vfv,

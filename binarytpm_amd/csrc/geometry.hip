@@ -40,7 +40,7 @@ extern "C" __global__ __launch_bounds__(256) void btpm_geom_facets(
     const double* __restrict__ equpos, const int* __restrict__ fv, int nf, int nv_primary,
     const double* __restrict__ trig, const double* __restrict__ frame, int s0, int nsteps,
     double* __restrict__ rec, int* __restrict__ lit, float* __restrict__ cosang,
-    float* __restrict__ sun) {
+    float* __restrict__ sun, double sun_scale) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = blockIdx.y;
   if (f >= nf || sl >= nsteps) return;
@@ -88,9 +88,15 @@ extern "C" __global__ __launch_bounds__(256) void btpm_geom_facets(
   const size_t o = (size_t)sl * nf + f;
   lit[o] = ang > (kPi / 2.0) ? 0 : 1;
   cosang[o] = (float)cos(ang);
-  sun[3 * o] = (float)(iv0 / t);
-  sun[3 * o + 1] = (float)(iv1 / t);
-  sun[3 * o + 2] = (float)(iv2 / t);
+  if (sun_scale > 0.0) {  // thermal.cpp:1453-1455
+    sun[3 * o] = (float)(sun_scale * iv0 / t);
+    sun[3 * o + 1] = (float)(sun_scale * iv1 / t);
+    sun[3 * o + 2] = (float)(sun_scale * iv2 / t);
+  } else {
+    sun[3 * o] = (float)(iv0 / t);
+    sun[3 * o + 1] = (float)(iv1 / t);
+    sun[3 * o + 2] = (float)(iv2 / t);
+  }
   // record for the shadow tests
   double r2 = 0.0;
 #pragma unroll
@@ -326,7 +332,7 @@ extern "C" int btpm_geometry_illumination(btpm_ctx* ctx, const btpm_geometry* g,
     hipLaunchKernelGGL(btpm_geom_facets, dim3(grid1(nf, 256), ns), dim3(256), 0, st,
                        d_eq.as<double>(), d_fv.as<int>(), nf, g->nv_primary, d_trig.as<double>(),
                        d_frame.as<double>(), s0, ns, d_rec.as<double>(), d_lit.as<int>(),
-                       d_cos.as<float>(), d_sunc.as<float>());
+                       d_cos.as<float>(), d_sunc.as<float>(), g->sun_scale);
     hipLaunchKernelGGL(btpm_geom_shadow, dim3(grid1(nf, 256), ns), dim3(256), 0, st,
                        d_rec.as<double>(), nf, ns, d_lit.as<int>());
     hipLaunchKernelGGL(btpm_geom_finish, dim3(grid1(nf, 64), grid1(ns, 64)), dim3(256), 0, st,

@@ -529,6 +529,81 @@ extern "C" __global__ __launch_bounds__(256) void btpm_drf(
   }
 }
 
+// Crater sunlight (T3905/main.cpp:863-886, surfacegeometry; T3905/
+// GPU_code.cu:152-210, sfcpartialshadow): a workgroup per 64 (facet, step)
+// entries, lane = entry, wave w taking crater sub-facets j = w, w+4, ...
+// with the rim samples through scalar loads (as btpm_drf).  Per entry the
+// Sun vector sv in the facet frame; cos_j = (sv . n_j)/|sv|; the sunlit
+// fraction starts at 1 (0 when the facet is dark or cos_j <= 0) and loses
+// 1/(m*m) per sample below the rim along sv (normalised by its horizontal
+// length) while above 0 -- a host table gives it after c hits.
+extern "C" __global__ __launch_bounds__(256) void btpm_crater_sun(
+    const int* __restrict__ lit_in, const float* __restrict__ sun,
+    const float4* __restrict__ csub, const float4* __restrict__ samp,
+    const float* __restrict__ vtab, int ncf, int nsamp, float radius, long long n,
+    float* __restrict__ shadow, float* __restrict__ cosang) {
+  extern __shared__ float sm[];
+  float* tsh = sm;                         // [64][ncf + 1]
+  float* tco = sm + 64 * (ncf + 1);        // [64][ncf + 1]
+  float* svt = sm + 128 * (ncf + 1);       // [nsamp + 1]
+  for (int i = threadIdx.x; i <= nsamp; i += blockDim.x) svt[i] = vtab[i];
+  const long long q0 = (long long)blockIdx.x * 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const long long q = q0 + lane;
+  const bool act = q < n;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 1.0f, t = 1.0f, w0 = 0.0f, w1 = 0.0f, a = 0.0f;
+  int lit = 0;
+  if (act) {
+    s0 = sun[3 * q], s1 = sun[3 * q + 1], s2 = sun[3 * q + 2];
+    lit = lit_in[q];
+    t = sqrtf((s0 * s0) + (s1 * s1) + (s2 * s2));
+    if (lit == 1) {
+      const float h = sqrtf((s0 * s0) + (s1 * s1));
+      w0 = s0 / h;
+      w1 = s1 / h;
+      const float w2 = s2 / h;
+      a = fabsf(w2) / sqrtf((w0 * w0) + (w1 * w1));
+    }
+  }
+  const float nw1 = -1.0f * w1;
+  const float rr = radius * radius;
+  __syncthreads();
+  for (int j = wave; j < ncf; j += 4) {
+    const float4 nj = csub[j];
+    const float ill = ((s0 * nj.x) + (s1 * nj.y) + (s2 * nj.z)) / t;
+    float sh = 0.0f;
+    if (act && lit == 1 && !(ill <= 0.0f)) {
+      const float4* sp = samp + (size_t)__builtin_amdgcn_readfirstlane(j) * nsamp;
+      auto below = [&](const float4 m) -> int {
+        const float tc1 = (nw1 * m.x) + (w0 * m.y);
+        const float sr = sqrtf(rr - (tc1 * tc1));
+        const float tc0 = (w0 * m.x) + (w1 * m.y) - sr;
+        return (m.z <= (a * tc0)) ? 1 : 0;
+      };
+      int hits = 0, p = 0;
+      for (; p + 4 <= nsamp; p += 4) {
+        const int pu = __builtin_amdgcn_readfirstlane(p);
+        const float4 m0 = sp[pu], m1 = sp[pu + 1], m2 = sp[pu + 2], m3 = sp[pu + 3];
+        hits += below(m0) + below(m1) + below(m2) + below(m3);
+      }
+      for (; p < nsamp; ++p) hits += below(sp[__builtin_amdgcn_readfirstlane(p)]);
+      sh = svt[hits];
+    } else if (act && lit != 0 && lit != 1 && !(ill <= 0.0f)) {
+      sh = 1.0f;   // a flag other than 0 or 1: lit, no partial shadow (as the reference)
+    }
+    tsh[lane * (ncf + 1) + j] = sh;
+    tco[lane * (ncf + 1) + j] = ill;
+  }
+  __syncthreads();
+  const long long rows = min(64ll, n - q0);
+  for (int i = threadIdx.x; i < rows * ncf; i += blockDim.x) {
+    const int e = i / ncf, j = i - e * ncf;
+    shadow[q0 * ncf + i] = tsh[e * (ncf + 1) + j];
+    cosang[q0 * ncf + i] = tco[e * (ncf + 1) + j];
+  }
+}
+
 // ===========================================================================
 #include <algorithm>
 #include <climits>
@@ -550,6 +625,68 @@ struct btpm_vf {
 };
 
 namespace {
+// the crater model's sub-facet unit normals, its m*m rim-visibility samples
+// per sub-facet and the visibility after c shadowed samples
+void crater_samples(const float* cvtx, const int* cfv, int ncf, int m, std::vector<float4>& csub,
+                    std::vector<float4>& samp, std::vector<float>& vtab) {
+  const int nsamp = m * m;
+  // crater sub-facets (T3905/main.cpp:819-860): unit normal, and the
+  // rim-visibility samples vertex 2 + mul0 trivector0 + mul1 trivector1
+  // with the multipliers of :1101-1114
+  std::vector<float> mul(2 * (size_t)nsamp);
+  for (int k = 1; k != m + 1; ++k)
+    for (int l = 1; l != k + 1; ++l) {
+      const int p = ((k - 1) * (k - 1)) + (2 * (l - 1));
+      mul[2 * p] = (float)k - (1.0f / 3.0f);
+      mul[2 * p + 1] = (float)l - (2.0f / 3.0f);
+    }
+  for (int k = 2; k != m + 1; ++k)
+    for (int l = 1; l != k; ++l) {
+      const int p = ((k - 1) * (k - 1)) + (2 * (l - 1)) + 1;
+      mul[2 * p] = (float)k - (2.0f / 3.0f);
+      mul[2 * p + 1] = (float)l - (1.0f / 3.0f);
+    }
+  csub.assign(ncf, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  samp.assign((size_t)ncf * nsamp, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  for (int j = 0; j < ncf; ++j) {
+    const float* p0 = cvtx + 3 * (size_t)cfv[3 * j];
+    const float* p1 = cvtx + 3 * (size_t)cfv[3 * j + 1];
+    const float* p2 = cvtx + 3 * (size_t)cfv[3 * j + 2];
+    float a[3], b[3], n[3], t0[3], t1[3];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = p1[k] - p0[k];
+      b[k] = p2[k] - p0[k];
+    }
+    n[0] = (a[1] * b[2]) - (a[2] * b[1]);
+    n[1] = (a[2] * b[0]) - (a[0] * b[2]);
+    n[2] = (a[0] * b[1]) - (a[1] * b[0]);
+    const float l = std::sqrt((n[0] * n[0]) + (n[1] * n[1]) + (n[2] * n[2]));
+    csub[j] = make_float4(n[0] / l, n[1] / l, n[2] / l, 0.0f);
+    for (int k = 0; k < 3; ++k) {
+      t0[k] = -1.0f * b[k] / (float)m;
+      t1[k] = a[k] / (float)m;
+    }
+    for (int p = 0; p < nsamp; ++p) {
+      float mp[3];
+      for (int k = 0; k < 3; ++k)
+        mp[k] = p2[k] + (mul[2 * p] * t0[k]) + (mul[2 * p + 1] * t1[k]);
+      samp[(size_t)j * nsamp + p] = make_float4(mp[0], mp[1], mp[2], 0.0f);
+    }
+  }
+  // visibility after c shadowed samples (T3905/main.cpp:1165-1166)
+  vtab.assign(nsamp + 1, 0.0f);
+  {
+    const float step = 1.0f / (float)(m * m);
+    float vis = 1.0f;
+    vtab[0] = vis;
+    for (int c = 1; c <= nsamp; ++c) {
+      if (vis != 0.0f) vis = vis - step;
+      if (vis < 0.0f) vis = 0.0f;
+      vtab[c] = vis;
+    }
+  }
+}
+
 // exclusive scan of per-row counts into a CSR row pointer; false on overflow
 bool scan_rows(const std::vector<int>& cnt, std::vector<int>& rp) {
   rp.assign(cnt.size() + 1, 0);
@@ -735,60 +872,9 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int ncf = in->ncf, m = in->m, nsamp = m * m;
-  // crater sub-facets (T3905/main.cpp:819-860): unit normal, and the
-  // rim-visibility samples vertex 2 + mul0 trivector0 + mul1 trivector1
-  // with the multipliers of :1101-1114
-  std::vector<float> mul(2 * (size_t)nsamp);
-  for (int k = 1; k != m + 1; ++k)
-    for (int l = 1; l != k + 1; ++l) {
-      const int p = ((k - 1) * (k - 1)) + (2 * (l - 1));
-      mul[2 * p] = (float)k - (1.0f / 3.0f);
-      mul[2 * p + 1] = (float)l - (2.0f / 3.0f);
-    }
-  for (int k = 2; k != m + 1; ++k)
-    for (int l = 1; l != k; ++l) {
-      const int p = ((k - 1) * (k - 1)) + (2 * (l - 1)) + 1;
-      mul[2 * p] = (float)k - (2.0f / 3.0f);
-      mul[2 * p + 1] = (float)l - (1.0f / 3.0f);
-    }
-  std::vector<float4> csub(ncf), samp((size_t)ncf * nsamp);
-  for (int j = 0; j < ncf; ++j) {
-    const float* p0 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j];
-    const float* p1 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 1];
-    const float* p2 = in->crater_vtx + 3 * (size_t)in->crater_facets[3 * j + 2];
-    float a[3], b[3], n[3], t0[3], t1[3];
-    for (int k = 0; k < 3; ++k) {
-      a[k] = p1[k] - p0[k];
-      b[k] = p2[k] - p0[k];
-    }
-    n[0] = (a[1] * b[2]) - (a[2] * b[1]);
-    n[1] = (a[2] * b[0]) - (a[0] * b[2]);
-    n[2] = (a[0] * b[1]) - (a[1] * b[0]);
-    const float l = std::sqrt((n[0] * n[0]) + (n[1] * n[1]) + (n[2] * n[2]));
-    csub[j] = make_float4(n[0] / l, n[1] / l, n[2] / l, 0.0f);
-    for (int k = 0; k < 3; ++k) {
-      t0[k] = -1.0f * b[k] / (float)m;
-      t1[k] = a[k] / (float)m;
-    }
-    for (int p = 0; p < nsamp; ++p) {
-      float mp[3];
-      for (int k = 0; k < 3; ++k)
-        mp[k] = p2[k] + (mul[2 * p] * t0[k]) + (mul[2 * p + 1] * t1[k]);
-      samp[(size_t)j * nsamp + p] = make_float4(mp[0], mp[1], mp[2], 0.0f);
-    }
-  }
-  // visibility after c shadowed samples (T3905/main.cpp:1165-1166)
-  std::vector<float> vtab(nsamp + 1);
-  {
-    const float step = 1.0f / (float)(m * m);
-    float vis = 1.0f;
-    vtab[0] = vis;
-    for (int c = 1; c <= nsamp; ++c) {
-      if (vis != 0.0f) vis = vis - step;
-      if (vis < 0.0f) vis = 0.0f;
-      vtab[c] = vis;
-    }
-  }
+  std::vector<float4> csub, samp;
+  std::vector<float> vtab;
+  crater_samples(in->crater_vtx, in->crater_facets, ncf, m, csub, samp, vtab);
   DBuf d_vtx, d_fv, d_rp, d_vf, d_dir, d_cs, d_samp, d_vt, d_frame, d_row, d_out;
   upload(d_vtx, in->vtx, 3 * (size_t)in->nv, st);
   upload(d_fv, in->facets, 3 * (size_t)in->nf, st);
@@ -813,5 +899,61 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(drf, d_out.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  API_END
+}
+
+extern "C" int btpm_crater_sunlight(btpm_ctx* ctx, const btpm_crater_sun_in* in, float* shadow,
+                                    float* cosang) {
+  if (!ctx || !in || !shadow || !cosang || !in->shadow || !in->sun || !in->crater_vtx ||
+      !in->crater_facets)
+    return fail(BTPM_ERR_ARG, "null argument");
+  if (in->nf < 1 || in->ntau < 1 || in->ncv < 3 || in->ncf < 1 || in->ncf > 128 || in->m < 1 ||
+      in->m > 64)
+    return fail(BTPM_ERR_ARG, "bad sizes: nf %d ntau %d ncv %d ncf %d m %d", in->nf, in->ntau,
+                in->ncv, in->ncf, in->m);
+  for (int i = 0; i < 3 * in->ncf; ++i)
+    if (in->crater_facets[i] < 0 || in->crater_facets[i] >= in->ncv)
+      return fail(BTPM_ERR_ARG, "crater facet %d: vertex %d out of range", i / 3,
+                  in->crater_facets[i]);
+  CtxLock lk(ctx);
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int ncf = in->ncf, m = in->m, nsamp = m * m;
+  std::vector<float4> csub, samp;
+  std::vector<float> vtab;
+  crater_samples(in->crater_vtx, in->crater_facets, ncf, m, csub, samp, vtab);
+  DBuf d_cs, d_samp, d_vt;
+  upload(d_cs, csub.data(), csub.size(), st);
+  upload(d_samp, samp.data(), samp.size(), st);
+  upload(d_vt, vtab.data(), vtab.size(), st);
+  const size_t shm = (128 * ((size_t)ncf + 1) + (size_t)nsamp + 1) * sizeof(float);
+  // facet-step chunks of at most 2^22 entries (the [entries][ncf] outputs of
+  // a chunk: 1.6 GB for 100 sub-facets)
+  const long long total = (long long)in->nf * in->ntau;
+  const long long chunk = 1ll << 22;
+  DBuf d_lit, d_sun, d_sh, d_co;
+  const long long first = std::min(total, chunk);
+  d_lit.alloc((size_t)first * sizeof(int));
+  d_sun.alloc((size_t)first * 3 * sizeof(float));
+  d_sh.alloc((size_t)first * ncf * sizeof(float));
+  d_co.alloc((size_t)first * ncf * sizeof(float));
+  for (long long q0 = 0; q0 < total; q0 += chunk) {
+    const long long n = std::min(chunk, total - q0);
+    HIPCHK(hipMemcpyAsync(d_lit.p, in->shadow + q0, (size_t)n * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_sun.p, in->sun + 3 * q0, (size_t)n * 3 * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(btpm_crater_sun, dim3((unsigned)((n + 63) / 64)), dim3(256), shm, st,
+                       d_lit.as<int>(), d_sun.as<float>(), d_cs.as<float4>(), d_samp.as<float4>(),
+                       d_vt.as<float>(), ncf, nsamp, in->crater_radius, n, d_sh.as<float>(),
+                       d_co.as<float>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(shadow + q0 * ncf, d_sh.p, (size_t)n * ncf * sizeof(float),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cosang + q0 * ncf, d_co.p, (size_t)n * ncf * sizeof(float),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   API_END
 }

@@ -16,8 +16,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, DrfIn, Geometry, ObserveIn, Params,
-                   SolveInfo, VfMesh, check, lib)
+from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, CraterSunIn, DrfIn, Geometry,
+                   ObserveIn, Params, SolveInfo, VfMesh, check, lib)
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -323,7 +323,7 @@ class Illumination:
 
 def geometry_illumination(ctx: Context, equpos, facets, ntau: int, pol, rot, nv_primary=None,
                           rev_primary=1.0, rev_secondary=1.0, au_km=AU_KM,
-                          sun_facet=True) -> Illumination:
+                          sun_facet=True, sun_scale=0.0) -> Illumination:
     """Direct illumination and shadows of a (binary) shape over one cycle on
     the GPU: vertices equpos [nv][3] (km, each body's equatorial frame, the
     first nv_primary on the primary), facets [nf][3]; pol = (r AU, ecliptic
@@ -340,6 +340,7 @@ def geometry_illumination(ctx: Context, equpos, facets, ntau: int, pol, rot, nv_
     g.pol[:] = [float(x) for x in pol]
     g.rot[:] = [float(x) for x in rot]
     g.au_km = float(au_km)
+    g.sun_scale = float(sun_scale)
     shadow = np.empty((nf, ntau), np.int32)
     illum = np.empty((nf, ntau), np.float32)
     sun = np.empty((nf, ntau, 3), np.float32) if sun_facet else None
@@ -518,6 +519,28 @@ def probe_valu(ctx: Context, op: str = "fma", iters: int = 8192) -> float:
     r = C.c_double()
     check(lib().btpm_probe_valu(ctx.handle, code, int(iters), C.byref(r)))
     return float(r.value)
+
+
+def crater_sunlight(ctx: Context, shadow, sun, crater_vtx, crater_facets, m: int = 10,
+                    crater_radius: float = 10.0):
+    """Crater sunlight (the reference's surfacegeometry and sfcpartialshadow):
+    facet lit flags [nf][ntau] and Sun vectors in the facet frames
+    [nf][ntau][3] -> (sunlit fraction, cosine), each [nf][ntau][ncf]."""
+    lit = np.ascontiguousarray(shadow, dtype=np.int32)
+    sv = _f32(sun)
+    cv = _f32(crater_vtx)
+    cf = np.ascontiguousarray(crater_facets, dtype=np.int32)
+    nf, ntau = lit.shape
+    if sv.shape != (nf, ntau, 3):
+        raise ValueError("sun must be [nf][ntau][3]")
+    q = CraterSunIn()
+    q.nf, q.ntau, q.shadow, q.sun = nf, ntau, _ptr(lit, _ip), _ptr(sv)
+    q.ncv, q.ncf, q.crater_vtx, q.crater_facets = cv.shape[0], cf.shape[0], _ptr(cv), _ptr(cf, _ip)
+    q.m, q.crater_radius = int(m), float(crater_radius)
+    sh = np.empty((nf, ntau, cf.shape[0]), np.float32)
+    co = np.empty((nf, ntau, cf.shape[0]), np.float32)
+    check(lib().btpm_crater_sunlight(ctx.handle, C.byref(q), _ptr(sh), _ptr(co)))
+    return sh, co
 
 
 def crater_contract(ctx: Context, vf, e) -> np.ndarray:

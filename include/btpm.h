@@ -191,6 +191,9 @@ typedef struct {
     double pol[3];
     double rot[2];
     double au_km;
+    double sun_scale;    /* 0: sun_facet holds unit directions; > 0: sun_scale
+                            times the direction, as the reference's
+                            sfcillumvector (10000, thermal.cpp:1453-1455)   */
 } btpm_geometry;
 
 /* ---- library ------------------------------------------------------------ */
@@ -360,6 +363,27 @@ typedef struct {
 /* drf [nnz][ncf] (host memory): per list entry and crater sub-facet j,
  * VF vis_j cos_j / dir_z capped at 1.                                      */
 int btpm_crater_viewfactors(btpm_ctx *ctx, const btpm_drf_in *in, float *drf);
+
+/* Crater sunlight: replaces the crater part of surfacegeometry
+ * (3905/Thermal/NominalPole/main.cpp:863-886) and the sfcpartialshadow kernel
+ * (GPU_code.cu:152-210): per facet and step the cosine of the Sun on each
+ * crater sub-facet and its sunlit fraction from m*m rim samples.  Outputs are
+ * btpm_crater.cosang and .shadow.                                           */
+typedef struct {
+    int nf, ntau;
+    const int *shadow;           /* [nf][ntau] facet lit flags (1 lit, 0 dark)      */
+    const float *sun;            /* [nf][ntau][3] Sun vector in the facet frame, the
+                                    reference's sfcillumvector (geometry producer
+                                    with sun_scale 10000)                          */
+    int ncv, ncf;                /* ncf at most 128                                 */
+    const float *crater_vtx;
+    const int *crater_facets;
+    int m;                       /* rim samples per crater facet edge (10), <= 64   */
+    float crater_radius;         /* 10                                              */
+} btpm_crater_sun_in;
+/* shadow, cosang: [nf][ntau][ncf] (host memory)                             */
+int btpm_crater_sunlight(btpm_ctx *ctx, const btpm_crater_sun_in *in, float *shadow,
+                         float *cosang);
 
 /* ---- output files at scale ---------------------------------------------------
  * The stage's writers (T3905/main.cpp:2290-2341; 175706/Run-04/thermal.cpp:

@@ -195,6 +195,7 @@ typedef struct {
     double pol[3], rot[2];
     double au_km;
     int s_begin, s_count;    /* steps computed (outputs hold s_count steps) */
+    double sun_scale;        /* 0: unit sun_facet; > 0: scaled (reference 10000) */
 } orc_geometry_in;
 void orc_geometry(const orc_geometry_in *in, int *shadow, double *cosang, float *illum,
                   float *sun_facet, float *solar_out);
@@ -246,6 +247,24 @@ typedef struct {
     float crater_radius;
 } orc_drf_in;
 void orc_crater_drf(const orc_drf_in *in, float *drf);
+
+/* Crater sunlight (viewfactor.c): per facet and step the cosine of the Sun
+ * on every crater sub-facet (surfacegeometry, T3905/main.cpp:863-886) and
+ * its sunlit fraction from the m*m rim samples (sfcpartialshadow,
+ * T3905/GPU_code.cu:152-210).  shadow_in [nf][ntau] facet lit flags, sun
+ * [nf][ntau][3] the Sun vector in the facet frame (the reference's
+ * sfcillumvector); outputs [nf][ntau][ncf].                                */
+typedef struct {
+    int nf, ntau;
+    const int *shadow_in;
+    const float *sun;
+    int ncv, ncf;
+    const float *cvtx;
+    const int *cfv;
+    int m;
+    float crater_radius;
+} orc_crater_sun_in;
+void orc_crater_sun(const orc_crater_sun_in *in, float *shadow, float *cosang);
 
 /* Observe flux synthesis (observe.c, FP64): the shape (equpos [nv][3] in
  * each body's frame, km; the first nv_primary on the primary; fv [nf][3]),

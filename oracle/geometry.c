@@ -146,7 +146,11 @@ void orc_geometry(const orc_geometry_in *in, int *shadow, double *cosang, float 
             const size_t o = (size_t)f * ns + si;
             shadow[o] = ang > (PI / 2.0) ? 0 : 1;
             if (cosang) cosang[o] = cos(ang);
-            if (sun_facet) {  /* thermal.cpp:1453-1455, scaled to unit length */
+            if (sun_facet && in->sun_scale > 0.0) {  /* thermal.cpp:1453-1455 */
+                sun_facet[3 * o] = (float)(in->sun_scale * iv0 / t);
+                sun_facet[3 * o + 1] = (float)(in->sun_scale * iv1 / t);
+                sun_facet[3 * o + 2] = (float)(in->sun_scale * iv2 / t);
+            } else if (sun_facet) {  /* the same, scaled to unit length */
                 sun_facet[3 * o] = (float)(iv0 / t);
                 sun_facet[3 * o + 1] = (float)(iv1 / t);
                 sun_facet[3 * o + 2] = (float)(iv2 / t);

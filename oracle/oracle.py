@@ -47,7 +47,7 @@ class OrcGeometryIn(C.Structure):
         ("equpos", C.POINTER(C.c_double)), ("fv", C.POINTER(C.c_int)),
         ("rev_primary", C.c_double), ("rev_secondary", C.c_double),
         ("pol", C.c_double * 3), ("rot", C.c_double * 2), ("au_km", C.c_double),
-        ("s_begin", C.c_int), ("s_count", C.c_int),
+        ("s_begin", C.c_int), ("s_count", C.c_int), ("sun_scale", C.c_double),
     ]
 
 
@@ -335,7 +335,7 @@ class Geometry:
 
 
 def geometry(equpos, fv, ntau, pol, rot, nv_primary=None, rev_primary=1.0, rev_secondary=1.0,
-             au_km=AU_KM, steps=None):
+             au_km=AU_KM, steps=None, sun_scale=0.0):
     """Illumination geometry and shadows (geometry.c): the shape's vertices
     [nv][3] (km) and facets [nf][3], the first nv_primary vertices on the
     primary; pol = (r AU, ecliptic lat, lon), rot = (pole lat, lon), radians.
@@ -353,6 +353,7 @@ def geometry(equpos, fv, ntau, pol, rot, nv_primary=None, rev_primary=1.0, rev_s
     g.rot[:] = [float(x) for x in rot]
     g.au_km = float(au_km)
     g.s_begin, g.s_count = (0, int(ntau)) if steps is None else (int(steps[0]), int(steps[1]))
+    g.sun_scale = float(sun_scale)
     ns = g.s_count
     shadow = np.zeros((nf, ns), np.int32)
     cosang = np.zeros((nf, ns), np.float64)
@@ -522,3 +523,28 @@ def observe(equpos, fv, ntau, hel, obs, rot, visible, stemp, colour, nv_primary=
     ro = np.zeros((q.s_count, len(wl)), np.float64)
     lib().orc_observe(C.byref(q), _ptr(sm, _dp), _ptr(ro, _dp))
     return sm, ro
+
+
+class OrcCraterSunIn(C.Structure):
+    _fields_ = [
+        ("nf", C.c_int), ("ntau", C.c_int), ("shadow_in", _ip), ("sun", _fp), ("ncv", C.c_int),
+        ("ncf", C.c_int), ("cvtx", _fp), ("cfv", _ip), ("m", C.c_int),
+        ("crater_radius", C.c_float),
+    ]
+
+
+def crater_sun(shadow, sun, cvtx, cfv, m=10, crater_radius=10.0):
+    """(sunlit fraction, cosine) [nf][ntau][ncf] of the crater sub-facets
+    (viewfactor.c orc_crater_sun)."""
+    lit = _i(shadow)
+    sv = _f(sun)
+    cv, cf = _f(cvtx), _i(cfv)
+    nf, ntau = lit.shape
+    q = OrcCraterSunIn()
+    q.nf, q.ntau, q.shadow_in, q.sun = nf, ntau, _ptr(lit, _ip), _ptr(sv, _fp)
+    q.ncv, q.ncf, q.cvtx, q.cfv = cv.shape[0], cf.shape[0], _ptr(cv, _fp), _ptr(cf, _ip)
+    q.m, q.crater_radius = int(m), float(crater_radius)
+    sh = np.zeros((nf, ntau, cf.shape[0]), np.float32)
+    co = np.zeros((nf, ntau, cf.shape[0]), np.float32)
+    lib().orc_crater_sun(C.byref(q), _ptr(sh, _fp), _ptr(co, _fp))
+    return sh, co

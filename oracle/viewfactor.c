@@ -35,6 +35,10 @@
  *   calculate_surface_global_viewfactors :1054-1181
  *                               drf[k][j] = VF_k vis_j cos_j / dir_z with a
  *                               m*m sample rim visibility count, capped at 1
+ * and the crater sunlight of the same crater model: surfacegeometry
+ * (T3905/main.cpp:863-886) and sfcpartialshadow (T3905/GPU_code.cu:152-210),
+ * restated as written (the reference compiles the latter with nvcc, whose
+ * default contraction into fused multiply-adds would round differently).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -455,5 +459,83 @@ void orc_crater_drf(const orc_drf_in *in, float *drf)
     free(mul);
     free(cv2);
     free(ctv);
+    free(cn);
+}
+
+/* T3905/main.cpp:863-886 and T3905/GPU_code.cu:152-210 */
+void orc_crater_sun(const orc_crater_sun_in *in, float *shadow, float *cosang)
+{
+    const int m = in->m, ncf = in->ncf, ns = m * m;
+    float *cn = (float *)malloc(sizeof(float) * 3 * (size_t)ncf);
+    float *mp = (float *)malloc(sizeof(float) * 3 * (size_t)ncf * ns);
+    float *mul = (float *)malloc(sizeof(float) * 2 * (size_t)ns);
+    for (int k = 1; k != m + 1; ++k)
+        for (int l = 1; l != k + 1; ++l) {
+            const int p = ((k - 1) * (k - 1)) + (2 * (l - 1));
+            mul[2 * p] = (float)k - (1.0f / 3.0f);
+            mul[2 * p + 1] = (float)l - (2.0f / 3.0f);
+        }
+    for (int k = 2; k != m + 1; ++k)
+        for (int l = 1; l != k; ++l) {
+            const int p = ((k - 1) * (k - 1)) + (2 * (l - 1)) + 1;
+            mul[2 * p] = (float)k - (2.0f / 3.0f);
+            mul[2 * p + 1] = (float)l - (1.0f / 3.0f);
+        }
+    for (int j = 0; j < ncf; ++j) {
+        const float *p0 = in->cvtx + 3 * (size_t)in->cfv[3 * j];
+        const float *p1 = in->cvtx + 3 * (size_t)in->cfv[3 * j + 1];
+        const float *p2 = in->cvtx + 3 * (size_t)in->cfv[3 * j + 2];
+        float a[3], b[3], nn[3], t0[3], t1[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = p1[k] - p0[k];
+            b[k] = p2[k] - p0[k];
+        }
+        nn[0] = (a[1] * b[2]) - (a[2] * b[1]);
+        nn[1] = (a[2] * b[0]) - (a[0] * b[2]);
+        nn[2] = (a[0] * b[1]) - (a[1] * b[0]);
+        const float l = sqrtf((nn[0] * nn[0]) + (nn[1] * nn[1]) + (nn[2] * nn[2]));
+        for (int k = 0; k < 3; ++k) {
+            cn[3 * j + k] = nn[k] / l;
+            t0[k] = -1.0f * b[k] / (float)m;
+            t1[k] = a[k] / (float)m;
+        }
+        for (int p = 0; p < ns; ++p)
+            for (int k = 0; k < 3; ++k)
+                mp[(3 * (size_t)j * ns) + 3 * p + k] =
+                    p2[k] + (mul[2 * p] * t0[k]) + (mul[2 * p + 1] * t1[k]);
+    }
+    const float R = in->crater_radius, step = 1.0f / (float)(m * m);
+#pragma omp parallel for schedule(static)
+    for (long long q = 0; q < (long long)in->nf * in->ntau; ++q) {
+        const float *sv = in->sun + 3 * (size_t)q;
+        const int lit = in->shadow_in[q];
+        const float t = sqrtf((sv[0] * sv[0]) + (sv[1] * sv[1]) + (sv[2] * sv[2]));
+        float w[3], a = 0.0f;
+        if (lit == 1) {
+            const float s = sqrtf((sv[0] * sv[0]) + (sv[1] * sv[1]));
+            for (int k = 0; k < 3; ++k) w[k] = sv[k] / s;
+            a = fabsf(w[2]) / sqrtf((w[0] * w[0]) + (w[1] * w[1]));
+        }
+        for (int j = 0; j < ncf; ++j) {
+            const size_t o = (size_t)q * ncf + j;
+            const float ill = ((sv[0] * cn[3 * j]) + (sv[1] * cn[3 * j + 1]) + (sv[2] * cn[3 * j + 2])) / t;
+            float sh = 1.0f;
+            if (lit == 0) sh = 0.0f;
+            if (ill <= 0.0f) sh = 0.0f;
+            if (lit == 1)
+                for (int p = 0; p != ns; ++p) {
+                    const float *x = mp + (3 * (size_t)j * ns) + 3 * p;
+                    const float tc1 = (-1.0f * w[1] * x[0]) + (w[0] * x[1]);
+                    const float sr = sqrtf((R * R) - (tc1 * tc1));
+                    const float tc0 = (w[0] * x[0]) + (w[1] * x[1]) - sr;
+                    if (x[2] <= (a * tc0) && sh != 0.0f) sh = sh - step;
+                    if (sh < 0.0f) sh = 0.0f;
+                }
+            cosang[o] = ill;
+            shadow[o] = sh;
+        }
+    }
+    free(mul);
+    free(mp);
     free(cn);
 }

@@ -73,3 +73,40 @@ def test_geometry_config1_full_size(ctx):
         print(f"[parity] config 1 step {s}: shadow mismatches {mism}, max |d illum| {di:.2e}, "
               f"oracle {t_ref:.2f} s for the step", flush=True)
         assert mism == 0 and di <= 1e-3
+
+
+def test_crater_sunlight_parity(ctx):
+    """Crater sunlight (btpm_crater_sunlight) bit for bit against the
+    restatement: random Sun vectors (zenith and below-horizon ones included)
+    and lit flags, then the chain from the geometry producer's
+    sfcillumvector (sun_scale 10000) on a dented binary over 650 steps."""
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    c = np.load(os.path.join(gold, "new_crater2.npz"))
+    cv, cf = c["V"].astype(np.float32), c["F"].astype(np.int32)
+    rng = np.random.default_rng(4)
+    nf, nt = 300, 40
+    d = rng.standard_normal((nf, nt, 3))
+    d[0, 0] = (0.0, 0.0, 1.0)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    sun = (10000.0 * d).astype(np.float32)
+    lit = (rng.random((nf, nt)) > 0.3).astype(np.int32)
+    g = th.crater_sunlight(ctx, lit, sun, cv, cf)
+    o = O.crater_sun(lit, sun, cv, cf)
+    for name, a, b in (("shadow", g[0], o[0]), ("cosang", g[1], o[1])):
+        mism = int((a != b).sum())
+        print(f"[parity] crater sunlight random {name}: {mism} mismatches of {a.size}", flush=True)
+        assert mism == 0
+    assert 0.2 < float((o[0] > 0).mean()) < 0.6 and float(((o[0] > 0) & (o[0] < 1)).mean()) > 0.05
+    equ, fv, npv = inputs.binary_shape(2, 1, sep=1.2, dent=0.35)
+    kw = dict(nv_primary=npv, rev_primary=3.0, rev_secondary=1.0)
+    pos, pole = (1.0, 0.3, 0.0), (math.pi / 3, 0.4)
+    gg = th.geometry_illumination(ctx, equ, fv, 650, pos, pole, sun_scale=10000.0, **kw)
+    og = O.geometry(equ, fv, 650, pos, pole, sun_scale=10000.0, **kw)
+    assert np.array_equal(gg.shadow, og.shadow)
+    assert float(np.abs(gg.sun_facet - og.sun_facet).max()) <= 0.02   # 10000 x (2e-6)
+    g = th.crater_sunlight(ctx, gg.shadow, gg.sun_facet, cv, cf)
+    o = O.crater_sun(gg.shadow, gg.sun_facet, cv, cf)
+    mism = int((g[0] != o[0]).sum()) + int((g[1] != o[1]).sum())
+    print(f"[parity] crater sunlight chain: {mism} mismatches of {2 * g[0].size}", flush=True)
+    assert mism == 0

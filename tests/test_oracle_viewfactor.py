@@ -348,3 +348,58 @@ def test_drf_rim_visibility_matches_fp64(theta_deg):
             assert abs(drf[j] - want) < 1e-5 + 1e-4 * want, (phi, j, drf[j], want)
             tested += 1
         assert tested > 40
+
+
+# ---------------------------------------------------------------------------
+# crater sunlight
+# ---------------------------------------------------------------------------
+
+def _crater_normals(cv, cf):
+    c = cv.astype(np.float64)
+    n = np.cross(c[cf[:, 1]] - c[cf[:, 0]], c[cf[:, 2]] - c[cf[:, 0]])
+    return n / np.linalg.norm(n, axis=1)[:, None]
+
+
+def test_crater_sun_at_zenith_and_dark_facets():
+    """Sun along the facet normal: no rim shadow, every sub-facet facing up
+    fully lit with cos = n_z; a dark facet has no sunlit sub-facet."""
+    cv, cf = _crater()
+    sun = np.zeros((2, 1, 3), np.float32)
+    sun[:, 0, 2] = 10000.0
+    sh, co = O.crater_sun(np.array([[1], [0]], np.int32), sun, cv, cf)
+    nz = _crater_normals(cv, cf)[:, 2]
+    assert np.allclose(co[0, 0], nz, atol=1e-6) and np.array_equal(co[0, 0], co[1, 0])
+    assert np.array_equal(sh[0, 0], (co[0, 0] > 0).astype(np.float32))
+    assert not sh[1].any()
+
+
+@pytest.mark.parametrize("elev_deg", [10.0, 35.0])
+def test_crater_sun_rim_shadow_matches_ray_trace(elev_deg):
+    """Sunlit fractions = an FP64 ray trace from independently generated
+    samples (centroids of the similar m x m split): a sample is shadowed when
+    its ray towards the Sun leaves the rim plane z = 0 outside the rim circle
+    (radius 10).  Samples within 1e-4 of that boundary are excluded."""
+    cv, cf = _crater()
+    nrm = _crater_normals(cv, cf)
+    e = math.radians(elev_deg)
+    tested = 0
+    for az in (0.2, 2.0, 4.4):
+        u = np.array([math.cos(az), math.sin(az)])
+        sun = (10000.0 * np.array([math.cos(e) * u[0], math.cos(e) * u[1], math.sin(e)]))
+        sh, co = O.crater_sun(np.ones((1, 1), np.int32), sun.astype(np.float32).reshape(1, 1, 3),
+                              cv, cf)
+        for j in range(len(cf)):
+            cosj = float(nrm[j] @ sun) / float(np.linalg.norm(sun))
+            assert abs(co[0, 0, j] - cosj) < 1e-5
+            if cosj <= 1e-4:
+                assert sh[0, 0, j] == 0.0
+                continue
+            p = _similar_split_centroids(cv.astype(np.float64)[cf[j]], 10)
+            lam = -p[:, 2] / math.tan(e)
+            r = np.linalg.norm(p[:, :2] + lam[:, None] * u[None, :], axis=1)
+            if np.any(np.abs(r - 10.0) < 1e-4):
+                continue
+            want = 1.0 - (r > 10.0).sum() / 100.0
+            assert abs(sh[0, 0, j] - want) < 1e-5, (az, j, sh[0, 0, j], want)
+            tested += 1
+    assert tested > 100
