@@ -1,13 +1,14 @@
 """Writes the small data fixtures the CPU tests use, from data files that ship
 in the reference tree (data only: meshes, view-factor lists, photon momentum
-vectors, the mesh statistics file, colour corrections).  Run in a checkout that has the reference
+vectors, the mesh statistics file, colour corrections, light curves).  Run in a checkout that has the reference
 at REF:
 
     python tests/golden/make_fixtures.py [REF]
 
 Outputs (committed): tests/golden/new_crater2.npz,
 tests/golden/doppler_binary.npz, tests/golden/doppler_binary_properties.json,
-tests/golden/fg3_observe.npz (WISE colour corrections and the 1996 FG3 mesh)
+tests/golden/fg3_observe.npz (WISE colour corrections and the 1996 FG3 mesh),
+tests/golden/fg3_flux_set4.npz (shipped 1996 FG3 light curves, set 4)
 """
 import json
 import os
@@ -39,6 +40,14 @@ def main():
     V, F = formats.read_shape(os.path.join(R, "1996FG3_binary_lowres.txt"), 2052, 4096)
     np.savez_compressed(os.path.join(HERE, "fg3_observe.npz"), colour=cc, V=V,
                         F=F.astype(np.int32))
+    # the shipped smooth and rough light curves of observation set 4 at three
+    # thermal inertias (Observe's output files)
+    flux = {}
+    for ti in (10, 100, 300):
+        for kind in ("smooth", "rough"):
+            flux[f"{kind}_{ti}"] = np.loadtxt(
+                os.path.join(R, f"1996FG3_binary_lowres_{kind}_flux({ti}_4).txt")).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "fg3_flux_set4.npz"), **flux)
     props = {}
     with open(os.path.join(D, "doppler_binary_properties.txt")) as fh:
         for line in fh:
