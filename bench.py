@@ -495,7 +495,7 @@ def observe_phase(ctx, cpu=True):
     facet visibility from the geometry producer with the observer as the
     source, instantaneous-equilibrium temperatures from its Sun illumination
     (A = 0.01), crater temperatures and visible fractions from a fixed
-    sub-facet pattern.  GPU wall time of the call (9.6 GB of crater inputs
+    sub-facet pattern.  GPU wall time of the call (19.2 GB of crater inputs
     uploaded, included); the FP64 restatement on sampled steps."""
     import math
 

@@ -17,7 +17,8 @@
 //                  summed over the wave, scaled by A_facet / S_crater; the
 //                  smooth facet term; both written per facet, band and step.
 //   btpm_obs_sum   64 steps per workgroup, 16 facet ranges summed in order
-//                  and combined in order; the colour temperature fit (100..566 K, band ratios; the
+//                  and combined in order; the colour temperature fit (100..566 K, band ratios,
+//                  the table split over the 16 waves, first minimum kept; the
 //                  smooth fit keeps the reference's comma-operator terms,
 //                  :782, i.e. fits band 1 alone) and the colour corrections.
 //
@@ -179,6 +180,8 @@ extern "C" __global__ __launch_bounds__(1024) void btpm_obs_sum(
     const double* __restrict__ colour, int tmin, int tcount, double* __restrict__ smooth,
     double* __restrict__ rough) {
   __shared__ double part[16][6][64];
+  __shared__ double fchi[2][16][64];
+  __shared__ int fk[2][16][64];
   const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
   const int sl = blockIdx.x * 64 + lane;
   const int per = (nf + 15) / 16;
@@ -194,7 +197,7 @@ extern "C" __global__ __launch_bounds__(1024) void btpm_obs_sum(
 #pragma unroll
   for (int k = 0; k < 6; ++k) part[seg][k][lane] = acc[k];
   __syncthreads();
-  if (seg != 0 || sl >= nsteps) return;
+  // every wave forms the step's band fluxes (the 16 ranges in order) ...
   double sm[3] = {0.0, 0.0, 0.0}, ro[3] = {0.0, 0.0, 0.0};
   for (int g = 0; g < 16; ++g)
 #pragma unroll
@@ -202,30 +205,59 @@ extern "C" __global__ __launch_bounds__(1024) void btpm_obs_sum(
       sm[i] = sm[i] + part[g][i][lane];
       ro[i] = ro[i] + part[g][3 + i][lane];
     }
+  // ... and scans its own range of colour temperatures: the first minimum of
+  // the range (strict <, from 100.0 like the full scan); squares as products,
+  // the correctly rounded value of pow(x, 2.0)
+  const int kper = (tcount + 15) / 16;
+  const int k0 = seg * kper, k1 = min(tcount, k0 + kper);
   double cs_s = 100.0, cs_r = 100.0;
-  int bt_s = tmin, bt_r = tmin;
-  for (int k = 0; k < tcount; ++k) {
-    const double b0 = bn[3 * k], b1 = bn[3 * k + 1], b2 = bn[3 * k + 2];
+  int bt_s = -1, bt_r = -1;
+  {
     double am = (sm[0] + sm[1] + sm[2]) / 3.0;
-    double m2 = sm[0] / am;
-    double chi = pow((b0 - m2) / m2, 2.0) + 2.0 + 2.0;
-    if (chi < cs_s) {
-      cs_s = chi;
-      bt_s = tmin + k;
-    }
+    const double s2 = sm[0] / am;
     am = (ro[0] + ro[1] + ro[2]) / 3.0;
-    m2 = ro[0] / am;
-    const double m3 = ro[1] / am, m4 = ro[2] / am;
-    chi = pow((b0 - m2) / m2, 2.0) + pow((b1 - m3) / m3, 2.0) + pow((b2 - m4) / m4, 2.0);
-    if (chi < cs_r) {
-      cs_r = chi;
-      bt_r = tmin + k;
+    const double r2 = ro[0] / am, r3 = ro[1] / am, r4 = ro[2] / am;
+    for (int k = k0; k < k1; ++k) {
+      const double b0 = bn[3 * k], b1 = bn[3 * k + 1], b2 = bn[3 * k + 2];
+      const double e0 = (b0 - s2) / s2;
+      double chi = e0 * e0 + 2.0 + 2.0;
+      if (chi < cs_s) {
+        cs_s = chi;
+        bt_s = k;
+      }
+      const double d0 = (b0 - r2) / r2, d1 = (b1 - r3) / r3, d2 = (b2 - r4) / r4;
+      chi = d0 * d0 + d1 * d1 + d2 * d2;
+      if (chi < cs_r) {
+        cs_r = chi;
+        bt_r = k;
+      }
+    }
+  }
+  fchi[0][seg][lane] = cs_s;
+  fchi[1][seg][lane] = cs_r;
+  fk[0][seg][lane] = bt_s;
+  fk[1][seg][lane] = bt_r;
+  __syncthreads();
+  if (seg != 0 || sl >= nsteps) return;
+  // ranges in order with strict <: the first minimum of the whole table
+  cs_s = 100.0;
+  cs_r = 100.0;
+  bt_s = 0;
+  bt_r = 0;
+  for (int g = 0; g < 16; ++g) {
+    if (fchi[0][g][lane] < cs_s) {
+      cs_s = fchi[0][g][lane];
+      bt_s = fk[0][g][lane];
+    }
+    if (fchi[1][g][lane] < cs_r) {
+      cs_r = fchi[1][g][lane];
+      bt_r = fk[1][g][lane];
     }
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    smooth[3 * (size_t)sl + i] = sm[i] * colour[(size_t)i * tcount + (bt_s - tmin)];
-    rough[3 * (size_t)sl + i] = ro[i] * colour[(size_t)i * tcount + (bt_r - tmin)];
+    smooth[3 * (size_t)sl + i] = sm[i] * colour[(size_t)i * tcount + bt_s];
+    rough[3 * (size_t)sl + i] = ro[i] * colour[(size_t)i * tcount + bt_r];
   }
 }
 
