@@ -134,7 +134,8 @@ __device__ __forceinline__ void load4(const gfloat* p, float (&v)[BTPM_BLK], int
 // the scaled profile as U_0 = (T_0 - Tref) / s.
 // The Newton start x0 is a predictor: the previous rotation's surface
 // temperature at this step shifted by the difference between the two
-// rotations at the previous step (the caller's dl = t - t_prev).
+// rotations, extrapolated linearly from the two previous steps (the caller's
+// dl = t - t_prev at step k-1, dl2 at step k-2: x0 = t_prev + 2 dl - dl2).
 template <int NX, typename SC>
 __device__ __forceinline__ double conduct_step(float (&u)[NX + 1], float kr, SC s, SC is,
                                                double src, double m1, double es, double Tref,
@@ -145,7 +146,16 @@ __device__ __forceinline__ double conduct_step(float (&u)[NX + 1], float kr, SC 
     t1 = fma(s, (double)u[1], Tref);
   else
     t1 = Tref + (double)(s * u[1]);
-  const double x = btpm_dev::sbc_newton(src, m1, t1, es, fmax(x0, 1.0), tol, cap);
+  // the first Newton iteration is straight-line code next to the FTCS of
+  // nodes 2..NX, which do not depend on it: the empty asm statements keep
+  // those nodes' updates ahead of the (rarely taken) iteration loop, so the
+  // scheduler fills the FP64 chain's latency with them
+  const double K = fma(m1, t1, src);
+  double x = fmax(x0, 1.0);
+  const double d0 = btpm_dev::newton_step(K, m1, es, x);
+#pragma unroll
+  for (int i = 2; i <= NX; ++i) asm volatile("" : "+v"(u[i]));
+  x = btpm_dev::newton_finish(K, m1, es, x, d0, tol, cap);
   if constexpr (std::is_same<SC, double>::value)
     u[0] = (float)((x - Tref) * is);
   else
@@ -252,7 +262,8 @@ __global__ __launch_bounds__(W, 4) void btpm_conduct_smooth(const btpm_tile* __r
   load8(Tin + coff, nprev);
 
   float dm = 0.0f;
-  float dl = 0.0f;  // T - T_prev at the previous step (predictor shift)
+  float dl = 0.0f;   // T - T_prev at the previous step (predictor shift)
+  float dl2 = 0.0f;  // and at the step before
   // the TI = 0 loop is a separate instantiation so the main loop stays lean
   auto rotation = [&](auto inst_tag) {
     constexpr bool INST = decltype(inst_tag)::value;
@@ -277,8 +288,9 @@ __global__ __launch_bounds__(W, 4) void btpm_conduct_smooth(const btpm_tile* __r
             sc.sd *= sc.sig;
             sc.isd *= sc.isig;
             t = (float)conduct_step<NX>(u, sc.kr, sc.sd, sc.isd, (double)nsrc[k], m1, es, Trefd,
-                                        tol, cap, (double)(tp + dl));
+                                        tol, cap, (double)(tp + (2.0f * dl - dl2)));
           }
+          dl2 = dl;
           dl = t - tp;
           dm = fmaxf(dm, fabsf(dl));
         }
@@ -566,7 +578,8 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   load8(Tin + coff, nprev);
 
   float dm = 0.0f;
-  float dl = 0.0f;  // T - T_prev at the previous step (predictor shift)
+  float dl = 0.0f;   // T - T_prev at the previous step (predictor shift)
+  float dl2 = 0.0f;  // and at the step before
   auto rotation = [&](auto inst_tag) {
     constexpr bool INST = decltype(inst_tag)::value;
     for (int b = 0; b < nb; ++b) {
@@ -608,8 +621,9 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
             // 1): a bounded perturbation of T_1 and T_0, not a gain on the profile
             const float2 sk = cc.sk[k];
             t = (float)conduct_step<NX>(u, kr, sk.x, sk.y, tot, cc.m1, es, Trefd, tol, cap,
-                                        (double)(tp + dl));
+                                        (double)(tp + (2.0f * dl - dl2)));
           }
+          dl2 = dl;
           dl = t - tp;
           dm = fmaxf(dm, fabsf(dl));
         }
