@@ -1106,6 +1106,82 @@ struct Sched {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace btpm {
+namespace {
+constexpr size_t kStageSlice = 256u << 20;
+
+void stage_buffers(btpm_ctx* ctx) {
+  for (int k = 0; k < 2; ++k)
+    if (!ctx->stage[k]) {
+      HIPCHK(hipHostMalloc((void**)&ctx->stage[k], kStageSlice, hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&ctx->stage_ev[k], hipEventDisableTiming));
+    }
+}
+
+// memcpy split over up to 8 host threads (pageable <-> pinned slices)
+void par_memcpy(char* d, const char* s, size_t n) {
+  const int nt = (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+  const size_t part = (n + nt - 1) / nt;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) {
+    const size_t a = std::min(n, (size_t)t * part), b = std::min(n, a + part);
+    if (a < b) pool.emplace_back([=] { std::memcpy(d + a, s + a, b - a); });
+  }
+  std::memcpy(d, s, std::min(n, part));
+  for (auto& th : pool) th.join();
+}
+}  // namespace
+
+void upload_staged(btpm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  hipStream_t st = ctx->stream;
+  if (bytes < 2 * kStageSlice) {
+    if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return;
+  }
+  stage_buffers(ctx);
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  size_t i = 0;
+  for (size_t off = 0; off < bytes; off += kStageSlice, ++i) {
+    const size_t n = std::min(kStageSlice, bytes - off);
+    const int k = (int)(i & 1);
+    // this buffer's previous slice has left for the device
+    if (i >= 2) HIPCHK(hipEventSynchronize(ctx->stage_ev[k]));
+    par_memcpy(ctx->stage[k], s + off, n);
+    HIPCHK(hipMemcpyAsync(d + off, ctx->stage[k], n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(ctx->stage_ev[k], st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+void download_staged(btpm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  hipStream_t st = ctx->stream;
+  if (bytes < 2 * kStageSlice) {
+    if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return;
+  }
+  stage_buffers(ctx);
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  const size_t ns = (bytes + kStageSlice - 1) / kStageSlice;
+  auto issue = [&](size_t j) {
+    const size_t off = j * kStageSlice, n = std::min(kStageSlice, bytes - off);
+    HIPCHK(hipMemcpyAsync(ctx->stage[j & 1], s + off, n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ctx->stage_ev[j & 1], st));
+  };
+  issue(0);
+  for (size_t j = 0; j < ns; ++j) {
+    // the other buffer's slice was copied out in the previous pass
+    if (j + 1 < ns) issue(j + 1);
+    HIPCHK(hipEventSynchronize(ctx->stage_ev[j & 1]));
+    const size_t off = j * kStageSlice;
+    par_memcpy(d + off, ctx->stage[j & 1], std::min(kStageSlice, bytes - off));
+  }
+}
+}  // namespace btpm
+
 extern "C" {
 
 int btpm_version(void) { return 100; }
@@ -1155,6 +1231,7 @@ int btpm_ctx_create(int device, btpm_ctx** out) {
   API_END
 }
 
+
 int btpm_ctx_destroy(btpm_ctx* c) {
   if (!c) return BTPM_OK;
   (void)hipSetDevice(c->device);
@@ -1163,6 +1240,10 @@ int btpm_ctx_destroy(btpm_ctx* c) {
   (void)hipFree(c->d_sets);
   (void)hipFree(c->d_status);
   (void)hipHostFree(c->h_status);
+  for (int k = 0; k < 2; ++k) {
+    if (c->stage[k]) (void)hipHostFree(c->stage[k]);
+    if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
+  }
   (void)hipStreamDestroy(c->stream);
   delete c;
   return BTPM_OK;

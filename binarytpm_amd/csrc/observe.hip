@@ -379,9 +379,11 @@ extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* sm
   upload(d_frame, frame, 15, st);
   upload(d_vis, in->visible, nser, st);
   upload(d_st, in->stemp, nser, st);
-  if (rough_on) {
-    upload(d_cvis, in->crater_visible, nser * ncf, st);
-    upload(d_ct, in->crater_temp, nser * ncf, st);
+  if (rough_on) {  // the large inputs (4 B per sub-facet and step each) through pinned slices
+    d_cvis.alloc(nser * ncf * sizeof(float));
+    upload_staged(ctx, d_cvis.p, in->crater_visible, nser * ncf * sizeof(float));
+    d_ct.alloc(nser * ncf * sizeof(float));
+    upload_staged(ctx, d_ct.p, in->crater_temp, nser * ncf * sizeof(float));
   }
   upload(d_cs, csub.data(), csub.size(), st);
   upload(d_pl, pl, 9, st);

@@ -79,8 +79,8 @@ extern "C" int btpm_probe_valu(btpm_ctx* ctx, int op, int iters, double* wave_in
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   const double waves = (double)blocks * 4.0;
   *wave_instr_per_s = waves * (double)iters * kChains / (ms * 1e-3);
   API_END

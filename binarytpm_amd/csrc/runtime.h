@@ -128,7 +128,22 @@ struct btpm_ctx {
   // autotuner decisions of this context (guarded by mu): (tunable, size
   // class) -> chosen value; kept for every later batch run or server
   std::map<std::pair<int, long long>, int> tuned;
+  // two pinned staging slices for large uploads (allocated on first use,
+  // freed with the context; used under mu)
+  char* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
+
+namespace btpm {
+// Host-to-device copy of a large pageable buffer, complete on return: slices
+// are copied by host threads into one of the context's two pinned staging
+// buffers while the DMA of the other slice runs on the context's stream.
+// Small copies go straight to hipMemcpyAsync.  Caller holds ctx->mu.
+void upload_staged(btpm_ctx* ctx, void* dst, const void* src, size_t bytes);
+// The device-to-host counterpart: the DMA of the next slice runs while host
+// threads copy the previous one out of its pinned buffer.
+void download_staged(btpm_ctx* ctx, void* dst, const void* src, size_t bytes);
+}  // namespace btpm
 
 // Takes ctx->mu for one API call, announcing itself so that a running
 // server lets it in between two scheduling steps (std::mutex is not fair).

@@ -897,8 +897,7 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
                      d_cs.as<float4>(), d_samp.as<float4>(), d_vt.as<float>(), ncf, nsamp,
                      in->crater_radius, nnz, d_out.as<float>());
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(drf, d_out.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  download_staged(ctx, drf, d_out.p, (size_t)n * sizeof(float));
   API_END
 }
 
@@ -949,11 +948,8 @@ extern "C" int btpm_crater_sunlight(btpm_ctx* ctx, const btpm_crater_sun_in* in,
                        d_vt.as<float>(), ncf, nsamp, in->crater_radius, n, d_sh.as<float>(),
                        d_co.as<float>());
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(shadow + q0 * ncf, d_sh.p, (size_t)n * ncf * sizeof(float),
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(cosang + q0 * ncf, d_co.p, (size_t)n * ncf * sizeof(float),
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    download_staged(ctx, shadow + q0 * ncf, d_sh.p, (size_t)n * ncf * sizeof(float));
+    download_staged(ctx, cosang + q0 * ncf, d_co.p, (size_t)n * ncf * sizeof(float));
   }
   API_END
 }
