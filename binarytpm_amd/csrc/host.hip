@@ -1454,7 +1454,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
         nchunks += (size_t)(body->rowptr[f + 1] - body->rowptr[f] + 31) / 32;
       }
       DBuf raw;
-      upload(raw, crater->drf, (size_t)body->nnz * BTPM_NS, st);
+      upload_big(ctx, raw, crater->drf, (size_t)body->nnz * BTPM_NS);
       upload(sh->cbase, cb.data(), cb.size(), st);
       sh->drf_scale.alloc((size_t)nf * sizeof(float));
       sh->drfA.alloc(std::max<size_t>(nchunks, 1) * BTPM_DRF_TILE * sizeof(_Float16));
@@ -1466,7 +1466,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
     }
     if (crater->illum) {
       DBuf raw;
-      upload(raw, crater->illum, (size_t)nf * nt * BTPM_NS, st);
+      upload_big(ctx, raw, crater->illum, (size_t)nf * nt * BTPM_NS);
       hipLaunchKernelGGL(btpm_pack_crater, dim3(grid1((size_t)nf * BTPM_NS * nb, 256)), dim3(256),
                          0, st, raw.as<float>(), sh->cillum.as<float>(), nf, nt, nb);
       HIPCHK(hipStreamSynchronize(st));
@@ -1478,8 +1478,8 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
         upload(dvis, crater->vis_table, (size_t)crater->n_el * crater->n_az * BTPM_NS, st);
         upload(dnrm, crater->normals, (size_t)BTPM_NS * 3, st);
       } else {
-        upload(dsh, crater->shadow, (size_t)nf * nt * BTPM_NS, st);
-        upload(dcos, crater->cosang, (size_t)nf * nt * BTPM_NS, st);
+        upload_big(ctx, dsh, crater->shadow, (size_t)nf * nt * BTPM_NS);
+        upload_big(ctx, dcos, crater->cosang, (size_t)nf * nt * BTPM_NS);
       }
       const float* rgip = nullptr;
       if (sh->drfA.p) {

@@ -143,6 +143,11 @@ void upload_staged(btpm_ctx* ctx, void* dst, const void* src, size_t bytes);
 // The device-to-host counterpart: the DMA of the next slice runs while host
 // threads copy the previous one out of its pinned buffer.
 void download_staged(btpm_ctx* ctx, void* dst, const void* src, size_t bytes);
+template <class T>
+void upload_big(btpm_ctx* ctx, DBuf& b, const T* h, size_t count) {
+  b.alloc(count * sizeof(T));
+  upload_staged(ctx, b.p, h, count * sizeof(T));
+}
 }  // namespace btpm
 
 // Takes ctx->mu for one API call, announcing itself so that a running
