@@ -586,7 +586,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (solves are "
-              "independent; the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "independent and would be dealt to one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from binarytpm_amd import thermal as th

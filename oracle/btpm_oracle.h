@@ -25,8 +25,8 @@
  *   shape->crater view factors drf[nnz][ns]: per CSR entry (f,g), the view
  *   factor of g seen from each of f's crater sub-facets.
  *
- * Pinning status of each function is stated in DESIGN.md's oracle table and
- * in tests/test_oracle.py.
+ * Each function is pinned by the CPU tests under tests/ (test_oracle*.py)
+ * against the reference's shipped data files and closed-form cases.
  */
 #ifndef BTPM_ORACLE_H
 #define BTPM_ORACLE_H

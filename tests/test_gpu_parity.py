@@ -1,5 +1,5 @@
 """GPU parity: the HIP library (through the C ABI) against the FP64 oracle on
-the same seeded inputs.  Bar: max |dT| <= 0.05 K (BASELINE tolerance), RMS
+the same seeded inputs.  Bar: max |dT| <= 0.05 K (the stage's stated tolerance), RMS
 reported; integer/flag outputs (rotation counts) within one rotation."""
 import numpy as np
 import pytest
