@@ -391,12 +391,27 @@ def geometry_phase(ctx, cpu=True):
     return out
 
 
-def valu_ceiling(ctx):
+def valu_ceiling(device=0):
     """Wave instructions per second the device issues for unpacked 32-bit
-    VALU instructions of one kind (calibration kernel btpm_probe_valu): the
-    ceiling against which the compute-bound kernels' issue rates are read."""
-    from binarytpm_amd import thermal as th
-    return {op: th.probe_valu(ctx, op, 16384) for op in ("fma", "add", "mul", "iadd", "sqrt")}
+    VALU instructions of one kind (calibration kernel of
+    binarytpm_amd/bench_tools/probe.hip, built into libbtpm_probe.so beside
+    the library but not part of it): the ceiling against which the
+    compute-bound kernels' issue rates are read."""
+    import ctypes as C
+    path = os.path.join(ROOT, "binarytpm_amd", "libbtpm_probe.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build with `python -m binarytpm_amd.build`")
+    L = C.CDLL(path)
+    L.btpm_probe_valu.restype = C.c_int
+    L.btpm_probe_valu.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    out = {}
+    for code, op in enumerate(("fma", "add", "mul", "sqrt", "iadd")):
+        r = C.c_double()
+        rc = L.btpm_probe_valu(device, code, 16384, C.byref(r))
+        if rc != 0:
+            raise RuntimeError(f"btpm_probe_valu({op}) failed with {rc}")
+        out[op] = r.value
+    return out
 
 
 def viewfactor_phase(ctx, cpu=True):
@@ -602,7 +617,7 @@ def main():
         del src, dst
     if args.producers_only:
         with th.Context(0) as ctx:
-            line = {"valu_ceiling_wave_instr_per_s": valu_ceiling(ctx),
+            line = {"valu_ceiling_wave_instr_per_s": valu_ceiling(),
                     "illumination_producer": geometry_phase(ctx, cpu=not args.no_cpu),
                     "viewfactor_producer": viewfactor_phase(ctx, cpu=not args.no_cpu),
                     "observe_fluxes": observe_phase(ctx, cpu=not args.no_cpu)}
@@ -729,7 +744,7 @@ def main():
             line["viewfactor_producer"] = viewfactor_phase(ctx, cpu=not args.no_cpu)
         if not args.no_observe:
             line["observe_fluxes"] = observe_phase(ctx, cpu=not args.no_cpu)
-        line["roofline"]["valu_ceiling_wave_instr_per_s"] = valu_ceiling(ctx)
+        line["roofline"]["valu_ceiling_wave_instr_per_s"] = valu_ceiling()
     print(json.dumps(line), flush=True)
 
 

@@ -133,7 +133,7 @@ class BatchStats(C.Structure):
         ("smooth_column_steps", C.c_longlong), ("crater_column_steps", C.c_longlong),
         ("max_resident", C.c_int), ("deferrals", C.c_int),
         ("tune_rotations", C.c_int), ("tune_decisions", C.c_int),
-        ("fast_rotations", C.c_longlong),
+        ("fast_rotations", C.c_longlong), ("mem_waits", C.c_int),
     ]
 
 
@@ -149,6 +149,7 @@ PROTOTYPES = [
     ("btpm_ctx_create", C.c_int, [C.c_int, C.POINTER(_vp)]),
     ("btpm_ctx_destroy", C.c_int, [_vp]),
     ("btpm_ctx_mem_info", C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    ("btpm_ctx_set_mem_limit", C.c_int, [_vp, C.c_size_t]),
     ("btpm_shape_create", C.c_int, [_vp, C.POINTER(Params), C.POINTER(Body), C.POINTER(Crater),
                                     C.POINTER(_vp)]),
     ("btpm_shape_destroy", C.c_int, [_vp]),
@@ -157,6 +158,7 @@ PROTOTYPES = [
     ("btpm_solve_create", C.c_int, [_vp, _vp, C.c_float, _fp, C.c_int, C.POINTER(_vp)]),
     ("btpm_solve_destroy", C.c_int, [_vp]),
     ("btpm_solve_set_flux_scale", C.c_int, [_vp, C.c_float]),
+    ("btpm_solve_warm_start", C.c_int, [_vp, _vp]),
     ("btpm_solve_run_stepwise", C.c_int, [_vp, C.c_int]),
     ("btpm_solve_run", C.c_int, [_vp]),
     ("btpm_solve_get_info", C.c_int, [_vp, C.POINTER(SolveInfo)]),
@@ -173,7 +175,6 @@ PROTOTYPES = [
                                        C.POINTER(C.c_int)]),
     ("btpm_server_stop", C.c_int, [_vp, C.POINTER(BatchStats)]),
     ("btpm_crater_contract", C.c_int, [_vp, _fp, _fp, C.c_int, _fp]),
-    ("btpm_probe_valu", C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     ("btpm_geometry_illumination", C.c_int,
      [_vp, C.POINTER(Geometry), C.POINTER(C.c_int), _fp, _fp, _fp]),
     ("btpm_vf_build", C.c_int, [_vp, C.POINTER(VfMesh), C.POINTER(_vp)]),

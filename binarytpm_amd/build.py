@@ -1,4 +1,5 @@
-"""Build libbtpm.so in-tree with hipcc for gfx950 (no JIT, no torch).
+"""Build libbtpm.so in-tree with hipcc for gfx950 (no JIT, no torch), and the
+benchmark's calibration library libbtpm_probe.so (not part of the drop-in).
 
 Usage: python -m binarytpm_amd.build   (or binarytpm_amd.build.build())
 """
@@ -10,8 +11,11 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["conduct.hip", "aux.hip", "stepwise.hip", "geometry.hip", "viewfactor.hip", "observe.hip", "io.hip", "probe.hip", "host.hip", "compat.hip"]
+SOURCES = ["conduct.hip", "aux.hip", "stepwise.hip", "geometry.hip", "viewfactor.hip", "observe.hip",
+           "io.hip", "host.hip", "compat.hip"]
 LIB = os.path.join(HERE, "libbtpm.so")
+PROBE_SRC = os.path.join(HERE, "bench_tools", "probe.hip")
+PROBE_LIB = os.path.join(HERE, "libbtpm_probe.so")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -22,19 +26,24 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+def _hipcc(srcs, out, verbose, extra=()):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [hipcc, *HIPCC_FLAGS, "-I", CSRC, "-I", os.path.join(HERE, "..", "include"),
-           *srcs, "-o", tmp, "-lpthread"]
+           *srcs, "-o", tmp, *extra]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"hipcc failed with status {res.returncode}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    _hipcc(srcs, LIB, verbose, ("-lpthread",))
+    _hipcc([PROBE_SRC], PROBE_LIB, verbose)
     return LIB
 
 

@@ -126,6 +126,24 @@ extern "C" __global__ void btpm_init_smooth(btpm_set_dev S) {
   S.dmax[c] = 0.0f;
 }
 
+// Warm start of a smooth set whose T and profile were copied from a finished
+// solve (the TI sweep's carried state, T175706/thermal.cpp:3676-3700): the
+// self-heating emission of the carried rotation, nothing frozen.
+extern "C" __global__ void btpm_warm_smooth(btpm_set_dev S) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ncols) return;
+  if (S.emis)  // facet-major [c][nb*BLK] from the blocked T
+    for (int b = 0; b < S.nb; ++b) {
+      const float* p = S.T + ((size_t)b * S.ncols + c) * BTPM_BLK;
+      for (int k = 0; k < BTPM_BLK; ++k) {
+        const float t = p[k];
+        S.emis[(size_t)c * S.nb * BTPM_BLK + (size_t)b * BTPM_BLK + k] = S.esia * (t * t) * (t * t);
+      }
+    }
+  S.frozen[c] = 0;
+  S.dmax[c] = 0.0f;
+}
+
 // roughness_initialise_temperatures, T175706/thermal_GPU.cu:374-423: two
 // passes per crater; src = (1-A) illum + RGH (blocked).  vf32[i*NS+j] =
 // VF(j->i).  One workgroup (128 lanes) per parent facet.

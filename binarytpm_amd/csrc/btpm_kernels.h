@@ -36,6 +36,7 @@ extern "C" __global__ void btpm_pack_crater(const float* raw, float* blk, int nf
 extern "C" __global__ void btpm_unpack_crater(const float* blk, float* raw, int nfac, int ntau,
                                               int nb);
 extern "C" __global__ void btpm_init_smooth(btpm_set_dev S);
+extern "C" __global__ void btpm_warm_smooth(btpm_set_dev S);
 extern "C" __global__ void btpm_init_crater(btpm_set_dev S, const float* vf32);
 extern "C" __global__ void btpm_pack_drf(const int* rowptr, const float* drf, const int* cbase,
                                          float* drf_scale, _Float16* drfA);

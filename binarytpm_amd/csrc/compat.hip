@@ -170,11 +170,18 @@ bool guarded(const char* name, F&& f) {
       c.ctx = x;
     }
     HIPCHK(hipSetDevice(c.ctx->device));
+    StreamScope scope(c.ctx->stream);  // buffers released in f() stay on this stream
     f();
     HIPCHK(hipStreamSynchronize(c.ctx->stream));
     return true;
   } catch (const HipError& e) {
     c.err = e.code;
+    return false;
+  } catch (const std::bad_alloc&) {
+    c.err = fail(BTPM_ERR_NOMEM, "%s: host allocation failed", name);
+    return false;
+  } catch (const std::exception& e) {
+    c.err = fail(BTPM_ERR_STATE, "%s: internal error: %s", name, e.what());
     return false;
   }
 }
@@ -523,6 +530,7 @@ void obtain_GPU_roughness_temperatures(float* sfc_stemp0_H, float* sfc_stemp1_H,
     hipLaunchKernelGGL(compat_profile_out, dim3(grid1(np, 256)), dim3(256), 0, st,
                        c.prof.as<float>(), pr.as<float>(), c.nf, BTPM_NX);
     HIPCHK(hipMemcpyAsync(sfc_temp_H, pr.p, np * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));  // the copy has left pr before it is released
   });
 }
 

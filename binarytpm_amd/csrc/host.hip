@@ -21,6 +21,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -35,30 +36,52 @@ using namespace btpm;
 
 namespace btpm {
 namespace {
+struct Cached {
+  void* p;
+  hipStream_t tag;  // stream that may still use the block (null: idle)
+};
 struct Pool {
   std::mutex mu;
-  std::map<int, std::multimap<size_t, void*>> cache;  // device -> capacity -> block
-  std::map<void*, std::pair<int, size_t>> live;       // block -> (device, capacity)
+  std::map<int, std::multimap<size_t, Cached>> cache;  // device -> capacity -> block
+  std::map<void*, std::pair<int, size_t>> live;        // block -> (device, capacity)
   std::map<int, size_t> cached;
   std::map<int, size_t> live_bytes;
+  std::map<int, size_t> limit;                          // 0 / absent: none
 };
 Pool& the_pool() {
   static Pool* p = new Pool;  // never destroyed: DBufs of static objects may outlive main
   return *p;
 }
+
 }  // namespace
+
+// Test hook: BTPM_INJECT_NOMEM=k (read when a batch or server starts) makes
+// every k-th column-set build (smooth or crater) fail as if the device were
+// full while another solve runs, so the scheduler's deferral path runs on
+// demand.
+int inject_period() {
+  const char* e = std::getenv("BTPM_INJECT_NOMEM");
+  return e ? std::max(0, std::atoi(e)) : 0;
+}
+
+hipStream_t& scope_stream() {
+  thread_local hipStream_t s = nullptr;
+  return s;
+}
 
 void* pool_alloc(size_t bytes) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   const size_t want = (bytes + 255) & ~(size_t)255;
+  const hipStream_t tag = scope_stream();
   Pool& P = the_pool();
+  size_t lim = 0;
   {
     std::lock_guard<std::mutex> lk(P.mu);
     auto& c = P.cache[dev];
-    auto it = c.lower_bound(want);
-    if (it != c.end() && it->first <= want + want / 4) {
-      void* p = it->second;
+    for (auto it = c.lower_bound(want); it != c.end() && it->first <= want + want / 4; ++it) {
+      if (it->second.tag != nullptr && it->second.tag != tag) continue;
+      void* p = it->second.p;
       const size_t cap = it->first;
       c.erase(it);
       P.cached[dev] -= cap;
@@ -66,9 +89,17 @@ void* pool_alloc(size_t bytes) {
       P.live_bytes[dev] += cap;
       return p;
     }
+    lim = P.limit.count(dev) ? P.limit[dev] : 0;
   }
+  // the device limit counts live and cached blocks: beyond it, the cache is
+  // returned first, and the allocation fails as the driver's would
+  auto over_limit = [&] {
+    if (lim == 0) return false;
+    std::lock_guard<std::mutex> lk(P.mu);
+    return P.live_bytes[dev] + P.cached[dev] + want > lim;
+  };
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, want);
+  hipError_t e = over_limit() ? hipErrorOutOfMemory : hipMalloc(&p, want);
   size_t fr0 = 0, fr1 = 0, tot = 0, trimmed = 0;
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -76,7 +107,7 @@ void* pool_alloc(size_t bytes) {
     trimmed = pool_cached(dev);
     pool_trim(dev);
     (void)hipMemGetInfo(&fr1, &tot);
-    e = hipMalloc(&p, want);
+    e = over_limit() ? hipErrorOutOfMemory : hipMalloc(&p, want);
   }
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -87,8 +118,9 @@ void* pool_alloc(size_t bytes) {
       live = P.live_bytes[dev];
     }
     fail(code, "hipMalloc(%zu): %s (device free %.1f GB, %.1f GB after returning %.1f GB of "
-         "cached blocks, of %.1f GB; the library holds %.1f GB)", want, hipGetErrorString(e),
-         fr0 / 1e9, fr1 / 1e9, trimmed / 1e9, tot / 1e9, live / 1e9);
+         "cached blocks, of %.1f GB; the library holds %.1f GB%s)", want, hipGetErrorString(e),
+         fr0 / 1e9, fr1 / 1e9, trimmed / 1e9, tot / 1e9, live / 1e9,
+         lim ? ", at its device limit" : "");
     throw HipError{code};
   }
   std::lock_guard<std::mutex> lk(P.mu);
@@ -109,7 +141,7 @@ void pool_free(void* p) {
   const size_t cap = it->second.second;
   P.live.erase(it);
   P.live_bytes[dev] -= cap;
-  P.cache[dev].emplace(cap, p);
+  P.cache[dev].emplace(cap, Cached{p, scope_stream()});
   P.cached[dev] += cap;
 }
 
@@ -120,12 +152,33 @@ size_t pool_cached(int device) {
   return it == P.cached.end() ? 0 : it->second;
 }
 
+size_t pool_live(int device) {
+  Pool& P = the_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.live_bytes.find(device);
+  return it == P.live_bytes.end() ? 0 : it->second;
+}
+
+void pool_set_limit(int device, size_t bytes) {
+  Pool& P = the_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  P.limit[device] = bytes;
+}
+
+size_t pool_limit(int device) {
+  Pool& P = the_pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.limit.find(device);
+  return it == P.limit.end() ? 0 : it->second;
+}
+
+// hipFree waits for the device's work on the block, so trimming needs no tag
 void pool_trim(int device) {
   Pool& P = the_pool();
   std::vector<void*> blocks;
   {
     std::lock_guard<std::mutex> lk(P.mu);
-    for (auto& kv : P.cache[device]) blocks.push_back(kv.second);
+    for (auto& kv : P.cache[device]) blocks.push_back(kv.second.p);
     P.cache[device].clear();
     P.cached[device] = 0;
   }
@@ -228,10 +281,17 @@ struct btpm_solve {
   int with_craters = 0;
   float flux_scale = 1.0f;  // multiplies every absorbed flux of the shape
   bool queued = false;      // handed to a server (not yet admitted)
+  bool mem_waited = false;  // admission waited for device memory
   // 0 created, 1 smooth running, 2 crater running, 3 done,
   // 4 smooth done and the crater set waiting for device memory
   std::atomic<int> phase{0};
   std::unique_ptr<SetHost> smooth, crater;
+  // warm start (the TI sweep's carried state): this solve's sets start from
+  // the final state of `warm`; `dependents` counts solves that will still
+  // copy this one's state (it cannot be destroyed before they have)
+  btpm_solve* warm = nullptr;
+  bool warm_pending = false;  // a copy from `warm` is still to be taken
+  std::atomic<int> dependents{0};
   btpm_solve_info info{};
   std::chrono::steady_clock::time_point t_submit;
   std::mutex mu;
@@ -269,6 +329,34 @@ void coefficients(const btpm_shape* sh, const btpm_solve* s, std::vector<float>&
     m1[f] = (float)(ti * std::sqrt(2.0 * kPi * n / sh->p.rot_period) / dz);
   }
   *inst = all_zero;
+}
+
+// Warm start.  A predecessor whose smooth set ran at TI 0 everywhere carries
+// no depth profile (instantaneous equilibrium never integrates one): the
+// successor then starts from the reference initial state, as the reference
+// initialises temperatures at the first TI > 0 of its sweep
+// (T3905/main.cpp:2503-2515).
+const SetHost* warm_smooth_source(const btpm_solve* s) {
+  const btpm_solve* w = s->warm;
+  if (!s->warm_pending || !w || !w->smooth || !w->smooth->T.p || !w->smooth->prof.p) return nullptr;
+  if (w->smooth->d.inst) return nullptr;
+  return w->smooth.get();
+}
+// the crater set takes the predecessor's crater state iff both carry craters
+bool crater_takes_warm(const btpm_solve* s) {
+  return s->warm_pending && s->warm && s->with_craters && s->warm->with_craters;
+}
+const SetHost* warm_crater_source(const btpm_solve* s) {
+  if (!crater_takes_warm(s)) return nullptr;
+  const btpm_solve* w = s->warm;
+  if (!w->crater || !w->crater->T.p || !w->crater->prof.p || w->crater->d.inst) return nullptr;
+  return w->crater.get();
+}
+// the copies are enqueued on the context's stream before the predecessor
+// can be destroyed (its destroy synchronises that stream first)
+void release_warm(btpm_solve* s, hipStream_t) {
+  s->warm_pending = false;
+  if (s->warm) s->warm->dependents--;
 }
 
 // allocate and initialise the smooth column set of a solve
@@ -316,9 +404,20 @@ void build_smooth(btpm_solve* s, hipStream_t st) {
   // src = (1-A) F
   hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st, S->src.as<float>(),
                      sh->F.as<float>(), sh->one_minus_a.as<float>(), s->flux_scale, d.ncols, nser);
-  hipLaunchKernelGGL(btpm_init_smooth, dim3(grid1(d.ncols, 256)), dim3(256), 0, st, d);
+  const SetHost* W = warm_smooth_source(s);
+  if (W) {
+    // the previous solve's final depth profiles and last-rotation surface
+    // temperatures; emission and freeze state derived as at initialisation
+    HIPCHK(hipMemcpyAsync(S->T.p, W->T.p, nser * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(S->prof.p, W->prof.p, (size_t)BTPM_PSTRIDE * d.ncols * sizeof(float),
+                          hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(btpm_warm_smooth, dim3(grid1(d.ncols, 256)), dim3(256), 0, st, d);
+  } else {
+    hipLaunchKernelGGL(btpm_init_smooth, dim3(grid1(d.ncols, 256)), dim3(256), 0, st, d);
+  }
   HIPCHK(hipGetLastError());
   s->smooth = std::move(S);
+  if (s->warm_pending && !crater_takes_warm(s)) release_warm(s, st);
 }
 
 // RGH from the converged smooth temperatures, crater source and initial state
@@ -380,10 +479,28 @@ void build_crater(btpm_solve* s, hipStream_t st) {
                        d.ncols, nser);
     HIPCHK(hipStreamSynchronize(st));
   }
-  hipLaunchKernelGGL(btpm_init_crater, dim3(d.ngroups), dim3(128), 0, st, d,
-                     sh->cvf32.as<float>());
+  const SetHost* W = warm_crater_source(s);
+  if (W) {
+    // the previous solve's final crater profiles and surface temperatures
+    HIPCHK(hipMemcpyAsync(S->T.p, W->T.p, nser * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(S->prof.p, W->prof.p, (size_t)BTPM_PSTRIDE * d.ncols * sizeof(float),
+                          hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemsetAsync(S->frozen.p, 0, S->frozen.n, st));
+    HIPCHK(hipMemsetAsync(S->dmax.p, 0, S->dmax.n, st));
+  } else if (d.inst) {
+    // TI = 0 starts from zero temperatures (roughness_zero_temperatures,
+    // T175706/thermal_GPU.cu:348-367, chosen at thermal.cpp:3567-3574)
+    HIPCHK(hipMemsetAsync(S->T.p, 0, S->T.n, st));
+    HIPCHK(hipMemsetAsync(S->prof.p, 0, S->prof.n, st));
+    HIPCHK(hipMemsetAsync(S->frozen.p, 0, S->frozen.n, st));
+    HIPCHK(hipMemsetAsync(S->dmax.p, 0, S->dmax.n, st));
+  } else {
+    hipLaunchKernelGGL(btpm_init_crater, dim3(d.ngroups), dim3(128), 0, st, d,
+                       sh->cvf32.as<float>());
+  }
   HIPCHK(hipGetLastError());
   s->crater = std::move(S);
+  if (crater_takes_warm(s)) release_warm(s, st);
 }
 
 // smooth set no longer needed for stepping: keep T (results) and prof
@@ -641,13 +758,47 @@ struct Sched {
     if (active.empty()) return true;  // always let one in
     const size_t need = smooth_bytes(s->shape) + (s->with_craters ? crater_bytes(s->shape) : 0);
     const size_t fr = free_bytes();
-    return fr > headroom + reserved && need <= fr - headroom - reserved;
+    if (fr > headroom + reserved && need <= fr - headroom - reserved) return true;
+    if (!s->mem_waited) {
+      s->mem_waited = true;
+      stats.mem_waits++;
+    }
+    return false;
+  }
+
+  // A warm-started solve waits for its predecessor: the smooth set for the
+  // predecessor's smooth phase, the crater set for the whole predecessor.
+  static bool smooth_ready(const btpm_solve* s) {
+    if (!s->warm_pending || !s->warm) return true;
+    const int ph = s->warm->phase;
+    return ph == 2 || ph == 3 || ph == 4;
+  }
+  static bool crater_ready(const btpm_solve* s) {
+    return !crater_takes_warm(s) || s->warm->phase == 3;
+  }
+
+  // the injected out-of-memory of the BTPM_INJECT_NOMEM test hook (only
+  // while another solve runs, which guarantees a retry)
+  int builds = 0;
+  const int inject_k = inject_period();
+  void maybe_inject(bool running) {
+    const int k = inject_k;
+    if (k > 0 && running && ++builds % k == 0)
+      throw HipError{fail(BTPM_ERR_NOMEM, "injected out-of-memory (BTPM_INJECT_NOMEM=%d)", k)};
   }
 
   void admit() {
-    while (!pending.empty() && can_admit(pending.front())) {
-      btpm_solve* s = pending.front();
+    // first come, first served among the solves that are ready; a ready
+    // solve that does not fit stops admission until memory frees up
+    for (auto it = pending.begin(); it != pending.end();) {
+      btpm_solve* s = *it;
+      if (!smooth_ready(s)) {
+        ++it;
+        continue;
+      }
+      if (!can_admit(s)) break;
       try {
+        maybe_inject(!active.empty());
         build_smooth(s, ctx->stream);
       } catch (const HipError& e) {
         // the footprint estimate was short: wait for running solves to finish
@@ -655,7 +806,7 @@ struct Sched {
         stats.deferrals++;
         break;
       }
-      pending.pop_front();
+      it = pending.erase(it);
       if (trace_on())
         std::fprintf(stderr, "[btpm] admit %p nf %d craters %d: free %.1f GB, reserved %.1f GB, "
                      "headroom %.1f GB, crater need %.1f GB, active %zu\n", (void*)s,
@@ -675,9 +826,11 @@ struct Sched {
   // Out of memory while other solves still run is not an error: the solve
   // waits and is retried after the next chunk.
   bool try_build_crater(btpm_solve* s) {
+    if (!crater_ready(s)) return false;
     bool running = false;
     for (btpm_solve* o : active) running = running || o->phase == 1 || o->phase == 2;
     try {
+      maybe_inject(running);
       build_crater(s, ctx->stream);
     } catch (const HipError& e) {
       if (e.code == BTPM_ERR_NOMEM && !running)
@@ -1125,7 +1278,12 @@ void par_memcpy(char* d, const char* s, size_t n) {
   std::vector<std::thread> pool;
   for (int t = 1; t < nt; ++t) {
     const size_t a = std::min(n, (size_t)t * part), b = std::min(n, a + part);
-    if (a < b) pool.emplace_back([=] { std::memcpy(d + a, s + a, b - a); });
+    if (a >= b) continue;
+    try {
+      pool.emplace_back([=] { std::memcpy(d + a, s + a, b - a); });
+    } catch (const std::system_error&) {
+      std::memcpy(d + a, s + a, b - a);  // no thread to spare: copy here
+    }
   }
   std::memcpy(d, s, std::min(n, part));
   for (auto& th : pool) th.join();
@@ -1588,10 +1746,41 @@ int btpm_solve_set_flux_scale(btpm_solve* s, float scale) {
 int btpm_solve_destroy(btpm_solve* s) {
   if (!s) return BTPM_OK;
   CtxLock lk(s->ctx);
+  if (s->dependents > 0)
+    return fail(BTPM_ERR_STATE, "the solve is the warm start of %d solve(s) that have not taken "
+                "its state yet", s->dependents.load());
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->warm_pending && s->warm) s->warm->dependents--;
   delete s;
   return BTPM_OK;
+}
+
+int btpm_solve_warm_start(btpm_solve* s, btpm_solve* from) {
+  if (!s || !from) return fail(BTPM_ERR_ARG, "null argument");
+  if (s == from) return fail(BTPM_ERR_ARG, "a solve cannot start from itself");
+  if (s->shape != from->shape)
+    return fail(BTPM_ERR_ARG, "warm start needs the same shape (state layouts differ)");
+  CtxLock lk(s->ctx);
+  if (s->phase != 0 || s->queued) return fail(BTPM_ERR_STATE, "solve already started");
+  for (const btpm_solve* w = from; w; w = w->warm_pending ? w->warm : nullptr)
+    if (w == s) return fail(BTPM_ERR_ARG, "warm-start chain would form a cycle");
+  if (s->warm_pending && s->warm) s->warm->dependents--;
+  s->warm = from;
+  s->warm_pending = true;
+  from->dependents++;
+  return BTPM_OK;
+}
+
+int btpm_ctx_set_mem_limit(btpm_ctx* ctx, size_t bytes) {
+  if (!ctx) return fail(BTPM_ERR_ARG, "null ctx");
+  CtxLock lk(ctx);
+  API_BEGIN
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  pool_trim(ctx->device);  // the cap counts cached blocks too: start from none
+  pool_set_limit(ctx->device, bytes);
+  API_END
 }
 
 int btpm_batch_run(btpm_ctx* ctx, btpm_solve** solves, int n, const btpm_batch_opts* opts,
@@ -1600,6 +1789,16 @@ int btpm_batch_run(btpm_ctx* ctx, btpm_solve** solves, int n, const btpm_batch_o
   for (int i = 0; i < n; ++i)
     if (!solves[i] || solves[i]->phase != 0 || solves[i]->queued)
       return fail(BTPM_ERR_STATE, "solve %d not fresh", i);
+  {
+    // every warm-start predecessor has finished or runs in this batch
+    std::set<const btpm_solve*> in(solves, solves + n);
+    for (int i = 0; i < n; ++i) {
+      const btpm_solve* w = solves[i]->warm_pending ? solves[i]->warm : nullptr;
+      if (w && w->phase != 3 && !in.count(w))
+        return fail(BTPM_ERR_STATE, "solve %d starts from a solve that neither finished nor runs "
+                    "in this batch", i);
+    }
+  }
   CtxLock lk(ctx);
   if (ctx->servers > 0) return fail(BTPM_ERR_STATE, "a server is running on this context");
   API_BEGIN
@@ -1693,6 +1892,8 @@ void run_stepwise_set(btpm_solve* s, SetHost* S, int cap, int* rev_out, int* con
 int btpm_solve_run_stepwise(btpm_solve* s, int rotation_cap) {
   if (!s) return fail(BTPM_ERR_ARG, "null solve");
   if (s->phase != 0 || s->queued) return fail(BTPM_ERR_STATE, "solve not fresh");
+  if (s->warm_pending && s->warm && s->warm->phase != 3)
+    return fail(BTPM_ERR_STATE, "the warm-start predecessor has not finished");
   btpm_ctx* ctx = s->ctx;
   CtxLock lk(ctx);
   if (ctx->servers > 0) return fail(BTPM_ERR_STATE, "a server is running on this context");
@@ -1897,12 +2098,35 @@ struct btpm_server {
   btpm_batch_stats stats{};
 };
 
+// Records the error that ended the server loop and wakes every waiter
+// (btpm_server_wait's predicate sees the error).
+static void server_failed(btpm_server* srv, Sched* sc, int code) {
+  srv->errmsg = last_error();
+  srv->err = code;  // after errmsg: a waiter that sees err reads a complete message
+  std::vector<btpm_solve*> all;
+  if (sc) {
+    all.insert(all.end(), sc->active.begin(), sc->active.end());
+    all.insert(all.end(), sc->pending.begin(), sc->pending.end());
+  }
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    all.insert(all.end(), srv->inbox.begin(), srv->inbox.end());
+  }
+  for (btpm_solve* s : all) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cv.notify_all();
+  }
+  std::lock_guard<std::mutex> lk(srv->done_mu);
+  srv->done_cv.notify_all();
+}
+
 static void server_loop(btpm_server* srv) {
   btpm_ctx* ctx = srv->ctx;
   std::unique_ptr<Sched> sc;
   try {
     {
       std::lock_guard<std::mutex> glk(ctx->mu);
+      StreamScope scope(ctx->stream);
       HIPCHK(hipSetDevice(ctx->device));
       sc = std::make_unique<Sched>(ctx, srv->opt);
       sc->on_complete = [srv] {
@@ -1926,32 +2150,21 @@ static void server_loop(btpm_server* srv) {
       // calls waiting for it (results, destroys) go first
       while (ctx->waiting.load() > 0) std::this_thread::yield();
       std::lock_guard<std::mutex> glk(ctx->mu);
+      StreamScope scope(ctx->stream);
       sc->step();
     }
     sc->stats.wall_s =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - sc->t0).count();
     srv->stats = sc->stats;
   } catch (const HipError& e) {
-    srv->errmsg = last_error();
-    srv->err = e.code;  // after errmsg: a waiter that sees err reads a complete message
-    // wake every waiter now (btpm_server_wait's predicate sees the error)
-    std::vector<btpm_solve*> all;
-    if (sc) {
-      all.insert(all.end(), sc->active.begin(), sc->active.end());
-      all.insert(all.end(), sc->pending.begin(), sc->pending.end());
-    }
-    {
-      std::lock_guard<std::mutex> lk(srv->mu);
-      all.insert(all.end(), srv->inbox.begin(), srv->inbox.end());
-    }
-    for (btpm_solve* s : all) {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->cv.notify_all();
-    }
-    std::lock_guard<std::mutex> lk(srv->done_mu);
-    srv->done_cv.notify_all();
+    server_failed(srv, sc.get(), e.code);
+  } catch (const std::bad_alloc&) {
+    server_failed(srv, sc.get(), fail(BTPM_ERR_NOMEM, "server: host allocation failed"));
+  } catch (const std::exception& e) {
+    server_failed(srv, sc.get(), fail(BTPM_ERR_STATE, "server: internal error: %s", e.what()));
   }
   std::lock_guard<std::mutex> glk(ctx->mu);
+  StreamScope scope(ctx->stream);
   sc.reset();
   ctx->servers--;
 }
@@ -1977,6 +2190,8 @@ int btpm_server_start(btpm_ctx* ctx, const btpm_batch_opts* opts, btpm_server** 
 int btpm_server_submit(btpm_server* srv, btpm_solve* s) {
   if (!srv || !s) return fail(BTPM_ERR_ARG, "null argument");
   if (s->phase != 0 || s->queued) return fail(BTPM_ERR_STATE, "solve already used");
+  if (s->warm_pending && s->warm && s->warm->phase != 3 && !s->warm->queued)
+    return fail(BTPM_ERR_STATE, "submit the warm-start predecessor first");
   {
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return fail(BTPM_ERR_STATE, "server stopping");

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <atomic>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <string>
@@ -45,6 +46,9 @@ struct HipError {
     }                                                                               \
   } while (0)
 
+// Every exception stops at the C ABI: HIP and argument failures keep their
+// code, host allocation failures are BTPM_ERR_NOMEM and any other C++
+// exception (e.g. a thread the runtime could not start) is BTPM_ERR_STATE.
 #define API_BEGIN try {
 #define API_END                                              \
   }                                                          \
@@ -53,6 +57,9 @@ struct HipError {
   }                                                          \
   catch (const std::bad_alloc&) {                            \
     return ::btpm::fail(BTPM_ERR_NOMEM, "host allocation failed"); \
+  }                                                          \
+  catch (const std::exception& e) {                          \
+    return ::btpm::fail(BTPM_ERR_STATE, "internal error: %s", e.what()); \
   }                                                          \
   return BTPM_OK;
 
@@ -64,12 +71,33 @@ inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b)
 // are handed out again best-fit (capacity within 1.25x of the request):
 // returning tens of GB to the driver and allocating them again can stall a
 // single hipMalloc for seconds.  A failed hipMalloc empties the device's
-// cache and retries once.  Every release in the library happens after the
-// owning stream has been synchronised, so a cached block is idle.
+// cache and retries once.
+// A block released while work on its stream may still use it (a copy in
+// flight, or an error unwinding an API call) is tagged with the stream of
+// the releasing thread's scope (StreamScope; CtxLock sets one) and is handed
+// out again only to that stream, where stream order puts the new user's
+// work after the old.  Blocks released outside any scope (after a
+// synchronisation) are idle and go to any stream.
+// An optional per-device limit caps the bytes the library holds (live plus
+// cached): an allocation beyond it fails as if the device were full.
 void* pool_alloc(size_t bytes);   // throws HipError
 void pool_free(void* p);
 size_t pool_cached(int device);   // bytes held in the cache
+size_t pool_live(int device);     // bytes handed out and not yet released
 void pool_trim(int device);       // return the device's cache to the driver
+void pool_set_limit(int device, size_t bytes);  // 0 = no limit
+size_t pool_limit(int device);
+
+// the stream that the calling thread's library work is enqueued on (null
+// outside any scope)
+hipStream_t& scope_stream();
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(scope_stream()) { scope_stream() = s; }
+  ~StreamScope() { scope_stream() = prev; }
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
+};
 
 struct DBuf {
   void* p = nullptr;
@@ -152,14 +180,21 @@ void upload_big(btpm_ctx* ctx, DBuf& b, const T* h, size_t count) {
 
 // Takes ctx->mu for one API call, announcing itself so that a running
 // server lets it in between two scheduling steps (std::mutex is not fair).
+// Buffers released inside the call are tagged with the context's stream.
 struct CtxLock {
   btpm_ctx* c;
+  hipStream_t prev;
   explicit CtxLock(btpm_ctx* ctx) : c(ctx) {
     c->waiting++;
     c->mu.lock();
     c->waiting--;
+    prev = btpm::scope_stream();
+    btpm::scope_stream() = c->stream;
   }
-  ~CtxLock() { c->mu.unlock(); }
+  ~CtxLock() {
+    btpm::scope_stream() = prev;
+    c->mu.unlock();
+  }
   CtxLock(const CtxLock&) = delete;
   CtxLock& operator=(const CtxLock&) = delete;
 };

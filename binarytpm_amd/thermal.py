@@ -16,8 +16,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (NS, NX, BatchOpts, BatchStats, Body, Crater, CraterSunIn, DrfIn, Geometry,
-                   ObserveIn, Params, SolveInfo, VfMesh, check, lib)
+from ._lib import (NS, NX, BatchOpts, BatchStats, Body, BtpmError, Crater, CraterSunIn, DrfIn,
+                   Geometry, ObserveIn, Params, SolveInfo, VfMesh, check, lib)
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -65,6 +65,10 @@ class Context:
         fr, tot = C.c_size_t(), C.c_size_t()
         check(lib().btpm_ctx_mem_info(self._h, C.byref(fr), C.byref(tot)))
         return fr.value, tot.value
+
+    def set_mem_limit(self, nbytes: int):
+        """Cap the device memory the library holds on this device (0: none)."""
+        check(lib().btpm_ctx_set_mem_limit(self._h, int(nbytes)))
 
     def close(self):
         if self._h:
@@ -186,18 +190,23 @@ class Solve:
     """One thermal-inertia solve of a shape (smooth facets, then craters)."""
 
     def __init__(self, ctx: Context, shape: Shape, ti: float, ti_per_facet=None,
-                 with_craters: bool = False, flux_scale: float = 1.0):
+                 with_craters: bool = False, flux_scale: float = 1.0,
+                 warm_from: Optional["Solve"] = None):
         tpf = None if ti_per_facet is None else _f32(ti_per_facet, (shape.nf,))
         h = C.c_void_p()
         check(lib().btpm_solve_create(ctx.handle, shape.handle, float(ti), _ptr(tpf),
                                       int(bool(with_craters)), C.byref(h)))
         self._h = h
+        rc = 0
         if flux_scale != 1.0:
             rc = lib().btpm_solve_set_flux_scale(h, float(flux_scale))
-            if rc != 0:
-                lib().btpm_solve_destroy(h)
-                self._h = None
-                check(rc)
+        if rc == 0 and warm_from is not None:
+            rc = lib().btpm_solve_warm_start(h, warm_from.handle)
+        if rc != 0:
+            msg = lib().btpm_last_error()
+            lib().btpm_solve_destroy(h)
+            self._h = None
+            raise BtpmError(rc, msg.decode() if msg else "")
         self.shape = shape
         self.ti = float(ti)
         self.with_craters = bool(with_craters)
@@ -251,8 +260,10 @@ class Solve:
         return out
 
     def close(self):
+        """Destroy the solve (an error while another solve still has to take
+        its state as a warm start; the handle stays valid then)."""
         if self._h:
-            lib().btpm_solve_destroy(self._h)
+            check(lib().btpm_solve_destroy(self._h))
             self._h = None
 
 
@@ -510,15 +521,6 @@ def observe(ctx: Context, equpos, facets, ntau: int, hel, obs, rot, visible, ste
     check(lib().btpm_observe(ctx.handle, C.byref(q), sm.ctypes.data_as(C.POINTER(C.c_double)),
                              ro.ctypes.data_as(C.POINTER(C.c_double))))
     return sm, ro
-
-
-def probe_valu(ctx: Context, op: str = "fma", iters: int = 8192) -> float:
-    """VALU issue ceiling (wave instructions per second) of one unpacked FP32
-    instruction kind: "fma", "add", "mul", "sqrt" or "iadd" (32-bit integer)."""
-    code = {"fma": 0, "add": 1, "mul": 2, "sqrt": 3, "iadd": 4}[op]
-    r = C.c_double()
-    check(lib().btpm_probe_valu(ctx.handle, code, int(iters), C.byref(r)))
-    return float(r.value)
 
 
 def crater_sunlight(ctx: Context, shadow, sun, crater_vtx, crater_facets, m: int = 10,

@@ -167,10 +167,13 @@ typedef struct {
     long long smooth_column_steps;
     long long crater_column_steps;
     int max_resident;            /* most solves resident at once           */
-    int deferrals;               /* set builds postponed for lack of memory */
+    int deferrals;               /* set builds that met an out-of-memory and
+                                    were retried after other solves ran   */
     int tune_rotations;          /* rotations run as timed tuning trials    */
     int tune_decisions;          /* size classes the autotuner decided      */
     long long fast_rotations;    /* extra fast-lane rotations enqueued      */
+    int mem_waits;               /* solves whose admission waited for device
+                                    memory (the HBM-footprint budget)      */
 } btpm_batch_stats;
 
 /* Illumination geometry and shadows of a (binary) shape over one cycle: the
@@ -210,6 +213,14 @@ int btpm_ctx_destroy(btpm_ctx *ctx);
  * library keeps cached for reuse (released buffers are not handed back to
  * the driver until the context is destroyed or an allocation fails).       */
 int btpm_ctx_mem_info(btpm_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+/* Caps the device memory the library holds on the context's device (live
+ * and cached buffers of every context there) at `bytes`; 0 removes the cap.
+ * An allocation beyond it fails as if the device were full, and batch /
+ * server admission budgets against it.  The call returns the library's
+ * cached (idle) blocks to the driver.  Lets a serving process share a GPU.
+ * No counterpart in the reference (which allocates whatever its compile-time
+ * sizes need).                                                              */
+int btpm_ctx_set_mem_limit(btpm_ctx *ctx, size_t bytes);
 
 /* ---- shapes (inputs shared by many solves) ---------------------------------
  * Uploads the body (and crater model), runs facet scattering, RGI and crater
@@ -242,6 +253,22 @@ int btpm_solve_destroy(btpm_solve *s);
  * the solar constant into its illumination (T3905/main.cpp:905); it lets
  * requests at different heliocentric distances share one resident shape.  */
 int btpm_solve_set_flux_scale(btpm_solve *s, float scale);
+/* Warm start: s starts from the final state of `from` (same shape) instead
+ * of the reference initial state -- the carried state of the reference's TI
+ * sweep, which initialises once and runs every later thermal inertia from
+ * the previous one's converged temperatures (175706/Run-04/thermal.cpp:
+ * 3554-3576, 3676-3700; 3905/Thermal/NominalPole/main.cpp:2478-2517).  The
+ * smooth set takes from's final depth profiles and last-rotation surface
+ * temperatures; the crater set takes from's crater state when both carry
+ * craters (otherwise it starts as usual).  A `from` at TI 0 everywhere
+ * carries no depth profile, so s then starts from the initial state, as the
+ * reference initialises at its first TI > 0.  Convergence status restarts
+ * (the rev > 5 gate counts from 0 again, reset_asteroid_status).
+ * `from` must have finished, or run before s in the same batch or server
+ * (s's smooth phase waits for from's smooth phase, its crater phase for the
+ * whole of from).  Until s has taken the copy, from cannot be destroyed
+ * (BTPM_ERR_STATE).  Call before s runs.                                   */
+int btpm_solve_warm_start(btpm_solve *s, btpm_solve *from);
 /* Runs one solve alone through the persistent kernels, checking
  * convergence on the host after every rotation (no batching, no graphs).   */
 int btpm_solve_run(btpm_solve *s);
@@ -295,13 +322,6 @@ int btpm_server_stop(btpm_server *srv, btpm_batch_stats *stats);
  * q[n][j] = sum_i VF(j->i) e[n][i] for n < ncol, computed with the same
  * fp16 hi/lo MFMA split; e is [ncol][100], q is [ncol][100].                */
 int btpm_crater_contract(btpm_ctx *ctx, const float *vf, const float *e, int ncol, float *q);
-/* VALU issue ceiling of the device: a calibration kernel of 8 independent
- * chains per lane of one unpacked 32-bit instruction kind (op 0 fma, 1 add,
- * 2 mul, 3 hardware sqrt, 4 integer add), iters iterations, 8 waves per
- * SIMD; returns the
- * wave instructions issued per second (the unit of the compute-bound
- * kernels' counter summaries).  No counterpart in the reference.           */
-int btpm_probe_valu(btpm_ctx *ctx, int op, int iters, double *wave_instr_per_s);
 
 /* Outputs [nf][ntau] (host memory): shadow (1 lit, 0 back-facing or shadowed
  * by a facet nearer the Sun, including the partner body: eclipses), illum =

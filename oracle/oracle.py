@@ -272,6 +272,63 @@ def crater_solve(p, vf, albedo, ti, illum, rgh_, numrevs=None, temp=None, stemp=
                        st.newton_iters, st.newton_capped, st.column_steps)
 
 
+def ti_sweep(p, F, albedo, tis, numrevs=None, csr=None, crater=None, es=None):
+    """The reference's thermal-inertia sweep with its carried state
+    (T175706/thermal.cpp:3554-3576 initialise once, :3676-3760 every TI from
+    the previous TI's final state; T3905/main.cpp:2478-2517 the same with the
+    initialisation at the first TI > 0): per TI the smooth facets to
+    convergence from the previous TI's final profiles and surface series,
+    then RGH from the converged smooth temperatures and the crater columns
+    from the previous TI's final crater state.  Convergence status restarts
+    at every TI (reset_asteroid_status), so the rev > 5 gate applies afresh.
+
+    The first TI starts from the initial state of a single solve: smooth
+    facets from smooth_init; craters from crater_init with the RGH of the
+    converged smooth temperatures (the reference builds its first crater
+    state from the RGH of the initial smooth temperatures; the converged
+    periodic state is the same), or from zero temperatures at TI 0
+    (roughness_zero_temperatures, T175706/thermal_GPU.cu:348-367).  A TI-0
+    smooth solve carries no depth profile, so the next TI > 0 starts from the
+    initial state again, as the reference initialises at its first TI > 0.
+
+    crater = dict(vf=[100*100], illum=[nf][ntau][100], drf=[nnz][100] or
+    None) with the crater csr equal to `csr`; es = eps*sigma for RGH.
+    Returns [(smooth SolveResult, crater SolveResult or None)] per TI."""
+    F = _d(F)
+    nf = F.shape[0]
+    es = p.eps * p.sigma if es is None else es
+    out = []
+    st_s = None            # carried smooth (temp, stemp), None = initial state
+    st_c = None            # carried crater (temp, stemp)
+    for ti in tis:
+        ti = float(ti)
+        if st_s is None:
+            sm = smooth_solve(p, F, albedo, ti, numrevs=numrevs, csr=csr)
+        else:
+            sm = smooth_solve(p, F, albedo, ti, numrevs=numrevs, csr=csr, temp=st_s[0],
+                              stemp=st_s[1])
+        st_s = None if ti == 0.0 else (sm.temp, sm.stemp)
+        cr = None
+        if crater is not None:
+            ns = crater["illum"].shape[2]
+            if crater.get("drf") is not None and csr is not None:
+                rgh_ = rgh(csr, crater["drf"], sm.stemp, es, nf)
+            else:
+                rgh_ = np.zeros_like(_d(crater["illum"]))
+            if st_c is not None:
+                tc, sc = st_c
+            elif ti == 0.0:
+                tc = np.zeros((nf, p.nx + 1, ns))
+                sc = np.zeros(crater["illum"].shape)
+            else:
+                tc = sc = None
+            cr = crater_solve(p, crater["vf"], albedo, ti, crater["illum"], rgh_,
+                              numrevs=numrevs, temp=tc, stemp=sc)
+            st_c = None if ti == 0.0 else (cr.temp, cr.stemp)
+        out.append((sm, cr))
+    return out
+
+
 # ---- refcpu (float OpenMP restatement, the timed CPU baseline) -------------
 
 def refcpu_threads():
@@ -280,7 +337,10 @@ def refcpu_threads():
 
 def refcpu_smooth_solve(F, albedo, ti, ntau, rot_period, tacc=0.01, numrevs=None, csr=None,
                         eps=0.95, sigma=5.67e-8, ir_abs=1.0, nx=56, max_rev=301,
-                        check_after=5, nthreads=0):
+                        check_after=5, nthreads=0, state=None):
+    """refcpu smooth solve.  state: None (initial state), or a dict with
+    float32 'temp' [nf][nx+1] and 'stemp' [nf][ntau] to start from (the TI
+    sweep's carried state); the dict is updated with the final state."""
     F = _f(F)
     nf = F.shape[0]
     albedo = _f(np.broadcast_to(albedo, (nf,)))
@@ -290,38 +350,53 @@ def refcpu_smooth_solve(F, albedo, ti, ntau, rot_period, tacc=0.01, numrevs=None
         rp, ci, vv = _i(csr[0]), _i(csr[1]), _f(csr[2])
     else:
         rp = ci = vv = None
-    out = np.zeros((nf, ntau), dtype=np.float32)
+    warm = state is not None and "temp" in state
+    out = np.ascontiguousarray(state["stemp"], np.float32).copy() if warm else \
+        np.zeros((nf, ntau), dtype=np.float32)
+    temp = np.ascontiguousarray(state["temp"], np.float32).copy() if warm else \
+        np.zeros((nf, nx + 1), dtype=np.float32)
     steps = C.c_long(0)
     fn = rclib().rc_smooth_solve
     fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                    C.c_float, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _ip, _ip, _fp, _fp,
-                   C.c_int, C.POINTER(C.c_long)]
+                   C.c_int, C.POINTER(C.c_long), _fp, C.c_int]
     fn.restype = C.c_int
     rev = fn(nf, ntau, nx, rot_period, eps, sigma, ir_abs, tacc, max_rev, check_after,
              _ptr(F, _fp), _ptr(albedo, _fp), _ptr(ti, _fp), _ptr(numrevs, _fp),
              _ptr(rp, _ip), _ptr(ci, _ip), _ptr(vv, _fp), _ptr(out, _fp), nthreads,
-             C.byref(steps))
+             C.byref(steps), _ptr(temp, _fp), int(warm))
+    if state is not None:
+        state["temp"], state["stemp"] = temp, out.copy()
     return out, rev, steps.value
 
 
 def refcpu_crater_solve(vf, albedo, ti, illum, rgh_, rot_period, tacc=0.01, numrevs=None,
                         eps=0.95, sigma=5.67e-8, ir_abs=1.0, nx=56, max_rev=301,
-                        check_after=5, nthreads=0):
+                        check_after=5, nthreads=0, state=None):
+    """refcpu crater solve; state as for refcpu_smooth_solve ('temp'
+    [nfac][nx+1][ns], 'stemp' [nfac][ntau][ns])."""
     illum, rgh_, vf = _f(illum), _f(rgh_), _f(vf)
     nfac, nt, ns = illum.shape
     albedo = _f(np.broadcast_to(albedo, (nfac,)))
     ti = _f(np.broadcast_to(ti, (nfac,)))
     numrevs = _f(np.ones(nfac) if numrevs is None else np.broadcast_to(numrevs, (nfac,)))
-    out = np.zeros((nfac, nt, ns), dtype=np.float32)
+    warm = state is not None and "temp" in state
+    out = np.ascontiguousarray(state["stemp"], np.float32).copy() if warm else \
+        np.zeros((nfac, nt, ns), dtype=np.float32)
+    temp = np.ascontiguousarray(state["temp"], np.float32).copy() if warm else \
+        np.zeros((nfac, nx + 1, ns), dtype=np.float32)
     steps = C.c_long(0)
     fn = rclib().rc_crater_solve
     fn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                    C.c_float, C.c_float, C.c_int, C.c_int, _fp, _fp, _fp, _fp, _fp, _fp, _fp,
-                   C.c_int, C.POINTER(C.c_long)]
+                   C.c_int, C.POINTER(C.c_long), _fp, C.c_int]
     fn.restype = C.c_int
     rev = fn(nfac, ns, nt, nx, rot_period, eps, sigma, ir_abs, tacc, max_rev, check_after,
              _ptr(vf, _fp), _ptr(albedo, _fp), _ptr(ti, _fp), _ptr(numrevs, _fp),
-             _ptr(illum, _fp), _ptr(rgh_, _fp), _ptr(out, _fp), nthreads, C.byref(steps))
+             _ptr(illum, _fp), _ptr(rgh_, _fp), _ptr(out, _fp), nthreads, C.byref(steps),
+             _ptr(temp, _fp), int(warm))
+    if state is not None:
+        state["temp"], state["stemp"] = temp, out.copy()
     return out, rev, steps.value
 
 

@@ -64,15 +64,18 @@ int rc_max_threads(void)
 #endif
 }
 
-/* Smooth solve from the reference initial state.  Arrays follow the oracle
- * layouts; stemp_out[nf][ntau] receives the final surface temperatures.
- * Returns the number of rotations.                                          */
+/* Smooth solve.  Arrays follow the oracle layouts; stemp_out[nf][ntau]
+ * receives the final surface temperatures.  With warm = 0 it starts from the
+ * reference initial state; with warm = 1 from temp_io[nf][nx+1] and the
+ * surface series already in stemp_out (the TI sweep's carried state,
+ * T175706/thermal.cpp:3676-3700).  temp_io (may be NULL) receives the final
+ * depth profiles.  Returns the number of rotations.                         */
 int rc_smooth_solve(int nf, int ntau, int nx, float rot_period, float eps,
                     float sigma, float ir_abs, float tacc, int max_rev,
                     int check_after, const float *F, const float *albedo,
                     const float *ti, const float *numrevs, const int *rowptr,
                     const int *colidx, const float *vf, float *stemp_out,
-                    int nthreads, long *column_steps)
+                    int nthreads, long *column_steps, float *temp_io, int warm)
 {
     rc_set_threads(nthreads);
     rc_facet *A = (rc_facet *)calloc((size_t)nf, sizeof(rc_facet));
@@ -97,6 +100,12 @@ int rc_smooth_solve(int nf, int ntau, int nx, float rot_period, float eps,
 #pragma omp parallel for schedule(static)
     for (int f = 0; f < nf; ++f) {
         rc_facet *C = A + f;
+        if (warm) {
+            memcpy(C->temp, temp_io + (long)f * (nx + 1), sizeof(float) * (size_t)(nx + 1));
+            memcpy(C->stemp1, stemp_out + (long)f * ntau, sizeof(float) * (size_t)ntau);
+            memcpy(C->stemp0, C->stemp1, sizeof(float) * (size_t)ntau);
+            continue;
+        }
         float t = 0.0f;
         for (int s = 0; s < ntau; ++s) t = t + ((1.0f / ntau) * C->illum[s]);
         t = powf((1.0f - C->albedo) * t / (eps * sigma), 0.25f);
@@ -161,6 +170,7 @@ int rc_smooth_solve(int nf, int ntau, int nx, float rot_period, float eps,
     }
     for (int f = 0; f < nf; ++f) {
         memcpy(stemp_out + (long)f * ntau, A[f].stemp1, sizeof(float) * (size_t)ntau);
+        if (temp_io) memcpy(temp_io + (long)f * (nx + 1), A[f].temp, sizeof(float) * (size_t)(nx + 1));
         free(A[f].temp);
         free(A[f].stemp0);
         free(A[f].stemp1);
@@ -172,26 +182,35 @@ int rc_smooth_solve(int nf, int ntau, int nx, float rot_period, float eps,
 }
 
 /* Crater solve: nfac parent facets x ns sub-facets, layouts of the oracle
- * ([f][s][j], vf[i*ns+j] = VF(j->i)).  Initial state by the two-pass
- * roughness_initialise_temperatures.  stemp_out[nfac][ntau][ns].            */
+ * ([f][s][j], vf[i*ns+j] = VF(j->i)).  stemp_out[nfac][ntau][ns].  With
+ * warm = 0 the initial state is the two-pass
+ * roughness_initialise_temperatures; with warm = 1 it is temp_io
+ * [nfac][nx+1][ns] and the series already in stemp_out (the carried state of
+ * the TI sweep).  temp_io (may be NULL) receives the final profiles.        */
 int rc_crater_solve(int nfac, int ns, int ntau, int nx, float rot_period,
                     float eps, float sigma, float ir_abs, float tacc,
                     int max_rev, int check_after, const float *vf,
                     const float *albedo, const float *ti, const float *numrevs,
                     const float *illum, const float *rgh, float *stemp_out,
-                    int nthreads, long *column_steps)
+                    int nthreads, long *column_steps, float *temp_io, int warm)
 {
     rc_set_threads(nthreads);
     const float EPSSIGMA = eps * sigma;
     const size_t per_f = (size_t)ntau * ns;
-    float *temp = (float *)malloc(sizeof(float) * (size_t)nfac * (nx + 1) * ns);
+    const size_t ntemp = (size_t)nfac * (nx + 1) * ns;
+    float *temp = (float *)malloc(sizeof(float) * ntemp);
     float *stemp0 = (float *)malloc(sizeof(float) * (size_t)nfac * per_f);
     float *stemp1 = stemp_out;
     int *conv = (int *)calloc((size_t)nfac, sizeof(int));
 
+    if (warm) {
+        memcpy(temp, temp_io, sizeof(float) * ntemp);
+        memcpy(stemp0, stemp1, sizeof(float) * (size_t)nfac * per_f);
+    }
     /* roughness_initialise_temperatures, T175706/thermal_GPU.cu:374-423 */
 #pragma omp parallel for schedule(dynamic)
     for (int f = 0; f < nfac; ++f) {
+        if (warm) continue;
         float *st0 = (float *)malloc(sizeof(float) * (size_t)ns);
         for (int j = 0; j < ns; ++j) {
             float t = 0.0f;
@@ -285,6 +304,7 @@ int rc_crater_solve(int nfac, int ns, int ntau, int nx, float rot_period,
         }
         ++rev;
     }
+    if (temp_io) memcpy(temp_io, temp, sizeof(float) * ntemp);
     free(temp);
     free(stemp0);
     free(conv);

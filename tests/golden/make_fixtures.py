@@ -8,7 +8,8 @@ at REF:
 Outputs (committed): tests/golden/new_crater2.npz,
 tests/golden/doppler_binary.npz, tests/golden/doppler_binary_properties.json,
 tests/golden/fg3_observe.npz (WISE colour corrections and the 1996 FG3 mesh),
-tests/golden/fg3_flux_set4.npz (shipped 1996 FG3 light curves, set 4)
+tests/golden/fg3_flux_set4.npz (shipped 1996 FG3 light curves of set 4 at 10
+smooth and 6 rough thermal inertias)
 """
 import json
 import os
@@ -40,11 +41,13 @@ def main():
     V, F = formats.read_shape(os.path.join(R, "1996FG3_binary_lowres.txt"), 2052, 4096)
     np.savez_compressed(os.path.join(HERE, "fg3_observe.npz"), colour=cc, V=V,
                         F=F.astype(np.int32))
-    # the shipped smooth and rough light curves of observation set 4 at three
-    # thermal inertias (Observe's output files)
+    # the shipped light curves of observation set 4 (Observe's output files):
+    # smooth at TI 0 and every 50 from 50 to 400 plus 10, rough at TI 0, 10,
+    # 100, 200, 300 and 400
     flux = {}
-    for ti in (10, 100, 300):
-        for kind in ("smooth", "rough"):
+    for kind, tis in (("smooth", (0, 10, 50, 100, 150, 200, 250, 300, 350, 400)),
+                      ("rough", (0, 10, 100, 200, 300, 400))):
+        for ti in tis:
             flux[f"{kind}_{ti}"] = np.loadtxt(
                 os.path.join(R, f"1996FG3_binary_lowres_{kind}_flux({ti}_4).txt")).astype(np.float32)
     np.savez_compressed(os.path.join(HERE, "fg3_flux_set4.npz"), **flux)

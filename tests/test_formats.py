@@ -90,3 +90,23 @@ def test_producer_files_layout(tmp_path):
     assert open(p + "_map.txt").read().splitlines()[2:4] == ["0\t1.5\t0", "0\t1\t2\t2"]
     formats.write_facet_map(p + "_tot.txt", V, F, np.array([0.5, 0, 0.25], np.float32))
     assert open(p + "_tot.txt").read().splitlines()[-1] == "1\t2\t0\t0.25"
+
+
+def test_text6_matches_printf():
+    """The vectorised 6-digit read-back of the light-curve regression equals
+    formatting each float with %g and parsing it, on random temperatures and
+    on the values next to every decade boundary."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "fg3reg", os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                               "test_gpu_regression_fg3.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    rng = np.random.default_rng(0)
+    edges = np.float32([0.01, 0.1, 1.0, 10.0, 100.0, 1000.0])
+    a = np.concatenate([rng.uniform(0, 600, 200_000), 10.0 ** rng.uniform(-3, 3, 50_000),
+                        np.nextafter(edges, np.float32(0)), np.nextafter(edges, np.float32(1e4)),
+                        edges, [0.0, 99.9999995, 999.9995, 123.4565]]).astype(np.float32)
+    slow = np.array([float(f"{x:g}") for x in a], np.float64)
+    assert np.array_equal(mod._text6(a), slow)

@@ -190,6 +190,86 @@ def test_compat_crater_loop_matches_oracle():
     L.freeup_GPU_globalheating_arrays()
 
 
+def test_compat_crater_inst_equilibrium_matches_oracle():
+    """TI = 0 through the reference's calls, in its order for
+    instant_equib_model = 1 (175706/Run-04/thermal.cpp:3567-3574, 3736-3745):
+    load_GPU_globalheating, GPU_zero_temperatures, then GPU_inst_equilibrium
+    rotations with GPU_converged after rev > 5; against the oracle's
+    instantaneous crater equilibrium from zero temperatures."""
+    L = _lib_compat()
+    case, crater = inputs.rough_primary(subdiv=1)          # 80 facets
+    nf, nt = case.illum.shape
+    p = O.params(nt, case.rot_period, tacc=0.01)
+    F = O.facet_scatter(case.illum, 0.1, case.csr, multiple=True, acc=1e-4)
+    rgi = O.rgi(case.csr, crater["drf"], F, np.full(nf, 0.1), nf).astype(np.float32)
+    sm = O.smooth_solve(p, F, 0.1, 0.0, csr=case.csr)
+    rgh = O.rgh(case.csr, crater["drf"], sm.stemp, ES, nf).astype(np.float32)
+    il = O.crater_scatter(crater["vf"], 0.1, crater["shadow"], crater["cosang"], rgi,
+                          crater["solar"], multiple=True, acc=1e-4)
+    ref = O.crater_solve(p, crater["vf"], 0.1, 0.0, il, rgh, temp=np.zeros((nf, 57, 100)),
+                         stemp=np.zeros(il.shape))
+
+    assert L.btpm_compat_configure(nf, nt, 56) == 0
+    L.set_cuda_device()
+    vf = np.ascontiguousarray(crater["vf"].reshape(100, 100), np.float32)
+    L.initialise_GPU_constants(np.float32(case.rot_period), _fp(vf), _surface(crater["geom"]))
+    L.initialise_GPU_facet_properties_arrays()
+    conv = np.zeros(nf, np.bool_)
+    alb = np.full(nf, 0.1, np.float32)
+    tis = np.zeros(nf, np.float32)
+    nrv = np.ones(nf, np.float32)
+    bp = conv.ctypes.data_as(C.POINTER(C.c_bool))
+    L.set_GPU_facet_properties(bp, _fp(alb), _fp(tis), _fp(nrv))
+    L.initialise_GPU_shadow_arrays()
+    L.initialise_GPU_globalillumination_arrays()
+    sh = np.ascontiguousarray(crater["shadow"], np.float32)
+    cs = np.ascontiguousarray(crater["cosang"], np.float32)
+    vec = np.zeros((nf, nt, 3), np.float32)
+    L.load_GPU_globalillumination(_fp(rgi))
+    L.initialise_GPU_illumination_arrays()
+    L.GPU_sfcpartialshadow(np.zeros((nf, nt), np.int32).ctypes.data_as(C.POINTER(C.c_int)),
+                           _fp(sh), _fp(vec), _fp(cs))
+    L.GPU_single_scattering(np.float32(crater["solar"]))
+    L.GPU_multiple_scattering(np.float32(crater["solar"]), np.float32(1e-4))
+    L.freeup_GPU_shadow_arrays()
+    L.freeup_GPU_globalillumination_arrays()
+    L.initialise_GPU_temperature_arrays()
+    L.initialise_GPU_globalheating_arrays()
+    L.load_GPU_globalheating(_fp(rgh))
+    L.GPU_zero_temperatures()
+    s0 = np.ones((nf, nt, 100), np.float32)
+    s1 = np.ones((nf, nt, 100), np.float32)
+    tp = np.ones((nf, 57, 100), np.float32)
+    L.obtain_GPU_roughness_temperatures(_fp(s0), _fp(s1), _fp(tp))
+    assert not s0.any() and not s1.any() and not tp.any()
+    rev = 0
+    done = False
+    while not done and rev < 301:
+        L.GPU_inst_equilibrium(EPS, SIGMA, IR_ABS)
+        if rev > 5:
+            L.GPU_converged(np.float32(0.01))
+            L.obtain_GPU_facet_properties(bp, _fp(alb), _fp(tis), _fp(nrv))
+            done = bool(conv.all())
+        rev += 1
+    L.obtain_GPU_roughness_temperatures(_fp(s0), _fp(s1), _fp(tp))
+    assert L.btpm_compat_status() == 0, _lib.lib().btpm_last_error()
+    d = np.abs(s1.astype(np.float64) - ref.stemp)
+    print(f"[parity] compat crater TI 0: rev {rev} (oracle {ref.rotations}), "
+          f"max|dT| {d.max():.5f} K rms {np.sqrt((d ** 2).mean()):.5f} K", flush=True)
+    assert done and abs(rev - ref.rotations) <= 1
+    assert d.max() <= 0.05
+    # radiative equilibrium with the previous rotation's crater self-heating,
+    # es T^4 = (1 - A) illum + RGH + sum_i VF(j->i) es T_prev,i^4, up to the
+    # GPU illumination's difference from the oracle's (< 0.05 W/m^2)
+    e_prev = ES * s0.astype(np.float64) ** 4
+    q = rgh + np.einsum("fsi,ij->fsj", e_prev, vf.astype(np.float64))
+    lhs = ES * s1.astype(np.float64) ** 4
+    rhs = 0.9 * il + q
+    assert (np.abs(lhs - rhs) - 1e-5 * rhs).max() < 0.06
+    L.freeup_GPU_temperature_arrays()
+    L.freeup_GPU_globalheating_arrays()
+
+
 def test_cli_file_contract(tmp_path):
     """btpm_thermal reads the reference's input files and writes its
     temperature files; their contents match the FP64 oracle."""

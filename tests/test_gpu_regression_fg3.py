@@ -1,9 +1,11 @@
 """End-to-end regression against the reference's own shipped results: the
 1996 FG3 light curves of observation set 4 (175706/Run-04/
 1996FG3_binary_lowres_{smooth,rough}_flux(TI_4).txt, fixture
-tests/golden/fg3_flux_set4.npz), regenerated with this library alone at the
-reference's configuration (175706/Run-04/thermal.cpp:17-80, visibility.cpp:
-18-49, observe_generator.cpp:18-45):
+tests/golden/fg3_flux_set4.npz: TI 0 and 10..400), regenerated with this
+library alone at the reference's configuration (175706/Run-04/thermal.cpp:
+17-80, visibility.cpp:18-49, observe_generator.cpp:18-45), through the
+reference's own TI sweep: TI 0 by instantaneous equilibrium, then TI 10, 20,
+..., 400 each warm-started from the previous one (thermal.cpp:3676-3760):
 
   illumination and eclipses  geometry producer (Sun at the heliocentric
                              position, 9:2 revolutions over 5850 steps)
@@ -54,8 +56,24 @@ def _check(name, ours, ref):
 
 
 def _text6(T):
-    """The temperatures as Observe reads them back from the 6-digit text file."""
-    return np.array([float(f"{x:g}") for x in T.reshape(-1)], np.float64).reshape(T.shape)
+    """The temperatures as Observe reads them back from the 6-digit text file
+    (ostream's default %g).  Vectorised: with X the decimal exponent of the
+    value rounded to 6 digits, x * 10^(5-X) is exact in double for a float x
+    (24 + 20 bits), so rint of it is the correctly rounded digit string and
+    k / 10^(5-X) the correctly rounded parse (test_text6_matches_printf)."""
+    x = np.asarray(T, np.float32).astype(np.float64)
+    out = np.zeros_like(x)
+    nz = x > 0
+    v = x[nz]
+    X = np.floor(np.log10(v))
+    k = np.rint(v * 10.0 ** (5 - X))
+    up = k >= 1e6                       # rounding carried into the next decade
+    X[up] += 1
+    k[up] = np.rint(v[up] * 10.0 ** (5 - X[up]))
+    d = 5 - X
+    r = np.where(d >= 0, k / 10.0 ** np.maximum(d, 0), k * 10.0 ** np.maximum(-d, 0))
+    out[nz] = r
+    return out
 
 
 @pytest.fixture(scope="module")
@@ -74,26 +92,49 @@ def fg3(ctx):
                 p=th.params(NTAU, period, tacc=0.001, eps=0.95))
 
 
+def _flux(fg3, ctx, stemp, crater=None):
+    return th.observe(ctx, fg3["V"], fg3["F"], NTAU, HEL, OBS, ROT, fg3["seen"].shadow,
+                      _text6(stemp), fg3["cc"], crater=crater, **KW)
+
+
 def test_fg3_smooth_light_curves(ctx, fg3):
+    """TI 0 (instantaneous equilibrium, Run-04 with instant_equib_model = 1)
+    and the reference's own sweep TI 10, 20, ..., 400 warm-started from one
+    TI to the next as single_model runs it (thermal.cpp:3676-3700), checked
+    at every TI whose light curve ships in the fixture."""
     ref = np.load(os.path.join(GOLD, "fg3_flux_set4.npz"))
     shape = th.Shape(ctx, fg3["p"], fg3["sun"].illum, 0.01, numrevs=fg3["numrevs"], scatter=0)
     try:
-        for ti in (10.0, 100.0, 300.0):
-            s = th.Solve(ctx, shape, ti)
-            th.batch_run(ctx, [s])
+        s = th.Solve(ctx, shape, 0.0).run()
+        sm, _ = _flux(fg3, ctx, s.surface_temps())
+        s.close()
+        _check("smooth TI 0 (instantaneous equilibrium)", sm, ref["smooth_0"].astype(np.float64))
+        prev, rot = None, []
+        t0 = time.time()
+        for ti in range(10, 410, 10):
+            s = th.Solve(ctx, shape, float(ti), warm_from=prev).run()
             inf = s.info()
-            sm, _ = th.observe(ctx, fg3["V"], fg3["F"], NTAU, HEL, OBS, ROT, fg3["seen"].shadow,
-                               _text6(s.surface_temps()), fg3["cc"], **KW)
-            s.close()
             assert inf.converged
-            _check(f"smooth TI {ti:g} ({inf.rotations} rotations)", sm,
-                   ref[f"smooth_{int(ti)}"].astype(np.float64))
+            rot.append(inf.rotations)
+            if prev is not None:
+                prev.close()
+            prev = s
+            if f"smooth_{ti}" in ref:
+                sm, _ = _flux(fg3, ctx, s.surface_temps())
+                _check(f"smooth TI {ti} (warm sweep, {inf.rotations} rotations)", sm,
+                       ref[f"smooth_{ti}"].astype(np.float64))
+        prev.close()
+        print(f"[regression] warm smooth sweep 10..400: {sum(rot)} rotations "
+              f"({time.time() - t0:.1f} s): {rot}", flush=True)
     finally:
         shape.close()
 
 
-def test_fg3_rough_light_curve(ctx, fg3):
-    """The crater path end to end: 4096 facets x 100 crater sub-facets."""
+def test_fg3_rough_light_curves(ctx, fg3):
+    """The crater path end to end (4096 facets x 100 crater sub-facets): TI 0
+    (zero temperatures, then instantaneous equilibrium of the craters) and
+    the warm-started sweep TI 10, 20, ..., 400 of single_model, each TI's
+    crater state carried to the next, checked at the TIs in the fixture."""
     ref = np.load(os.path.join(GOLD, "fg3_flux_set4.npz"))
     c = fg3["c"]
     cV, cF = c["V"].astype(np.float32), c["F"].astype(np.int32)
@@ -104,21 +145,34 @@ def test_fg3_rough_light_curve(ctx, fg3):
                         scatter_acc=1e-4)
     shape = th.Shape(ctx, fg3["p"], sun.illum, 0.01, numrevs=fg3["numrevs"], scatter=0, crater=cm)
     del csh, ccos, cm
+    cvis, _ = th.crater_sunlight(ctx, seen.shadow, seen.sun_facet, cV, cF)
+
+    def check(s, name, key):
+        sm, ro = _flux(fg3, ctx, s.surface_temps(), crater=(cV, cF, cvis, s.crater_temps()))
+        _check(f"rough {name}", ro, ref[f"rough_{key}"].astype(np.float64))
+        _check(f"smooth {name} (with the crater model)", sm, ref[f"smooth_{key}"].astype(np.float64))
+
     try:
-        s = th.Solve(ctx, shape, 100.0, with_craters=True)
-        t0 = time.time()
-        th.batch_run(ctx, [s])
+        s = th.Solve(ctx, shape, 0.0, with_craters=True).run()
         inf = s.info()
-        print(f"[regression] TI 100 solve: {inf.rotations} smooth and {inf.crater_rotations} "
-              f"crater rotations in {time.time() - t0:.1f} s", flush=True)
         assert inf.converged and inf.crater_converged
-        T6 = _text6(s.surface_temps())
-        CT = s.crater_temps()
+        check(s, f"TI 0 ({inf.crater_rotations} crater rotations)", 0)
         s.close()
-        cvis, _ = th.crater_sunlight(ctx, seen.shadow, seen.sun_facet, cV, cF)
-        sm, ro = th.observe(ctx, fg3["V"], fg3["F"], NTAU, HEL, OBS, ROT, seen.shadow, T6,
-                            fg3["cc"], crater=(cV, cF, cvis, CT), **KW)
-        _check("rough TI 100", ro, ref["rough_100"].astype(np.float64))
-        _check("smooth TI 100 (with the crater model)", sm, ref["smooth_100"].astype(np.float64))
+        prev, rot = None, []
+        t0 = time.time()
+        for ti in range(10, 410, 10):
+            s = th.Solve(ctx, shape, float(ti), with_craters=True, warm_from=prev).run()
+            inf = s.info()
+            assert inf.converged and inf.crater_converged
+            rot.append((inf.rotations, inf.crater_rotations))
+            if prev is not None:
+                prev.close()
+            prev = s
+            if f"rough_{ti}" in ref:
+                check(s, f"TI {ti} (warm sweep, {inf.rotations} smooth / {inf.crater_rotations} "
+                      "crater rotations)", ti)
+        prev.close()
+        print(f"[regression] warm rough sweep 10..400 in {time.time() - t0:.1f} s, rotations "
+              f"(smooth, crater): {rot}", flush=True)
     finally:
         shape.close()

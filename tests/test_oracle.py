@@ -337,6 +337,99 @@ def test_refcpu_baseline_agrees_with_oracle(small_body):
     assert abs(rev - r.rotations) <= 2
 
 
+# ---- the TI sweep's carried state (warm start) -----------------------------
+
+def _small_rough():
+    case, crater = inputs.rough_primary(subdiv=0, nfac=None, density=0.2)   # 20 facets
+    nf = case.illum.shape[0]
+    F = O.facet_scatter(case.illum, 0.1, case.csr, multiple=True, acc=1e-4)
+    rgi = O.rgi(case.csr, crater["drf"], F, np.full(nf, 0.1), nf)
+    il = O.crater_scatter(crater["vf"], 0.1, crater["shadow"], crater["cosang"], rgi,
+                          crater["solar"], multiple=True, acc=1e-4)
+    return case, dict(vf=crater["vf"], illum=il, drf=crater["drf"]), F
+
+
+def test_ti_sweep_first_solve_is_the_cold_solve():
+    """The sweep's first TI starts from the initial state: it equals the
+    single solve bit for bit, smooth facets and craters."""
+    case, cr, F = _small_rough()
+    nf = F.shape[0]
+    p = O.params(650, case.rot_period, tacc=0.01)
+    (sm, c), = O.ti_sweep(p, F, 0.1, [120.0], csr=case.csr, crater=cr)
+    cold = O.smooth_solve(p, F, 0.1, 120.0, csr=case.csr)
+    np.testing.assert_array_equal(sm.stemp, cold.stemp)
+    rgh = O.rgh(case.csr, cr["drf"], cold.stemp, ES, nf)
+    cc = O.crater_solve(p, cr["vf"], 0.1, 120.0, cr["illum"], rgh)
+    np.testing.assert_array_equal(c.stemp, cc.stemp)
+
+
+def test_ti_sweep_warm_states_reach_the_cold_periodic_state():
+    """The periodic state does not depend on where the iteration starts:
+    every warm-started TI of a chain matches its own cold solve.  The freeze
+    test stops a column once it changes by less than tacc per rotation, which
+    leaves it up to tacc / (1 - lambda) from the periodic state (lambda its
+    slowest decay per rotation), on the side it came from: 0.05 K for the
+    smooth facets here, 0.15 K for the slower craters.  And a carried state
+    is really used: starting from the state of the same TI converges at the
+    first allowed test (rev > 5, 7 rotations), moving less than tacc per
+    rotation."""
+    case, cr, F = _small_rough()
+    nf = F.shape[0]
+    p = O.params(650, case.rot_period, tacc=0.01)
+    tis = [40.0, 60.0, 90.0, 130.0]
+    chain = O.ti_sweep(p, F, 0.1, tis, csr=case.csr, crater=cr)
+    for ti, (sm, c) in zip(tis, chain):
+        cold = O.smooth_solve(p, F, 0.1, ti, csr=case.csr)
+        assert np.abs(sm.stemp - cold.stemp).max() < 0.05
+        rgh = O.rgh(case.csr, cr["drf"], cold.stemp, ES, nf)
+        cc = O.crater_solve(p, cr["vf"], 0.1, ti, cr["illum"], rgh)
+        assert np.abs(c.stemp - cc.stemp).max() < 0.15
+        if ti != tis[0]:                                 # the carried state is nearer
+            assert c.column_steps < cc.column_steps
+    sm, c = chain[-1]
+    again = O.ti_sweep(p, F, 0.1, [130.0, 130.0], csr=case.csr, crater=cr)
+    r2s, r2c = again[1]
+    assert r2s.rotations == p.check_after + 2 and r2c.rotations == p.check_after + 2
+    # each of those rotations moves the state by less than tacc (the decay
+    # that froze it continues)
+    assert np.abs(r2s.stemp - again[0][0].stemp).max() < (p.check_after + 2) * p.tacc
+    assert np.abs(r2c.stemp - again[0][1].stemp).max() < (p.check_after + 2) * p.tacc
+
+
+def test_ti_sweep_restarts_after_instantaneous_equilibrium():
+    """The 3905 order: TI 0 first (no depth profile; craters from zero
+    temperatures), then the first TI > 0 from the initial state, so it
+    equals its cold solve bit for bit."""
+    case, cr, F = _small_rough()
+    nf = F.shape[0]
+    p = O.params(650, case.rot_period, tacc=0.01)
+    (s0, c0), (s1, c1) = O.ti_sweep(p, F, 0.1, [0.0, 50.0], csr=case.csr, crater=cr)
+    inst = O.smooth_solve(p, F, 0.1, 0.0, csr=case.csr)
+    np.testing.assert_array_equal(s0.stemp, inst.stemp)
+    rgh0 = O.rgh(case.csr, cr["drf"], inst.stemp, ES, nf)
+    z = O.crater_solve(p, cr["vf"], 0.1, 0.0, cr["illum"], rgh0, temp=np.zeros((nf, 57, 100)),
+                       stemp=np.zeros(cr["illum"].shape))
+    np.testing.assert_array_equal(c0.stemp, z.stemp)
+    cold = O.smooth_solve(p, F, 0.1, 50.0, csr=case.csr)
+    np.testing.assert_array_equal(s1.stemp, cold.stemp)
+
+
+def test_refcpu_warm_sweep_agrees_with_oracle(small_body):
+    """refcpu's carried state (the timed CPU baseline of the warm sweep)
+    against the oracle's chained sweep: temperatures within the parity
+    tolerance, rotations within 2."""
+    case = small_body
+    p = O.params(650, case.rot_period, tacc=0.01)
+    tis = [80.0, 110.0, 150.0]
+    ref = O.ti_sweep(p, case.illum, 0.1, tis)
+    state = {}
+    for ti, (r, _) in zip(tis, ref):
+        st, rev, _ = O.refcpu_smooth_solve(case.illum, 0.1, ti, 650, case.rot_period, tacc=0.01,
+                                           state=state)
+        assert np.abs(st - r.stemp).max() < 0.05
+        assert abs(rev - r.rotations) <= 2
+
+
 # --------------------------------------------------------------------------
 # illumination geometry and shadows (geometry.c)
 # --------------------------------------------------------------------------
