@@ -175,6 +175,7 @@ PROTOTYPES = [
                                        C.POINTER(C.c_int)]),
     ("btpm_server_stop", C.c_int, [_vp, C.POINTER(BatchStats)]),
     ("btpm_crater_contract", C.c_int, [_vp, _fp, _fp, C.c_int, _fp]),
+    ("btpm_crater_contract_noalloc", C.c_int, [_vp, _vp, _vp, C.c_int, _vp, _vp]),
     ("btpm_geometry_illumination", C.c_int,
      [_vp, C.POINTER(Geometry), C.POINTER(C.c_int), _fp, _fp, _fp]),
     ("btpm_vf_build", C.c_int, [_vp, C.POINTER(VfMesh), C.POINTER(_vp)]),

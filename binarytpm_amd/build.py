@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["conduct.hip", "aux.hip", "stepwise.hip", "geometry.hip", "viewfactor.hip", "observe.hip",
-           "io.hip", "host.hip", "compat.hip"]
+           "io.hip", "host.hip", "compat.hip", "noalloc.hip"]
 LIB = os.path.join(HERE, "libbtpm.so")
 PROBE_SRC = os.path.join(HERE, "bench_tools", "probe.hip")
 PROBE_LIB = os.path.join(HERE, "libbtpm_probe.so")

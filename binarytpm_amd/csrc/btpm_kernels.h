@@ -24,7 +24,8 @@ extern template __global__ void btpm_conduct_smooth<64>(const btpm_tile*, const 
 extern "C" __global__ void btpm_conduct_crater(const btpm_tile* tiles, const btpm_set_dev* sets,
                                                const btpm_status_dev* status);
 extern "C" __global__ void btpm_contract_kernel(const _Float16* vfA, const float* e, float* q,
-                                                int ncol, float unscale);
+                                                int ncol, float unscale_v,
+                                                const float* unscale_p);
 
 // aux.hip
 extern "C" __global__ void btpm_pack_blocked(const float* raw, float* blk, const float* scale,

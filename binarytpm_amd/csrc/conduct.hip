@@ -681,14 +681,18 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   }
 }
 
-// Standalone crater self-heating product (tests and tools):
+// Standalone crater self-heating product (tests, tools and the
+// allocation-free entry point):
 //   q[n][j] = sum_i VF(j->i) e[n][i],  n < ncol, with the kernel's own split.
+// The power-of-two unscale comes by value, or from device memory when
+// unscale_p is set (tiles packed on the device).
 extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_contract_kernel(
     const _Float16* __restrict__ vfA, const float* __restrict__ e, float* __restrict__ q,
-    int ncol, float unscale) {
+    int ncol, float unscale_v, const float* __restrict__ unscale_p) {
   __shared__ _Float16 lds_a[2][kMP * kLdsK];
   __shared__ _Float16 lds_b[2][kNP * kLdsK];
   __shared__ float lds_q[BTPM_BLK * kQCols];
+  const float unscale = unscale_p != nullptr ? *unscale_p : unscale_v;
   const int tid = (int)threadIdx.x;
   const int n0 = blockIdx.x * kNP;
   stage_vf(vfA, lds_a, tid);

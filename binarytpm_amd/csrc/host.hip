@@ -194,6 +194,8 @@ std::string& last_error() {
   return e;
 }
 
+void noalloc_setup(int device);  // noalloc.hip
+
 float vf_tiles(const float* vf, std::vector<_Float16>& a) {
   float vmax = 0.0f;
   for (int e = 0; e < BTPM_NS * BTPM_NS; ++e) vmax = std::max(vmax, std::fabs(vf[e]));
@@ -1377,6 +1379,7 @@ int btpm_ctx_create(int device, btpm_ctx** out) {
   HIPCHK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(BTPM_ERR_ARG, "device %d of %d", device, n);
   HIPCHK(hipSetDevice(device));
+  noalloc_setup(device);  // this device's scratch of the allocation-free entry point
   auto c = std::make_unique<btpm_ctx>();
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -2071,7 +2074,8 @@ int btpm_crater_contract(btpm_ctx* ctx, const float* vf, const float* e, int nco
   dQ.alloc((size_t)ncol * BTPM_NS * 4);
   const int per = BTPM_CRATERS_WG * BTPM_BLK;  // product columns per workgroup
   hipLaunchKernelGGL(btpm_contract_kernel, dim3((ncol + per - 1) / per), dim3(BTPM_CRATER_THREADS),
-                     0, st, dA.as<_Float16>(), dE.as<float>(), dQ.as<float>(), ncol, unscale);
+                     0, st, dA.as<_Float16>(), dE.as<float>(), dQ.as<float>(), ncol, unscale,
+                     (const float*)nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(q, dQ.p, dQ.n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

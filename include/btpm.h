@@ -322,6 +322,21 @@ int btpm_server_stop(btpm_server *srv, btpm_batch_stats *stats);
  * q[n][j] = sum_i VF(j->i) e[n][i] for n < ncol, computed with the same
  * fp16 hi/lo MFMA split; e is [ncol][100], q is [ncol][100].                */
 int btpm_crater_contract(btpm_ctx *ctx, const float *vf, const float *e, int ncol, float *q);
+/* The same product without any allocation, on device memory: vf [100*100]
+ * (vf[i*100+j] = VF(j->i)), e [ncol][100] and q [ncol][100] are device
+ * pointers; the packed fp16 hi/lo tiles of VF go to a fixed 50,432-byte
+ * array in device memory (one per device, held until the process ends)
+ * instead of a buffer the library allocates.  stream is a hipStream_t: the
+ * call is enqueued on it and returns at once (graph capture allowed); NULL
+ * runs on the context's stream and waits.  Results equal
+ * btpm_crater_contract's bit for bit.  Calls that overlap on one device
+ * race on the array (calls on one stream are ordered).  ncol at most
+ * BTPM_NOALLOC_MAX_COLS.  No counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops that must not allocate).                                           */
+#define BTPM_NOALLOC_MAX_COLS (1 << 24)
+int btpm_crater_contract_noalloc(btpm_ctx *ctx, const float *vf, const float *e, int ncol,
+                                 float *q, void *stream);
 
 /* Outputs [nf][ntau] (host memory): shadow (1 lit, 0 back-facing or shadowed
  * by a facet nearer the Sun, including the partner body: eclipses), illum =
