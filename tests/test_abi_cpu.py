@@ -18,6 +18,7 @@ def declared_functions():
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         src = re.sub(r"//[^\n]*", "", src)
+        src = re.sub(r"^\s*#[^\n]*", "", src, flags=re.M)   # preprocessor lines
         for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", src):
             name = m.group(1)
             if name not in ("if", "while", "for", "return", "sizeof"):
