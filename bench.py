@@ -9,7 +9,15 @@ spherical-cap crater of 100 sub-facets on every facet.  f = 0 solves are the
 smooth facets only (5120 columns); f > 0 solves are smooth facets, then RGH
 and 512,000 crater columns, each to convergence (TACC = 0.01 K, Newton
 TACC/10, freeze after rev > 5, cap 301).  All 128 run through the batched,
-graph-captured scheduler; inputs are resident in HBM before timing.
+graph-captured scheduler; inputs are resident in HBM before timing, and the
+results stay on the device (end_to_end adds copying them out).
+
+Beside the headline: the same sweep warm-started along each roughness
+class's TI chain as the reference runs it (warm_sweep), the reference-loop
+CPU baseline on a bounded sample (cpu_baseline, warm as the reference, and
+cold), the plain single-solve GPU path, a subset of the sweep on the
+config-1 binary primary (9750-step 15:4 cycle with eclipses), serving, and
+the producers and consumer around the stage.
 
 Prints ONE JSON line.  value = converged solves per hour (whole job).
 """
@@ -91,10 +99,34 @@ def run_sweep(ctx, shape, plan, opts):
     return solves, stats, wall, infos
 
 
-def cpu_baseline(ctx, shape, case, crater, plan, solves, n_craters=16, check_ti_idx=(6, 25)):
-    """Times the float OpenMP restatement of the reference loop shape on a
-    bounded sample and checks the GPU results of the same solves against the
-    FP64 oracle (smooth facets in full, `n_craters` crater parents)."""
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ctx, shape, case, crater, plan, solves, infos, n_craters=256, n_check=16,
+                 warm_ti_idx=(6, 25), cold_ti_idx=(6,)):
+    """Times the float OpenMP restatement of the reference loop shape (refcpu)
+    on a bounded sample of the sweep and checks the GPU results of the same
+    solves against the FP64 oracle.
+
+    Sample: the full 5120-facet smooth solve (self-heating couples every
+    facet) and `n_craters` crater parents (their view-factor rows kept, so RGH
+    and RGI are the full shape's).  Warm, as the reference runs its sweep
+    (each TI from the previous TI's converged state, thermal.cpp:3676-3760):
+    TI k = 6 and 25 of the 32 started from the GPU's converged state of TI
+    k-1 (its smooth profiles and series, and the sampled craters' state).
+    Cold: TI k = 6 from the initial state.  Each timed solve gives the CPU
+    cost per rotation (smooth) and per rotation and crater parent (craters);
+    a sweep's CPU time is that cost times the rotations every one of its
+    solves took (the GPU runs' counts, within one of the CPU's)."""
     from binarytpm_amd import inputs
     from oracle import oracle as O
     nf = case.illum.shape[0]
@@ -103,70 +135,209 @@ def cpu_baseline(ctx, shape, case, crater, plan, solves, n_craters=16, check_ti_
     tis = inputs.sweep_inertias(32)
     t0 = time.time()
     F = O.facet_scatter(case.illum, 0.1, case.csr, multiple=True, acc=1e-4)
-    log(f"oracle facet scattering {time.time() - t0:.1f} s")
-    sel = np.linspace(0, nf - 1, n_craters).astype(int)
     rp, ci, vv = case.csr
-    # CSR restricted to the sampled crater parents (rows), columns unchanged
-    sub_rp = np.concatenate([[0], np.cumsum(rp[sel + 1] - rp[sel])]).astype(np.int32)
-    rows = np.concatenate([np.arange(rp[f], rp[f + 1]) for f in sel])
-    sub_csr = (sub_rp, ci[rows], vv[rows])
-    sub_drf = crater["drf"][rows]
-    rgi = O.rgi(sub_csr, sub_drf, F, np.full(nf, 0.1), n_craters)
-    il = O.crater_scatter(crater["vf"], 0.1, crater["shadow"][sel], crater["cosang"][sel], rgi,
-                          crater["solar"], multiple=True, acc=1e-4)
-    t_smooth, t_crater, n_cs_smooth, n_cs_crater = [], [], [], []
+
+    def parents(sel):
+        sub_rp = np.concatenate([[0], np.cumsum(rp[sel + 1] - rp[sel])]).astype(np.int32)
+        rows = np.concatenate([np.arange(rp[f], rp[f + 1]) for f in sel])
+        sub_csr = (sub_rp, ci[rows], vv[rows])
+        sub_drf = crater["drf"][rows]
+        rgi = O.rgi(sub_csr, sub_drf, F, np.full(nf, 0.1), len(sel))
+        il = O.crater_scatter(crater["vf"], 0.1, crater["shadow"][sel], crater["cosang"][sel],
+                              rgi, crater["solar"], multiple=True, acc=1e-4)
+        return sub_csr, sub_drf, il
+
+    sel = np.linspace(0, nf - 1, n_craters).astype(int)
+    csr_t, drf_t, il_t = parents(sel)
+    chk = np.linspace(0, nf - 1, n_check).astype(int)
+    csr_c, drf_c, il_c = parents(chk)
+    log(f"CPU sample inputs (oracle scattering of {n_craters} + {n_check} parents) "
+        f"{time.time() - t0:.1f} s")
+
+    def rc_solves(ti, state_s=None, state_c=None):
+        t = time.time()
+        st_rc, rev_s, cs_s = O.refcpu_smooth_solve(F, 0.1, ti, case.ntau, case.rot_period,
+                                                   tacc=0.01, csr=case.csr, nthreads=threads,
+                                                   state=state_s)
+        ts = time.time() - t
+        rgh_rc = O.rgh(csr_t, drf_t, st_rc, es, n_craters)
+        t = time.time()
+        _, rev_c, cs_c = O.refcpu_crater_solve(crater["vf"], 0.1, ti, il_t, rgh_rc,
+                                               case.rot_period, tacc=0.01, nthreads=threads,
+                                               state=state_c)
+        tc = time.time() - t
+        return dict(ts=ts, tc=tc, rs=rev_s, rc=rev_c, cs=cs_s + cs_c)
+
+    runs = {"warm": [], "cold": []}
+    for k in warm_ti_idx:
+        g = solves[plan.index((float(tis[k - 1]), 0.5))]
+        st_s = {"temp": g.profile(), "stemp": g.surface_temps()}
+        st_c = {"temp": g.crater_profile()[sel], "stemp": g.crater_temps()[sel]}
+        runs["warm"].append(rc_solves(float(tis[k]), st_s, st_c))
+    for k in cold_ti_idx:
+        runs["cold"].append(rc_solves(float(tis[k])))
+    for kind, ks in (("warm", warm_ti_idx), ("cold", cold_ti_idx)):
+        for k, r in zip(ks, runs[kind]):
+            log(f"refcpu TI={tis[k]:.1f} {kind}: smooth {r['ts']:.1f} s ({r['rs']} rot), "
+                f"{n_craters} crater parents {r['tc']:.1f} s ({r['rc']} rot)")
+    # cost per rotation: smooth facets, and one crater parent
+    all_runs = runs["warm"] + runs["cold"]
+    t_rot_s = sum(r["ts"] for r in all_runs) / sum(r["rs"] for r in all_runs)
+    t_rot_c = sum(r["tc"] for r in all_runs) / sum(r["rc"] for r in all_runs) / n_craters
+
+    def sweep_s(inf_list, fr):
+        return sum(t_rot_s * i.rotations + (t_rot_c * nf * i.crater_rotations if f > 0 else 0.0)
+                   for i, f in zip(inf_list, fr))
+
+    # FP64 oracle check of the GPU results at the warm TIs (cold f = 0.5 solves)
     dmax, sq, cnt = 0.0, 0.0, 0
-    for k in check_ti_idx:
+    p = O.params(case.ntau, case.rot_period, tacc=0.01)
+    for k in warm_ti_idx:
         ti = float(tis[k])
-        # CPU baseline timing (refcpu: float, OpenMP, reference loop shape)
-        t0 = time.time()
-        st_rc, rev_rc, cs = O.refcpu_smooth_solve(F, 0.1, ti, case.ntau, case.rot_period,
-                                                  tacc=0.01, csr=case.csr, nthreads=threads)
-        t_smooth.append(time.time() - t0)
-        n_cs_smooth.append(cs)
-        rgh_rc = O.rgh(sub_csr, sub_drf, st_rc, es, n_craters)
-        t0 = time.time()
-        _, rev_c, cs = O.refcpu_crater_solve(crater["vf"], 0.1, ti, il, rgh_rc, case.rot_period,
-                                             tacc=0.01, nthreads=threads)
-        t_crater.append(time.time() - t0)
-        n_cs_crater.append(cs)
-        log(f"refcpu TI={ti:.1f}: smooth {t_smooth[-1]:.1f} s ({rev_rc} rot), "
-            f"{n_craters}-crater sample {t_crater[-1]:.1f} s ({rev_c} rot)")
-        # FP64 oracle check of the GPU results for this TI (f = 0.5 solve)
-        p = O.params(case.ntau, case.rot_period, tacc=0.01)
         sm = O.smooth_solve(p, F, 0.1, ti, csr=case.csr)
-        rgh = O.rgh(sub_csr, sub_drf, sm.stemp, es, n_craters)
-        cr = O.crater_solve(p, crater["vf"], 0.1, ti, il, rgh)
+        rgh = O.rgh(csr_c, drf_c, sm.stemp, es, n_check)
+        cr = O.crater_solve(p, crater["vf"], 0.1, ti, il_c, rgh)
         idx = plan.index((ti, 0.5))
         g_s = solves[idx].surface_temps().astype(np.float64)
-        g_c = solves[idx].crater_temps()[sel].astype(np.float64)
+        g_c = solves[idx].crater_temps()[chk].astype(np.float64)
         for a, b in ((g_s, sm.stemp), (g_c, cr.stemp)):
             d = np.abs(a - b)
             dmax = max(dmax, float(d.max()))
             sq += float((d ** 2).sum())
             cnt += d.size
         log(f"oracle check TI={ti:.1f}: running max|dT| {dmax:.4f} K")
-    # extrapolate: f = 0 solve = smooth; f > 0 solve = smooth + all craters
-    t_s = float(np.mean(t_smooth))
-    t_c_full = float(np.mean(t_crater)) * nf / n_craters
-    n0 = sum(1 for _, f in plan if f == 0)
-    n1 = len(plan) - n0
-    cpu_sweep_s = n0 * t_s + n1 * (t_s + t_c_full)
-    sample_s = sum(t_smooth) + sum(t_crater)
-    colsteps = sum(n_cs_smooth) + sum(n_cs_crater)
+    sample_s = sum(r["ts"] + r["tc"] for r in all_runs)
+    colsteps = sum(r["cs"] for r in all_runs)
+    cold_s = sweep_s(infos, [f for _, f in plan])
+    tis_w = [round(float(tis[k]), 1) for k in warm_ti_idx]
     return {
-        "value": 3600.0 * len(plan) / cpu_sweep_s,
+        "value": None,            # the warm sweep's, filled in by the caller
         "unit": "converged solves/hour",
         "cores": threads,
+        "cpu_model": cpu_model(),
         "kind": "port",
-        "sample": (f"refcpu (float OpenMP restatement of the reference loop shape) on TI "
-                   f"{[round(float(tis[k]), 1) for k in check_ti_idx]}: full 5120-facet smooth "
-                   f"solve + {n_craters} of 5120 crater parents, extrapolated to the "
-                   f"128-solve sweep; sample {sample_s:.1f} s of CPU wall time"),
+        "sample": (f"refcpu (float OpenMP restatement of the reference loop shape): the full "
+                   f"5120-facet smooth solve and {n_craters} of 5120 crater parents at TI "
+                   f"{tis_w}, each from the converged state of the previous TI of the sweep "
+                   f"(as the reference runs it), and at TI "
+                   f"{[round(float(tis[k]), 1) for k in cold_ti_idx]} from the initial state; "
+                   f"{sample_s:.1f} s of timed CPU work, {t_rot_s:.3f} s per smooth rotation "
+                   f"and {t_rot_c * 1e3:.2f} ms per crater parent and rotation, times every "
+                   f"solve's rotations in the sweep"),
         "facet_layer_steps_per_s": colsteps * NX / sample_s,
+        "cold": {"value": 3600.0 * len(plan) / cold_s, "sweep_s": cold_s,
+                 "note": "the same per-rotation costs times the cold sweep's rotations"},
+        "t_rot_smooth_s": t_rot_s,
+        "t_rot_crater_parent_s": t_rot_c,
+        "sweep_cost": sweep_s,       # for the caller (removed before printing)
         "check_max_dT_K": dmax,
         "check_rms_dT_K": float(np.sqrt(sq / max(cnt, 1))),
+        "check_sample": f"GPU f=0.5 solves at TI {tis_w}: smooth facets in full, "
+                        f"{n_check} crater parents, vs the FP64 oracle",
     }
+
+
+def warm_sweep(ctx, shape, plan, opts):
+    """The sweep as the reference runs it: per roughness class one chain over
+    the 32 TIs in ascending order, each solve warm-started from the previous
+    one's final state (btpm_solve_warm_start); the four chains run as one
+    batch (the scheduler admits a solve once its predecessor's phase is done),
+    every f > 0 chain computed (no deduplication)."""
+    from binarytpm_amd import thermal as th
+    fracs = sorted({f for _, f in plan})
+    solves, order = [], []
+    for f in fracs:
+        prev = None
+        for ti in sorted({t for t, g in plan if g == f}):
+            s = th.Solve(ctx, shape, ti, with_craters=f > 0, warm_from=prev)
+            solves.append(s)
+            order.append((ti, f))
+            prev = s
+    t0 = time.perf_counter()
+    st = th.batch_run(ctx, solves, opts)
+    wall = time.perf_counter() - t0
+    infos = [s.info() for s in solves]
+    out = {
+        "solves_per_hour": 3600.0 * st.converged / wall,
+        "wall_s": wall,
+        "converged": int(st.converged),
+        "rotations_smooth": int(sum(i.rotations for i in infos)),
+        "rotations_crater": int(sum(i.crater_rotations for i in infos)),
+        "facet_layer_steps_per_s": st.column_steps * NX / wall,
+    }
+    return out, solves, order
+
+
+def fetch_all(solves):
+    """Copies every solve's results to host memory (surface temperatures,
+    and crater temperatures of the rough solves), as a caller of the stage
+    takes them; reused host buffers."""
+    t0 = time.perf_counter()
+    nbytes = 0
+    buf_s = buf_c = None
+    for s in solves:
+        buf_s = s.surface_temps(buf_s)
+        nbytes += buf_s.nbytes
+        if s.with_craters:
+            buf_c = s.crater_temps(buf_c)
+            nbytes += buf_c.nbytes
+    return time.perf_counter() - t0, nbytes
+
+
+def binary_phase(ctx, n_ti=4):
+    """The sweep on the config-1 binary's primary: 15:4 super-cycle over 9750
+    steps (numrevs 15), eclipse masks from the secondary, the primary's facet
+    view factors and a cap crater on every facet (512,000 crater columns, 40 GB
+    of crater state per solve): a stratified subset of the 128 solves -- TIs
+    k = 0, 10, 21, 31 of the 32 at f = 0 and f = 0.5 -- in one batch, admitted
+    by HBM footprint."""
+    from binarytpm_amd import inputs
+    from binarytpm_amd import thermal as th
+    t0 = time.time()
+    case, crater = inputs.rough_binary_primary()
+    log(f"binary primary inputs in {time.time() - t0:.1f} s ({case.illum.shape[1]} steps, "
+        f"{(1 - case.mask.mean()) * 100:.1f} % of facet-steps eclipsed)")
+    cg, tab = crater["geom"], crater["table"]
+    p = th.params(case.ntau, case.rot_period, tacc=0.01)
+    t0 = time.time()
+    shape = th.Shape(ctx, p, case.illum, 0.1, numrevs=case.numrevs, csr=case.csr, scatter=2,
+                     scatter_acc=1e-4,
+                     crater=th.CraterModel(vf=cg.vf, sun_local=crater["sun_local"],
+                                           vis_table=tab.vis, normals=cg.normals,
+                                           solar=crater["solar"], multiple=True,
+                                           scatter_acc=1e-4, drf=crater["drf"]))
+    log(f"binary primary shape resident in {time.time() - t0:.1f} s")
+    del crater
+    tis = inputs.sweep_inertias(32)
+    ks = [0, 10, 21, 31][:n_ti]
+    plan = [(float(tis[k]), f) for k in ks for f in (0.0, 0.5)]
+    solves = [th.Solve(ctx, shape, ti, with_craters=f > 0) for ti, f in plan]
+    opts = th.batch_opts(rotations_per_sync=4, use_graph=True, profile=True)
+    t0 = time.perf_counter()
+    st = th.batch_run(ctx, solves, opts)
+    wall = time.perf_counter() - t0
+    infos = [s.info() for s in solves]
+    out = {
+        "workload": (f"config-1 binary primary: 5120 facets x {case.ntau} steps (15:4 cycle, "
+                     "eclipses), 3 %-dense view factors, 45-deg cap crater x100 per facet; "
+                     f"TI k={ks} of the 32 x f in {{0, 0.5}} ({len(plan)} solves)"),
+        "wall_s": wall,
+        "solves_per_hour": 3600.0 * st.converged / wall,
+        "converged": int(st.converged),
+        "max_resident": int(st.max_resident),
+        "mem_waits": int(st.mem_waits),
+        "facet_layer_steps_per_s": st.column_steps * NX / wall,
+        "crater_avg_launch_ms": 1000.0 * st.crater_kernel_s / max(st.crater_launches, 1),
+        "crater_kernel_gbs": (BYTES_PER_COLSTEP * st.crater_column_steps / st.crater_kernel_s
+                              / 1e9 if st.crater_kernel_s > 0 else 0.0),
+        "rotations": [(i.rotations, i.crater_rotations) for i in infos],
+    }
+    log(f"binary primary: {len(plan)} solves in {wall:.2f} s, {st.max_resident} resident at "
+        f"most, crater kernel {out['crater_avg_launch_ms']:.1f} ms per launch")
+    for s in solves:
+        s.close()
+    shape.close()
+    return out
 
 
 def plain_stepwise(ctx, shape, plan, infos):
@@ -590,6 +761,12 @@ def main():
                     help="skip the view-factor producer measurement")
     ap.add_argument("--no-observe", action="store_true",
                     help="skip the Observe flux synthesis measurement")
+    ap.add_argument("--no-warm", action="store_true",
+                    help="skip the warm-started (chained) sweep")
+    ap.add_argument("--no-binary", action="store_true",
+                    help="skip the sweep subset on the config-1 binary primary")
+    ap.add_argument("--no-fetch", action="store_true",
+                    help="skip copying every result to the host after the timed sweep")
     ap.add_argument("--producers-only", action="store_true",
                     help="measure only the illumination and view-factor producers (one JSON "
                          "line with their results; for profiling)")
@@ -659,6 +836,13 @@ def main():
         sk_s = sum(s.smooth_kernel_s for s in stats)
         sk_n = sum(s.smooth_launches for s in stats)
         sk_cs = sum(s.smooth_column_steps for s in stats)
+        hk_s = sum(s.heat_kernel_s for s in stats)
+        hk_n = sum(s.heat_launches for s in stats)
+        hk_sr = sum(s.heat_set_rotations for s in stats)
+        nnz, nfac = len(case.csr[1]), int(case.illum.shape[0])
+        # facet self-heating: per set-rotation the view-factor entries (index +
+        # value), the emission read and the source read plus total written
+        heat_bytes = 8.0 * nnz + 3.0 * 4.0 * nfac * case.ntau
         lat = np.array([i.latency_s for i in infos])
         achieved = BYTES_PER_COLSTEP * ck_cs / ck_s / 1e9 if ck_s > 0 else 0.0
         tpc = measured_traffic_per_colstep()
@@ -689,6 +873,9 @@ def main():
             "facet_layer_steps_per_s": colsteps * NX / T,
             "facet_layer_steps_per_s_incl_frozen": colsteps_all * NX / T,
             "solves_converged": int(conv),
+            "timed_region": ("the sweep's solves from submission to convergence, inputs "
+                             "resident in HBM; every result stays on the device (see "
+                             "end_to_end for copying them out)"),
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),
             "roofline": {
@@ -717,7 +904,32 @@ def main():
                     if (ck_s > 0 and summary_value(VALU_FILE, "mfma_flop_per_column_step"))
                     else None),
             },
+            "selfheat_roofline": {
+                "kernel": "btpm_selfheat_ell_h32 / btpm_selfheat_ell (one launch per shape)",
+                "bound": "hbm",
+                "bytes_per_set_rotation": heat_bytes,
+                "achieved": heat_bytes * hk_sr / hk_s / 1e9 if hk_s > 0 else 0.0,
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": heat_bytes * hk_sr / hk_s / 1e9 / HBM_PEAK_GBS if hk_s > 0 else 0.0,
+                "avg_launch_ms": 1000.0 * hk_s / max(hk_n, 1),
+                "launches": hk_n,
+                "set_rotations_per_launch": hk_sr / max(hk_n, 1),
+            },
         }
+        if not args.no_fetch:
+            fs, nb = fetch_all(last)
+            line["end_to_end"] = {
+                "fetch_s": fs,
+                "fetch_gb": nb / 1e9,
+                "fetch_gbs": nb / fs / 1e9,
+                "solves_per_hour": 3600.0 * len(plan) / (T / args.steps + fs),
+                "note": ("the last timed sweep plus copying every solve's surface temperatures "
+                         "and every rough solve's crater temperatures [nf][ntau][100] to host "
+                         "memory"),
+            }
+            log(f"end to end: results {nb / 1e9:.1f} GB copied out in {fs:.2f} s "
+                f"({nb / fs / 1e9:.1f} GB/s)")
         if not args.no_plain:
             pl = plain_stepwise(ctx, shape, plan, infos)
             line["plain"] = pl
@@ -727,14 +939,35 @@ def main():
             line["batched_over_single_solve_persistent"] = line["value"] / v1
             log(f"plain stepwise {pl['solves_per_hour']:.0f} solves/h, persistent single-solve "
                 f"{v1:.0f} solves/h")
+        cb = None
         if not args.no_cpu:
-            cb = cpu_baseline(ctx, shape, case, crater, plan, last)
-            line["cpu_baseline"] = cb
-            line["vs_cpu"] = line["value"] / cb["value"]
-            line["max_dT_K"] = cb["check_max_dT_K"]
-            line["rms_dT_K"] = cb["check_rms_dT_K"]
+            cb = cpu_baseline(ctx, shape, case, crater, plan, last, infos)
         for s in last:
             s.close()
+        ws = None
+        if not args.no_warm:
+            ws, wsolves, worder = warm_sweep(ctx, shape, plan, opts)
+            line["warm_sweep"] = ws
+            log(f"warm-started sweep: {ws['wall_s']:.2f} s ({ws['solves_per_hour']:.0f} solves/h), "
+                f"rotations smooth {ws['rotations_smooth']} crater {ws['rotations_crater']}")
+            winfos = [x.info() for x in wsolves]
+            for s in reversed(wsolves):
+                s.close()
+        if cb is not None:
+            cost = cb.pop("sweep_cost")
+            if ws is not None:
+                # the reference's own sweep: warm, its rotations those of the warm chains
+                warm_s = cost(winfos, [f for _, f in worder])
+                cb["value"] = 3600.0 * len(plan) / warm_s
+                cb["sweep_s"] = warm_s
+                ws["vs_cpu"] = ws["solves_per_hour"] / cb["value"]
+            else:
+                cb["value"] = cb["cold"]["value"]
+            line["cpu_baseline"] = cb
+            line["vs_cpu"] = line["value"] / cb["value"]
+            line["vs_cpu_cold"] = line["value"] / cb["cold"]["value"]
+            line["max_dT_K"] = cb["check_max_dT_K"]
+            line["rms_dT_K"] = cb["check_rms_dT_K"]
         shape.close()
         if not args.no_serve:
             line["serving"] = serve_phase(ctx, args.serve_n, args.serve_rate, rps=args.serve_rps)
@@ -745,6 +978,11 @@ def main():
         if not args.no_observe:
             line["observe_fluxes"] = observe_phase(ctx, cpu=not args.no_cpu)
         line["roofline"]["valu_ceiling_wave_instr_per_s"] = valu_ceiling()
+    if not args.no_binary:
+        # a context of its own: its 40 GB crater sets are returned to the
+        # driver when it closes instead of staying in the device cache
+        with th.Context(0) as ctx2:
+            line["binary_primary"] = binary_phase(ctx2)
     print(json.dumps(line), flush=True)
 
 

@@ -134,6 +134,8 @@ class BatchStats(C.Structure):
         ("max_resident", C.c_int), ("deferrals", C.c_int),
         ("tune_rotations", C.c_int), ("tune_decisions", C.c_int),
         ("fast_rotations", C.c_longlong), ("mem_waits", C.c_int),
+        ("heat_kernel_s", C.c_double), ("heat_launches", C.c_int),
+        ("heat_set_rotations", C.c_longlong),
     ]
 
 

@@ -667,7 +667,7 @@ struct Sched {
   bool fast_active = false;
   btpm_batch_stats stats{};
   // launch timing (opts.profile): events around each conduction launch
-  std::vector<hipEvent_t> ev_s, ev_c;  // pairs (start, end) per rotation
+  std::vector<hipEvent_t> ev_s, ev_c, ev_h;  // pairs (start, end) per rotation
   // Admission works on the device's live free memory, so results that
   // completed solves keep until they are destroyed are accounted for:
   size_t headroom = 0;  // kept free: (1 - mem_fraction) of what was free at start
@@ -698,6 +698,7 @@ struct Sched {
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_s) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_c) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_h) (void)hipEventDestroy(e);
   }
 
   void make_events(int R) {
@@ -710,6 +711,7 @@ struct Sched {
     };
     grow(ev_s, 2 * (size_t)R);
     grow(ev_c, 2 * (size_t)R);
+    grow(ev_h, 2 * (size_t)R);
   }
 
   void drop_graph() {
@@ -1067,7 +1069,8 @@ struct Sched {
   void profiled_rotation(int i, bool direct) {
     hipStream_t st = ctx->stream;
     for (int ph = 0; ph < 4; ++ph) {
-      hipEvent_t* ev = ph == 1 ? &ev_s[2 * i] : ph == 2 ? &ev_c[2 * i] : nullptr;
+      hipEvent_t* ev = ph == 0 ? &ev_h[2 * i] : ph == 1 ? &ev_s[2 * i] : &ev_c[2 * i];
+      if (ph == 3) ev = nullptr;
       const bool empty = phase_empty(ph);
       if (empty) continue;
       if (ev) HIPCHK(hipEventRecord(ev[0], st));
@@ -1163,6 +1166,12 @@ struct Sched {
           HIPCHK(hipEventElapsedTime(&ms, ev_s[2 * i], ev_s[2 * i + 1]));
           stats.smooth_kernel_s += ms * 1e-3;
           stats.smooth_launches++;
+        }
+        if (!phase_empty(0)) {
+          HIPCHK(hipEventElapsedTime(&ms, ev_h[2 * i], ev_h[2 * i + 1]));
+          stats.heat_kernel_s += ms * 1e-3;
+          stats.heat_launches += (int)heat.size();
+          for (const HeatGroup& g : heat) stats.heat_set_rotations += g.nsets;
         }
         if (n_tiles_c) {
           HIPCHK(hipEventElapsedTime(&ms, ev_c[2 * i], ev_c[2 * i + 1]));
@@ -1942,7 +1951,9 @@ static int fetch_series(btpm_solve* s, SetHost* S, float* out, int crater) {
   else
     hipLaunchKernelGGL(btpm_unpack_blocked, dim3(grid1((size_t)S->d.ncols * nb, 256)), dim3(256),
                        0, c->stream, S->T.as<float>(), raw.as<float>(), S->d.ncols, nt, nb);
-  HIPCHK(hipMemcpyAsync(out, raw.p, raw.n, hipMemcpyDeviceToHost, c->stream));
+  // large results leave through the context's pinned slices (host threads
+  // copy one slice out while the DMA fills the other)
+  download_staged(c, out, raw.p, raw.n);
   HIPCHK(hipStreamSynchronize(c->stream));
   API_END
 }

@@ -520,6 +520,38 @@ def rough_primary(subdiv: int = 4, ntau: int = 650, r_au: float = 1.1, seed: int
     return case, crater
 
 
+def rough_binary_primary(ntau_per_primary_rev: int = 650, nrev_p: int = 15, nrev_s: int = 4,
+                         seed: int = 1, gamma_deg: float = 45.0):
+    """The sweep's shape on the config-1 binary: its 5120-facet primary over
+    the 15:4 super-cycle (9750 steps, numrevs 15) with the eclipse masks the
+    1280-facet secondary casts, the primary's seeded facet view factors and a
+    45-degree cap crater on every facet.  The secondary only eclipses (its
+    own temperatures are not part of these solves).  Crater sunlight comes as
+    sun_local (each facet's Sun direction in its crater frame, scaled by its
+    eclipse mask) with the sunlit-fraction table, which the GPU evaluates in
+    its scattering kernel, so no [nf][ntau][100] arrays are built.
+    Returns (SmoothCase of the primary, crater inputs dict)."""
+    ntau = ntau_per_primary_rev * nrev_p
+    axes_p = (0.6, 0.5, 0.45)
+    Vp, Fp = ellipsoid(4, axes_p)
+    Vs, Fs = ellipsoid(3, (0.3, 0.3, 0.3))
+    bi = binary_illumination(Vp, Fp, axes_p, Vs, Fs, 0.3, 3.0, ntau, nrev_p, nrev_s,
+                             solar_at(1.1))
+    _, _, _, frames = facet_geometry(Vp, Fp)
+    nf = len(Fp)
+    csr, dirs = sparse_viewfactors(nf, 0.03, 0.2, seed)
+    cg = crater_cap(gamma_deg)
+    table = CraterTable(cg)
+    drf = shape_to_crater_vf(csr, dirs, table)
+    sun = sun_body_frame(ntau, nrev_p)
+    case = SmoothCase(bi.D_primary, 0.1, ntau, P_PRIMARY * nrev_p,
+                      numrevs=np.full(nf, float(nrev_p), np.float32), csr=csr, dirs=dirs,
+                      frames=frames, sun=sun, mask=bi.mask_primary, scatter=2)
+    crater = dict(vf=cg.vf, sun_local=crater_sun_local(frames, sun, bi.mask_primary),
+                  solar=solar_at(1.1), drf=drf, geom=cg, table=table)
+    return case, crater
+
+
 def serving_shape(subdiv: int, seed: int, ntau: int = 650):
     """Config-4 request shape at 1 AU: a seeded icosphere-ellipsoid (semi-axes
     a = 0.3..1.2 km, b = a*U[0.7,1], c = b*U[0.7,1]; 20*4^subdiv facets), spin

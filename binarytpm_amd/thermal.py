@@ -36,6 +36,16 @@ def _ptr(a, t=_fp):
     return None if a is None else a.ctypes.data_as(t)
 
 
+def _out(out, shape):
+    """A C-contiguous float32 output array of `shape`: `out` itself (checked)
+    or a new one."""
+    if out is None:
+        return np.empty(shape, dtype=np.float32)
+    if out.dtype != np.float32 or out.shape != tuple(shape) or not out.flags.c_contiguous:
+        raise ValueError(f"out must be a C-contiguous float32 array of shape {tuple(shape)}")
+    return out
+
+
 def params(ntau: int, rot_period: float, tacc: float = 0.01, **kw) -> Params:
     """Model constants (reference globals, thermal.cpp:14-83); keyword
     arguments override eps, sigma, ir_abs, newton_tol, max_rev, check_after,
@@ -231,13 +241,17 @@ class Solve:
         check(lib().btpm_solve_get_info(self._h, C.byref(inf)))
         return inf
 
-    def surface_temps(self) -> np.ndarray:
-        out = np.empty((self.shape.nf, self.shape.ntau), dtype=np.float32)
+    def surface_temps(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """[nf][ntau] final surface temperatures (into `out` when given)."""
+        shape = (self.shape.nf, self.shape.ntau)
+        out = _out(out, shape)
         check(lib().btpm_solve_surface_temps(self._h, _ptr(out)))
         return out
 
-    def crater_temps(self) -> np.ndarray:
-        out = np.empty((self.shape.nf, self.shape.ntau, NS), dtype=np.float32)
+    def crater_temps(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """[nf][ntau][100] final crater temperatures (into `out` when given)."""
+        shape = (self.shape.nf, self.shape.ntau, NS)
+        out = _out(out, shape)
         check(lib().btpm_solve_crater_temps(self._h, _ptr(out)))
         return out
 

@@ -174,6 +174,12 @@ typedef struct {
     long long fast_rotations;    /* extra fast-lane rotations enqueued      */
     int mem_waits;               /* solves whose admission waited for device
                                     memory (the HBM-footprint budget)      */
+    /* with opts.profile: the facet self-heating phase (one launch per shape
+     * with view factors), its summed duration, launches and the smooth sets
+     * those launches covered (set-rotations)                                */
+    double heat_kernel_s;
+    int heat_launches;
+    long long heat_set_rotations;
 } btpm_batch_stats;
 
 /* Illumination geometry and shadows of a (binary) shape over one cycle: the
