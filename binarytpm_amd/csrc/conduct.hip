@@ -102,6 +102,8 @@ struct CraterConst {
   double pk[BTPM_BLK];
   double r0[2];         // residual of the scale carried from the last rescale
   double m1;            // Gamma sqrt(2 pi n / P) / dz
+  double es, tol;       // emissivity x sigma and the Newton tolerance, read from
+                        // LDS at each step (held in VGPRs they would cost 4)
 };
 
 // Fold the accumulated scale into the profile: U *= f with f = fp32(sd); the
@@ -544,17 +546,17 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
       scale_table(S.r[gt], lds_cc[tid].sk, lds_cc[tid].pk);
       lds_cc[tid].r0[0] = 1.0;
       lds_cc[tid].m1 = (double)S.m1[gt];
+      lds_cc[tid].es = (double)S.es;
+      lds_cc[tid].tol = (double)S.newton_tol;
     }
-    // the idle lanes' Q slots are never written by the product
-    if (tid >= P * NS)
-      for (int k = 0; k < BTPM_BLK; ++k) lds_q[k * kQCols + tid] = 0.0f;
   }
   __syncthreads();
+  // the idle lanes' Q slots are never written by the product
+  if (tid >= P * NS)
+    for (int k = 0; k < BTPM_BLK; ++k) lds_q[k * kQCols + tid] = 0.0f;
 
   const float kr = scaled_kr(S.r[gg]);
   CraterConst& cc = lds_cc[lcr];
-  const double es = (double)S.es;
-  const double tol = (double)S.newton_tol;
   const int cap = S.newton_cap;
   const float esia = S.esia;
   const int ncols = S.ncols;
@@ -566,7 +568,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   const int nb = S.nb;
   const int ntau = S.ntau;
   const size_t bstride = (size_t)ncols * BTPM_BLK;
-  const size_t coff = (size_t)c * BTPM_BLK;
+  const unsigned coff = (unsigned)c * BTPM_BLK;   // < 2^32 / 4: lane part of an address
   const int wave = tid >> 6;
   const int lane = tid & 63;
 
@@ -580,75 +582,111 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   float dm = 0.0f;
   float dl = 0.0f;   // T - T_prev at the previous step (predictor shift)
   float dl2 = 0.0f;  // and at the step before
-  auto rotation = [&](auto inst_tag) {
-    constexpr bool INST = decltype(inst_tag)::value;
-    for (int b = 0; b < nb; ++b) {
-      // previous-rotation emission of the block -> LDS (fp16 hi/lo split);
-      // T_prev itself is parked in lane-private LDS slots
-      if (tid < P * NS) {
-#pragma unroll
-        for (int k = 0; k < BTPM_BLK; ++k) {
-          const float t = nprev[k];
-          const float e = esia * (t * t) * (t * t);
-          const _Float16 hi = (_Float16)e;
-          const _Float16 lo = (_Float16)(e - (float)hi);
-          const int n = lcr * BTPM_BLK + k;
-          lds_b[0][n * kLdsK + j] = hi;
-          lds_b[1][n * kLdsK + j] = lo;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < BTPM_BLK; ++k) lds_p[k][tid] = nprev[k];
-      const bool more = b + 1 < nb;
-      const size_t o = (size_t)b * bstride + coff;
-      const size_t on = (size_t)(b + 1) * bstride + coff;
-      if (more) load8(Tin + on, nprev);
-      __syncthreads();
-      crater_product(lds_a, lds_b, lds_q, wave, lane, S.vf_unscale);
-      __syncthreads();
+  // the lane's part of a block address, 32 bits, added to the uniform block
+  // base at each use (the asm statement keeps the compiler from folding it
+  // into 64-bit per-lane pointers held across the loop)
+  auto lane_off = [&]() {
+    unsigned o = coff;
+    asm volatile("" : "+v"(o));
+    return o;
+  };
 
-      const int kend = min(BTPM_BLK, ntau - b * BTPM_BLK);
+  // previous-rotation emission of block b -> LDS (fp16 hi/lo split), T_prev
+  // parked in lane-private LDS slots, the next block's T_prev prefetched,
+  // then the block's self-heating product into lds_q
+  auto block_head = [&](int b, bool more) {
+    if (tid < P * NS) {
 #pragma unroll
       for (int k = 0; k < BTPM_BLK; ++k) {
-        const float tp = lds_p[k][tid];
-        float t = tp;
-        if (k < kend) {
-          const double tot = (double)(nsrc[k] + lds_q[k * kQCols + tid]);
-          if constexpr (INST) {  // roughness_inst_equilibrium, thermal_GPU.cu:581-641
-            t = (float)sqrt(sqrt(tot / es));
-          } else {
-            // the per-step conversions leave out the residual (within 6e-8 of
-            // 1): a bounded perturbation of T_1 and T_0, not a gain on the profile
-            const float2 sk = cc.sk[k];
-            t = (float)conduct_step<NX>(u, kr, sk.x, sk.y, tot, cc.m1, es, Trefd, tol, cap,
-                                        (double)(tp + (2.0f * dl - dl2)));
-          }
-          dl2 = dl;
-          dl = t - tp;
-          dm = fmaxf(dm, fabsf(dl));
-        }
-        lds_p[k][tid] = t;  // the slot now holds this rotation's T
-        if (k == BTPM_BLK / 2 - 1 && more) load4(srcp + on, nsrc, 0);
-        if (k == BTPM_BLK - 1 && more) load4(srcp + on, nsrc, 4);
-      }
-      asm volatile("" ::: "memory");
-      if constexpr (!INST) {
-        // U *= f = fp32(r0 sig^kend); the residual r0 sig^kend / f is carried
-        // (cc.r0 is double buffered by block parity: every lane reads one half,
-        // sub-facet 0 writes the other, and the block's barriers order them)
-        const double pb = cc.r0[b & 1] * cc.pk[kend - 1];
-        const float f = (float)pb;
-#pragma unroll
-        for (int i = 0; i <= NX; ++i) u[i] *= f;
-        if (j == 0) cc.r0[(b + 1) & 1] = fma(pb - (double)f, (double)__builtin_amdgcn_rcpf(f), 1.0);
-      }
-      if (active) {
-        float out[BTPM_BLK];
-#pragma unroll
-        for (int k = 0; k < BTPM_BLK; ++k) out[k] = lds_p[k][tid];
-        store8(Tout + o, out);
+        const float t = nprev[k];
+        const float e = esia * (t * t) * (t * t);
+        const _Float16 hi = (_Float16)e;
+        const _Float16 lo = (_Float16)(e - (float)hi);
+        const int n = lcr * BTPM_BLK + k;
+        lds_b[0][n * kLdsK + j] = hi;
+        lds_b[1][n * kLdsK + j] = lo;
       }
     }
+#pragma unroll
+    for (int k = 0; k < BTPM_BLK; ++k) lds_p[k][tid] = nprev[k];
+    if (more) load8(Tin + (size_t)(b + 1) * bstride + lane_off(), nprev);
+    __syncthreads();
+    crater_product(lds_a, lds_b, lds_q, wave, lane, S.vf_unscale);
+    __syncthreads();
+  };
+  auto block_tail = [&](int b) {
+    if (active) {
+      float out[BTPM_BLK];
+#pragma unroll
+      for (int k = 0; k < BTPM_BLK; ++k) out[k] = lds_p[k][tid];
+      store8(Tout + (size_t)b * bstride + lane_off(), out);
+    }
+  };
+  // U *= f = fp32(r0 sig^kend) at the block end; the residual r0 sig^kend / f
+  // is carried (cc.r0 is double buffered by block parity: every lane reads one
+  // half, sub-facet 0 writes the other, and the block's barriers order them)
+  auto block_scale = [&](int b, int kend) {
+    const double pb = cc.r0[b & 1] * cc.pk[kend - 1];
+    const float f = (float)pb;
+    if (j == 0) cc.r0[(b + 1) & 1] = fma(pb - (double)f, (double)__builtin_amdgcn_rcpf(f), 1.0);
+    return f;
+  };
+  auto after_step = [&](int k, float tp, float t) {
+    dl2 = dl;
+    dl = t - tp;
+    dm = fmaxf(dm, fabsf(dl));
+    lds_p[k][tid] = t;  // the slot now holds this rotation's T
+  };
+  auto next_src = [&](int b, int k) {
+    const gfloat* next = srcp + (size_t)(b + 1) * bstride;  // uniform block base
+    if (k == BTPM_BLK / 2 - 1) load4(next + lane_off(), nsrc, 0);
+    if (k == BTPM_BLK - 1) load4(next + lane_off(), nsrc, 4);
+  };
+
+  // the steps of block b, all 57 nodes on the vector ALUs.  FULL: the block
+  // has all 8 steps and a successor whose inputs are prefetched (every block
+  // but the last), so the unrolled steps carry no step-count tests
+  auto valu_steps = [&](int b, auto inst_tag, auto full_tag) {
+    constexpr bool INST = decltype(inst_tag)::value;
+    constexpr bool FULL = decltype(full_tag)::value;
+    const int kend = FULL ? BTPM_BLK : min(BTPM_BLK, ntau - b * BTPM_BLK);
+#pragma unroll
+    for (int k = 0; k < BTPM_BLK; ++k) {
+      const float tp = lds_p[k][tid];
+      if (FULL || k < kend) {
+        const double tot = (double)(nsrc[k] + lds_q[k * kQCols + tid]);
+        float t;
+        if constexpr (INST) {  // roughness_inst_equilibrium, thermal_GPU.cu:581-641
+          t = (float)sqrt(sqrt(tot / cc.es));
+        } else {
+          // the per-step conversions leave out the residual (within 6e-8 of
+          // 1): a bounded perturbation of T_1 and T_0, not a gain on the profile
+          const float2 sk = cc.sk[k];
+          t = (float)conduct_step<NX>(u, kr, sk.x, sk.y, tot, cc.m1, cc.es, Trefd, cc.tol, cap,
+                                      (double)(tp + (2.0f * dl - dl2)));
+        }
+        after_step(k, tp, t);
+      }
+      if constexpr (FULL) next_src(b, k);
+    }
+    asm volatile("" ::: "memory");
+    if constexpr (!INST) {
+      const float f = block_scale(b, kend);
+#pragma unroll
+      for (int i = 0; i <= NX; ++i) u[i] *= f;
+    }
+  };
+
+  // the TI = 0 loop is a separate instantiation so the main loop stays lean
+  auto rotation = [&](auto inst_tag) {
+    for (int b = 0; b + 1 < nb; ++b) {
+      block_head(b, true);
+      valu_steps(b, inst_tag, std::true_type{});
+      block_tail(b);
+    }
+    block_head(nb - 1, false);
+    valu_steps(nb - 1, inst_tag, std::false_type{});
+    block_tail(nb - 1);
   };
   if (S.inst) rotation(std::true_type{});
   else rotation(std::false_type{});
