@@ -779,6 +779,108 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell_h32(
   selfheat_ell_body<32>(wseg, ent, nf, nchunk, slots, sets, status);
 }
 
+// The same product with the lanes along time: lane l of a wave holds step
+// s0 + l, and the wave's 32 receivers are accumulators acc[r] in every lane.
+// A receiver's entries are the same for all lanes of the wave, so they come
+// through the scalar cache and the wave walks exactly its receivers' entries
+// (no padding), and every entry reads one contiguous emission row segment
+// E[g][s0 .. s0+63] from LDS (consecutive lanes on consecutive banks).  The
+// workgroup (8 waves, 256 receivers x 64 steps) stages chunks of 256
+// emitters x 64 steps (64 KB), two workgroups per CU.  Each receiver adds its
+// entries in emitter order (chunks in order, the per-chunk CSR of
+// btpm_selfheat_tiled within a chunk), so the sums equal that kernel's bit
+// for bit.
+namespace {
+constexpr int kRowsR = 32;                   // receivers per wave
+constexpr int kRowsWaves = 8;                // waves per workgroup
+constexpr int kRowsF = kRowsR * kRowsWaves;  // receivers per workgroup
+constexpr int kRowsH = 64;                   // steps per workgroup (one per lane)
+static_assert(kRowsF == BTPM_SH_ROWS_T, "receiver block of the time-lane kernel");
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(kRowsWaves * 64) void btpm_selfheat_rows(
+    const int* __restrict__ cptr, const int2* __restrict__ ent, int nf, int nchunk,
+    const int* __restrict__ slots, const btpm_set_dev* __restrict__ sets,
+    const btpm_status_dev* __restrict__ status) {
+  constexpr int kPer = kSHChunk * kRowsH / 4 / (kRowsWaves * 64);  // float4 staged per lane
+  static_assert(kPer * kRowsWaves * 64 * 4 == kSHChunk * kRowsH, "staging shape");
+  __shared__ __align__(16) float E[kSHChunk * kRowsH];
+  const int slot = slots[blockIdx.z];
+  if (status[slot].done) return;
+  const btpm_set_dev& S = sets[slot];
+  const int npad = S.nb * BTPM_BLK;
+  const int s0 = blockIdx.y * kRowsH;
+  if (s0 >= npad) return;
+  const int tid = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int f0 = blockIdx.x * kRowsF + wave * kRowsR;  // the wave's first receiver
+  // live receivers of the wave: bit r for receiver f0 + r (uniform)
+  const int fl = f0 + lane;
+  const bool lv = lane < kRowsR && fl < nf && (S.inst || !S.frozen[fl]);
+  const unsigned long long live = __ballot(lv);
+  if (__syncthreads_count(lane == 0 && live != 0ull) == 0) return;  // all frozen
+  const float* __restrict__ emis = S.emis;
+  float4 pre[kPer];
+  auto fetch = [&](int c) {
+    const int g0 = c * kSHChunk;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int idx = tid + i * kRowsWaves * 64;      // float4 index in the chunk
+      const int row = idx / (kRowsH / 4), q4 = 4 * (idx % (kRowsH / 4));
+      pre[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (g0 + row < nf && s0 + q4 < npad)
+        pre[i] = *reinterpret_cast<const float4*>(emis + (size_t)(g0 + row) * npad + s0 + q4);
+    }
+  };
+  float acc[kRowsR];
+#pragma unroll
+  for (int r = 0; r < kRowsR; ++r) acc[r] = 0.0f;
+  fetch(0);
+  for (int c = 0; c < nchunk; ++c) {
+    const int g0 = c * kSHChunk;
+    __syncthreads();  // the previous chunk has been consumed
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+      *reinterpret_cast<float4*>(&E[4 * (tid + i * kRowsWaves * 64)]) = pre[i];
+    if (c + 1 < nchunk) fetch(c + 1);
+    __syncthreads();
+    // E[(g - g0) * kRowsH + lane]: emitter g of the chunk at this lane's step
+    auto row = [&](int g) { return E[(g - g0) * kRowsH + lane]; };
+#pragma unroll
+    for (int r = 0; r < kRowsR; ++r) {
+      if (!((live >> r) & 1ull)) continue;
+      const int f = f0 + r;
+      const int k0 = cptr[(size_t)c * nf + f], k1 = cptr[(size_t)(c + 1) * nf + f];
+      float a = acc[r];
+      int k = k0;
+      for (; k + 4 <= k1; k += 4) {
+        const int2 e0 = ent[k], e1 = ent[k + 1], e2 = ent[k + 2], e3 = ent[k + 3];
+        const float x0 = row(e0.x), x1 = row(e1.x), x2 = row(e2.x), x3 = row(e3.x);
+        a = fmaf(__int_as_float(e0.y), x0, a);
+        a = fmaf(__int_as_float(e1.y), x1, a);
+        a = fmaf(__int_as_float(e2.y), x2, a);
+        a = fmaf(__int_as_float(e3.y), x3, a);
+      }
+      for (; k < k1; ++k) {
+        const int2 e0 = ent[k];
+        a = fmaf(__int_as_float(e0.y), row(e0.x), a);
+      }
+      acc[r] = a;
+    }
+  }
+  const int s = s0 + lane;
+  if (s >= npad) return;
+  const float* __restrict__ src = S.src;
+  float* __restrict__ y = S.qext;
+#pragma unroll
+  for (int r = 0; r < kRowsR; ++r) {
+    if (!((live >> r) & 1ull)) continue;
+    const size_t o = ((size_t)(s / BTPM_BLK) * nf + (f0 + r)) * BTPM_BLK + (s % BTPM_BLK);
+    y[o] = src[o] + acc[r];
+  }
+}
+
 // elementwise helpers over blocked arrays of n floats
 //   y = gain * scale_c * x
 extern "C" __global__ void btpm_scale_cols(float* __restrict__ y, const float* __restrict__ x,

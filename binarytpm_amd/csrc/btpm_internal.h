@@ -25,6 +25,8 @@
 #define BTPM_SH_STEPS 64
 #define BTPM_SH_STRIDE (BTPM_SH_STEPS + 4)  // LDS row stride of a staged chunk (CSR kernel)
 #define BTPM_SH_WAVES (BTPM_SH_ROWS / 64)     // waves per self-heating workgroup
+#define BTPM_SH_ROWS_T 256  // receivers per workgroup of the time-lane kernel (btpm_selfheat_rows)
+#define BTPM_SH_STEPS_T 64  // steps per workgroup of the time-lane kernel (one per lane)
 // the lane-interleaved (ELL) self-heating layout is used while its padded
 // entry count stays within this multiple of the view-factor count
 #define BTPM_SH_ELL_MAX_PAD 4

@@ -243,6 +243,9 @@ struct btpm_shape {
   // stays within BTPM_SH_ELL_MAX_PAD times the view-factor count
   DBuf wseg, ell;
   bool use_ell = false;
+  // BTPM_SH_LAYOUT forcing one kernel (no tuning): -1 time-lane, 1 the
+  // lane-per-receiver kernel of the layout (ELL or per-chunk CSR), 0 tuned
+  int force_heat = 0;
   DBuf cvf32;      // [100][100] crater VF (fp32, vf[i*100+j] = VF(j->i))
   DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
   DBuf cillum;     // [nb][nf*100][BLK] crater illumination (after scattering)
@@ -561,7 +564,10 @@ struct Sched {
     btpm_shape* shape;
     int nsets;
     std::unique_ptr<DBuf> slots;
-    int steps = BTPM_SH_STEPS;  // steps per workgroup of the ELL kernel
+    // kernel: 0 = btpm_selfheat_rows (time along the lanes); 64 / 32 = steps
+    // per workgroup of the lane-per-receiver kernel (ELL, or the per-chunk
+    // CSR btpm_selfheat_tiled when the shape has no ELL layout)
+    int steps = 0;
     bool trial = false;
   };
   std::vector<HeatGroup> heat;
@@ -570,8 +576,9 @@ struct Sched {
   // Tile choices per size class, measured on the device:
   //   kind 0: smooth conduction columns per workgroup {256, 128, 64}, per
   //           column-count class (ceil log2 of the set's columns);
-  //   kind 1: facet self-heating (ELL kernel) steps per workgroup {64, 32},
-  //           per (log2 nnz, log2 facets, step blocks) class.
+  //   kind 1: facet self-heating kernel {time-lane (0), ELL 64 and 32 steps
+  //           per workgroup}, per (log2 nnz, log2 facets, step blocks) class
+  //           (shapes without an ELL layout: {time-lane, per-chunk CSR}).
   // Every candidate computes bit-identical results (only the launch shape
   // differs), so a trial is an ordinary rotation of the running solves:
   // while a class is untuned the scheduler syncs after every rotation, runs
@@ -879,9 +886,10 @@ struct Sched {
           want_trial(smooth_key(kv.second.second), {256, 128, 64});
         }
       for (auto& kv : by_shape)
-        if (kv.first->use_ell) {
+        if (!kv.first->force_heat) {
           present.insert(heat_key(kv.first));
-          want_trial(heat_key(kv.first), {64, 32});
+          want_trial(heat_key(kv.first),
+                     kv.first->use_ell ? std::vector<int>{0, 64, 32} : std::vector<int>{0, 64});
         }
       while (!trials.empty() && !present.count(trials.front().key)) trials.pop_front();
       if (!trials.empty()) trials.front().units = 0.0;
@@ -928,9 +936,13 @@ struct Sched {
     for (auto& kv : by_shape) {
       HeatGroup g{kv.first, (int)kv.second.size(), std::make_unique<DBuf>()};
       upload(*g.slots, kv.second.data(), kv.second.size(), ctx->stream);
-      if (kv.first->use_ell) {
+      if (kv.first->force_heat) {
+        g.steps = kv.first->force_heat < 0 ? 0 : BTPM_SH_STEPS;
+      } else {
         const TuneKey k = heat_key(kv.first);
-        g.steps = choice(k, BTPM_SH_STEPS);
+        // untuned: ELL when its padding is bounded, else the time-lane kernel
+        // (skewed rows: a few receivers see most of the shape)
+        g.steps = choice(k, kv.first->use_ell ? BTPM_SH_STEPS : 0);
         g.trial = on_trial(k);
         if (g.trial)
           for (int sl : kv.second) trials.front().units += ctx->h_status[sl].norder;
@@ -999,10 +1011,18 @@ struct Sched {
       // every smooth set of a shape in one launch
       for (const HeatGroup& g : fast ? fheat : heat) {
         btpm_shape* sh = g.shape;
-        dim3 grid(grid1(sh->nf, BTPM_SH_ROWS), grid1((size_t)sh->nb * BTPM_BLK, g.steps),
-                  g.nsets);
+        const dim3 grid(grid1(sh->nf, BTPM_SH_ROWS),
+                        grid1((size_t)sh->nb * BTPM_BLK, g.steps > 0 ? g.steps : BTPM_SH_STEPS),
+                        g.nsets);  // the lane-per-receiver kernels
         if (g.trial) trial_event(st, 0);
-        if (sh->use_ell && g.steps == 32)
+        if (g.steps == 0)
+          hipLaunchKernelGGL(btpm_selfheat_rows,
+                             dim3(grid1(sh->nf, BTPM_SH_ROWS_T),
+                                  grid1((size_t)sh->nb * BTPM_BLK, BTPM_SH_STEPS_T), g.nsets),
+                             dim3(BTPM_SH_ROWS_T / 32 * 64), 0, st, sh->cptr.as<int>(),
+                             sh->sent.as<int2>(), sh->nf, sh->nchunk, g.slots->as<int>(),
+                             ctx->d_sets, ctx->d_status);
+        else if (sh->use_ell && g.steps == 32)
           hipLaunchKernelGGL(btpm_selfheat_ell_h32, grid, dim3(BTPM_SH_ROWS), 0, st,
                              sh->wseg.as<int2>(), sh->ell.as<int2>(), sh->nf, sh->nchunk,
                              g.slots->as<int>(), ctx->d_sets, ctx->d_status);
@@ -1522,19 +1542,23 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
             ws[2 * i + 1] = n;
             total += (size_t)n * 64;
           }
-      // BTPM_SH_LAYOUT=csr|ell overrides the choice (diagnostics and tests)
+      // BTPM_SH_LAYOUT=csr|ell|rows forces one kernel (diagnostics and tests)
       const char* lay = std::getenv("BTPM_SH_LAYOUT");
       const bool fits = total < (size_t)INT32_MAX;
-      if (lay && std::strcmp(lay, "csr") == 0)
+      if (lay && std::strcmp(lay, "csr") == 0) {
         sh->use_ell = false;
-      else if (lay && std::strcmp(lay, "ell") == 0)
+        sh->force_heat = 1;
+      } else if (lay && std::strcmp(lay, "ell") == 0) {
         sh->use_ell = fits;
-      else
-        sh->use_ell = fits && total <= (size_t)BTPM_SH_ELL_MAX_PAD * body->nnz;
-      if (!sh->use_ell) {
-        upload(sh->sent, se.data(), se.size(), st);
-        upload(sh->cptr, cp.data(), cp.size(), st);
+        sh->force_heat = 1;
       } else {
+        sh->use_ell = fits && total <= (size_t)BTPM_SH_ELL_MAX_PAD * body->nnz;
+        if (lay && std::strcmp(lay, "rows") == 0) sh->force_heat = -1;
+      }
+      // the per-chunk CSR serves the time-lane kernel and btpm_selfheat_tiled
+      upload(sh->sent, se.data(), se.size(), st);
+      upload(sh->cptr, cp.data(), cp.size(), st);
+      if (sh->use_ell) {
         std::vector<int> el(2 * total, 0);
         for (int rb = 0; rb < nrb; ++rb)
           for (int c = 0; c < nc; ++c)

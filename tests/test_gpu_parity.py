@@ -81,11 +81,12 @@ def test_smooth_inst_parity(ctx):
     assert _report("smooth TI=0", s.surface_temps(), ref.stemp) <= TOL_K
 
 
-@pytest.mark.parametrize("layout", ["ell", "csr"])
+@pytest.mark.parametrize("layout", ["ell", "csr", "rows"])
 def test_self_heating_super_cycle_parity(ctx, monkeypatch, layout):
     """Binary-like body: 2:1 super-cycle (numrevs 2 and 1 on ntau = 1300),
-    sparse facet view factors, facet multiple scattering; both self-heating
-    kernel layouts (lane-interleaved ELL and per-chunk CSR)."""
+    sparse facet view factors, facet multiple scattering; every self-heating
+    kernel (lane per receiver over the lane-interleaved ELL or the per-chunk
+    CSR, and lane per step over the per-chunk CSR)."""
     monkeypatch.setenv("BTPM_SH_LAYOUT", layout)
     nt = 1300
     Vp, Fp = inputs.ellipsoid(1, (0.6, 0.5, 0.45))
@@ -129,23 +130,27 @@ def _skewed_viewfactors(nf, seed=11):
 
 def test_self_heating_layouts_bitwise(ctx, monkeypatch):
     """1280 facets (5 receiver blocks x 5 emitter chunks) with skewed view
-    factor rows: the automatic choice (per-chunk CSR, since the ELL padding
-    would exceed its limit) and the forced ELL layout add each row's entries
-    in the same order, so their temperatures agree bit for bit; both match
-    the oracle."""
+    factor rows: the per-chunk CSR kernel (the ELL padding would exceed its
+    limit), the forced ELL layout, the time-lane kernel and the tuned choice
+    add each row's entries in the same order, so their temperatures agree
+    bit for bit; they match the oracle."""
     case, _ = inputs.serving_shape(3, 5)
     nf = case.illum.shape[0]
     csr = _skewed_viewfactors(nf)
     p = th.params(case.ntau, case.rot_period, tacc=0.01)
     out = {}
-    for layout in (None, "ell"):
+    for layout in ("csr", "ell", "rows", None):
         if layout is None:
             monkeypatch.delenv("BTPM_SH_LAYOUT", raising=False)
         else:
             monkeypatch.setenv("BTPM_SH_LAYOUT", layout)
         shape = th.Shape(ctx, p, case.illum, 0.1, csr=csr, scatter=1)
-        out[layout] = th.Solve(ctx, shape, 200.0).run().surface_temps()
-    assert np.array_equal(out[None], out["ell"])
+        s = th.Solve(ctx, shape, 200.0).run()
+        out[layout] = s.surface_temps()
+        s.close()
+        shape.close()
+    for layout in ("ell", "rows", None):
+        assert np.array_equal(out["csr"], out[layout]), layout
     F = O.facet_scatter(case.illum, 0.1, csr, multiple=False)
     ref = O.smooth_solve(O.params(case.ntau, case.rot_period, tacc=0.01), F, 0.1, 200.0, csr=csr)
     assert _report("skewed self-heating", out["ell"], ref.stemp) <= TOL_K
@@ -305,11 +310,12 @@ def test_crater_sun_table_and_flux_scale(ctx, rough80):
 
 
 def test_autotune_identical_results(rough80):
-    """The autotuner's trial rotations run candidate tile widths on the live
-    solves (smooth conduction 256/128/64 columns per workgroup, self-heating
-    64/32 steps per workgroup); every candidate does the same arithmetic, so a
-    tuned batch equals an untuned one bit for bit.  Decisions are kept in the
-    context: a second batch on it runs no trials."""
+    """The autotuner's trial rotations run candidate kernels on the live
+    solves (smooth conduction 256/128/64 columns per workgroup; self-heating
+    time-lane, ELL 64 and ELL 32 steps per workgroup, each candidate twice in
+    palindromic order); every candidate does the same arithmetic in the same
+    order, so a tuned batch equals an untuned one bit for bit.  Decisions are
+    kept in the context: a second batch on it runs no trials."""
     case, crater = rough80
     p = th.params(650, case.rot_period, tacc=0.01)
     cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
@@ -325,7 +331,7 @@ def test_autotune_identical_results(rough80):
             print(f"[tune] autotune={tune}: {st.tune_decisions} decisions, "
                   f"{st.tune_rotations} trial rotations", flush=True)
             if tune:
-                assert st.tune_decisions == 2 and st.tune_rotations == 10
+                assert st.tune_decisions == 2 and st.tune_rotations == 12
                 again = [th.Solve(c2, shape, 120.0, with_craters=True)]
                 st2 = th.batch_run(c2, again, th.batch_opts(autotune=True))
                 assert st2.tune_decisions == 0 and st2.tune_rotations == 0
