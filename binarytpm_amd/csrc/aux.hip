@@ -32,6 +32,14 @@ extern "C" __global__ void btpm_pack_blocked(const float* __restrict__ in, float
   o[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
 
+// Every quantity the unpack kernels move out of the device (surface and
+// crater temperatures, absorbed fluxes) is non-negative; values are floored
+// at +0.  A column that never receives energy cools towards 0 K, where the
+// solver's rounding can leave -1e-15 K or -0: a Planck function downstream
+// (the Observe stage's exp(hc / (lambda k T)) - 1) turns any T <= -0 into a
+// large negative intensity instead of the zero it gives at +0.  NaN passes.
+__device__ __forceinline__ float floor0(float v) { return v <= 0.0f ? 0.0f : v; }
+
 extern "C" __global__ void btpm_unpack_blocked(const float* __restrict__ in, float* __restrict__ out,
                                                int ncols, int ntau, int nb) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -42,7 +50,7 @@ extern "C" __global__ void btpm_unpack_blocked(const float* __restrict__ in, flo
 #pragma unroll
   for (int k = 0; k < BTPM_BLK; ++k) {
     const int s = b * BTPM_BLK + k;
-    if (s < ntau) out[(size_t)c * ntau + s] = p[k];
+    if (s < ntau) out[(size_t)c * ntau + s] = floor0(p[k]);
   }
 }
 
@@ -78,7 +86,7 @@ extern "C" __global__ void btpm_unpack_crater(const float* __restrict__ in, floa
 #pragma unroll
   for (int k = 0; k < BTPM_BLK; ++k) {
     const int s = b * BTPM_BLK + k;
-    if (s < ntau) out[((size_t)f * ntau + s) * BTPM_NS + j] = p[k];
+    if (s < ntau) out[((size_t)f * ntau + s) * BTPM_NS + j] = floor0(p[k]);
   }
 }
 
@@ -95,7 +103,7 @@ extern "C" __global__ void btpm_unpack_crater_rows(const float* __restrict__ in,
   float* o = out + (size_t)c * ntau + (size_t)b * BTPM_BLK;
 #pragma unroll
   for (int k = 0; k < BTPM_BLK; ++k)
-    if (b * BTPM_BLK + k < ntau) o[k] = p[k];
+    if (b * BTPM_BLK + k < ntau) o[k] = floor0(p[k]);
 }
 
 // ---------------------------------------------------------------------------

@@ -291,7 +291,10 @@ int btpm_solve_run_stepwise(btpm_solve *s, int rotation_cap);
 int btpm_solve_get_info(btpm_solve *s, btpm_solve_info *info);
 /* final surface temperatures: smooth [nf][ntau] (writeglobaltemperature's
  * values), crater [nf][ntau][100] (obtain_GPU_roughness_temperatures'
- * sfc_stemp1), depth profiles [nf][nx+1] / [nf][nx+1][100]                   */
+ * sfc_stemp1), depth profiles [nf][nx+1] / [nf][nx+1][100].  Surface
+ * temperatures are floored at +0 K (a facet the Sun never reaches cools to
+ * 0 K, where rounding can leave -1e-15 K or -0; a Planck function turns
+ * those into large negative intensities).                                   */
 int btpm_solve_surface_temps(btpm_solve *s, float *out);
 int btpm_solve_crater_temps(btpm_solve *s, float *out);
 int btpm_solve_profile(btpm_solve *s, float *out);

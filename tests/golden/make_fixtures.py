@@ -9,7 +9,9 @@ Outputs (committed): tests/golden/new_crater2.npz,
 tests/golden/doppler_binary.npz, tests/golden/doppler_binary_properties.json,
 tests/golden/fg3_observe.npz (WISE colour corrections and the 1996 FG3 mesh),
 tests/golden/fg3_flux_set4.npz (shipped 1996 FG3 light curves of set 4 at 10
-smooth and 6 rough thermal inertias)
+smooth and 6 rough thermal inertias), tests/golden/fg3_flux_sets013.npz (the
+shipped curves of observation sets 0, 1 and 3, smooth and rough at TI 0 and
+100)
 """
 import json
 import os
@@ -51,6 +53,15 @@ def main():
             flux[f"{kind}_{ti}"] = np.loadtxt(
                 os.path.join(R, f"1996FG3_binary_lowres_{kind}_flux({ti}_4).txt")).astype(np.float32)
     np.savez_compressed(os.path.join(HERE, "fg3_flux_set4.npz"), **flux)
+    # observation sets 0, 1 and 3 (their Sun and observer positions are in
+    # Run-00, Run-01 and Run-03; the curves ship in Run-04 with the others)
+    flux = {}
+    for obs_set in (0, 1, 3):
+        for kind in ("smooth", "rough"):
+            for ti in (0, 100):
+                flux[f"{kind}_{ti}_set{obs_set}"] = np.loadtxt(os.path.join(
+                    R, f"1996FG3_binary_lowres_{kind}_flux({ti}_{obs_set}).txt")).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "fg3_flux_sets013.npz"), **flux)
     props = {}
     with open(os.path.join(D, "doppler_binary_properties.txt")) as fh:
         for line in fh:

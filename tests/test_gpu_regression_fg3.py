@@ -40,6 +40,17 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 PI = 4.0 * math.atan(1.0)
 HEL = (1.160088, (-1.9521 / 180.0) * PI, (198.3142 / 180.0) * PI)
 OBS = (0.583905, (-3.882500 / 180.0) * PI, (257.492321 / 180.0) * PI)
+# the other observation sets whose runs ship their geometry: heliocentric and
+# geocentric polar positions (175706/Run-0<set>/thermal.cpp:29-31 and
+# observe_generator.cpp:28-33; Run-02 is not in the tree)
+OTHER_SETS = {
+    0: ((1.219839, (-1.8739 / 180.0) * PI, (190.1486 / 180.0) * PI),
+        (0.611435, (-3.741913 / 180.0) * PI, (134.857558 / 180.0) * PI)),
+    1: ((1.149516, (1.0595 / 180.0) * PI, (87.6639 / 180.0) * PI),
+        (0.595065, (2.050070 / 180.0) * PI, (147.127494 / 180.0) * PI)),
+    3: ((1.259034, (-1.8002 / 180.0) * PI, (184.3755 / 180.0) * PI),
+        (0.773389, (-2.932057 / 180.0) * PI, (236.378553 / 180.0) * PI)),
+}
 ROT = ((-83.0 / 180.0) * PI, (266.0 / 180.0) * PI)
 KW = dict(nv_primary=1026, rev_primary=9.0, rev_secondary=2.0)
 NTAU = 5850
@@ -174,5 +185,59 @@ def test_fg3_rough_light_curves(ctx, fg3):
         prev.close()
         print(f"[regression] warm rough sweep 10..400 in {time.time() - t0:.1f} s, rotations "
               f"(smooth, crater): {rot}", flush=True)
+    finally:
+        shape.close()
+
+
+@pytest.mark.parametrize("obs_set", sorted(OTHER_SETS))
+def test_fg3_other_observation_sets(ctx, fg3, obs_set):
+    """Observation sets 0, 1 and 3 (fixture tests/golden/fg3_flux_sets013.npz):
+    the same chain at another Sun and observer position each, smooth and rough
+    at TI 0 and at TI 100 (smooth reached by the warm sweep 10..100, the rough
+    solve started cold: the periodic state it converges to agrees with the
+    warm one within the 0.001 K tolerance)."""
+    hel, obs = OTHER_SETS[obs_set]
+    ref = np.load(os.path.join(GOLD, "fg3_flux_sets013.npz"))
+    V, F, cc, c = fg3["V"], fg3["F"], fg3["cc"], fg3["c"]
+    sun = th.geometry_illumination(ctx, V, F, NTAU, hel, ROT, sun_scale=10000.0, **KW)
+    seen = th.geometry_illumination(ctx, V, F, NTAU, obs, ROT, sun_scale=10000.0, **KW)
+    cV, cF = c["V"].astype(np.float32), c["F"].astype(np.int32)
+    csh, ccos = th.crater_sunlight(ctx, sun.shadow, sun.sun_facet, cV, cF)
+    cvis, _ = th.crater_sunlight(ctx, seen.shadow, seen.sun_facet, cV, cF)
+    cvf = formats.crater_vf_matrix((c["rowptr"], c["colidx"], c["vf"]))
+    cm = th.CraterModel(vf=cvf, shadow=csh, cosang=ccos, solar=sun.solar, multiple=True,
+                        scatter_acc=1e-4)
+    shape = th.Shape(ctx, fg3["p"], sun.illum, 0.01, numrevs=fg3["numrevs"], scatter=0, crater=cm)
+    del csh, ccos, cm
+
+    def flux(s, crater):
+        kw = dict(crater=(cV, cF, cvis, s.crater_temps())) if crater else {}
+        return th.observe(ctx, V, F, NTAU, hel, obs, ROT, seen.shadow, _text6(s.surface_temps()),
+                          cc, **KW, **kw)
+
+    def key(kind, ti):
+        return ref[f"{kind}_{ti}_set{obs_set}"].astype(np.float64)
+
+    try:
+        for ti in (0.0, 100.0):
+            s = th.Solve(ctx, shape, ti, with_craters=True).run()
+            inf = s.info()
+            assert inf.converged and inf.crater_converged
+            sm, ro = flux(s, True)
+            _check(f"set {obs_set} rough TI {ti:g} ({inf.crater_rotations} crater rotations)", ro,
+                   key("rough", int(ti)))
+            if ti == 0.0:
+                _check(f"set {obs_set} smooth TI 0", sm, key("smooth", 0))
+            s.close()
+        prev = None
+        for ti in range(10, 110, 10):
+            s = th.Solve(ctx, shape, float(ti), warm_from=prev).run()
+            assert s.info().converged
+            if prev is not None:
+                prev.close()
+            prev = s
+        sm, _ = flux(prev, False)
+        _check(f"set {obs_set} smooth TI 100 (warm sweep)", sm, key("smooth", 100))
+        prev.close()
     finally:
         shape.close()
