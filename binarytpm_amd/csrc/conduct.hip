@@ -48,8 +48,9 @@ namespace {
 // operations.  The per-step factor is made consistent with the rounded kr as
 // sig = 1 / (1 + 2 kr) (so constants stay exactly constant), tracked in FP64
 // with its inverse, and folded back into U once per 8-step block (U grows by
-// at most c^-8 < 1e32 in between for r <= BTPM_R_MAX).  The zero-flux bottom
-// keeps the copy node u[NX] = u[NX-1] of the reference.
+// at most c^-8 < 1e32 in between for r <= BTPM_R_MAX); the crater kernel
+// folds every 1 to 3 blocks, as rarely as the FP32 range allows for its r.
+// The zero-flux bottom keeps the copy node u[NX] = u[NX-1] of the reference.
 template <int NX>
 __device__ __forceinline__ void ftcs_scaled(float (&u)[NX + 1], float kr) {
   float before = u[0];
@@ -81,14 +82,14 @@ __device__ __forceinline__ Scale make_scale(float r) {
   return sc;
 }
 
-// Scale of the 8 steps of a block for one r, shared by all columns with that
-// r: sk[k] = (sig^(k+1), sig^-(k+1)) in FP32 for the per-step conversions and
-// pk[k] = sig^(k+1) in FP64 for the rescale at the block end.
-__device__ __forceinline__ void scale_table(float r, float2* sk, double* pk) {
+// Scale of the n steps between two rescales for one r, shared by all columns
+// with that r: sk[k] = (sig^(k+1), sig^-(k+1)) in FP32 for the per-step
+// conversions and pk[k] = sig^(k+1) in FP64 for the rescale.
+__device__ __forceinline__ void scale_table(float r, float2* sk, double* pk, int n) {
   const double isig = fma(2.0, (double)scaled_kr(r), 1.0);
   const double sig = 1.0 / isig;
   double p = 1.0, ip = 1.0;
-  for (int k = 0; k < BTPM_BLK; ++k) {
+  for (int k = 0; k < n; ++k) {
     p *= sig;
     ip *= isig;
     sk[k] = make_float2((float)p, (float)ip);
@@ -96,10 +97,21 @@ __device__ __forceinline__ void scale_table(float r, float2* sk, double* pk) {
   }
 }
 
+// The crater kernel folds the scale back into the profile every 1 to
+// kRescaleMax blocks: as rarely as U <= 1e4 K x sig^-(8 blocks) stays inside
+// the FP32 range for every crater of the workgroup (sig = 1 - 2r).
+constexpr int kRescaleMax = 3;
+__device__ __forceinline__ int rescale_blocks(float r) {
+  const double c = 1.0 - 2.0 * (double)r;
+  const double c2 = c * c, c4 = c2 * c2, c8 = c4 * c4;
+  const double lim = 1e4 / 3.4e38;  // 1e4 K / FLT_MAX
+  return c8 * c8 * c8 > lim ? 3 : c8 * c8 > lim ? 2 : 1;
+}
+
 // Per-crater constants of the crater kernel, shared by the crater's 100 lanes.
 struct CraterConst {
-  float2 sk[BTPM_BLK];  // scale_table
-  double pk[BTPM_BLK];
+  float2 sk[kRescaleMax * BTPM_BLK];  // scale_table
+  double pk[kRescaleMax * BTPM_BLK];
   double r0[2];         // residual of the scale carried from the last rescale
   double m1;            // Gamma sqrt(2 pi n / P) / dz
   double es, tol;       // emissivity x sigma and the Newton tolerance, read from
@@ -504,6 +516,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   __shared__ CraterConst lds_cc[P];            // per-crater constants
   __shared__ unsigned lds_cmax[P];
   __shared__ int lds_cnt;
+  __shared__ int lds_nres[P];                  // blocks per rescale per crater
 
   const btpm_tile tl = tiles[blockIdx.x];
   const btpm_set_dev& S = sets[tl.set];
@@ -543,7 +556,8 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
     if (tid < P) {
       lds_cmax[tid] = 0u;
       const int gt = S.order[tl.tile * P + tid < norder ? tl.tile * P + tid : last];
-      scale_table(S.r[gt], lds_cc[tid].sk, lds_cc[tid].pk);
+      scale_table(S.r[gt], lds_cc[tid].sk, lds_cc[tid].pk, kRescaleMax * BTPM_BLK);
+      lds_nres[tid] = rescale_blocks(S.r[gt]);
       lds_cc[tid].r0[0] = 1.0;
       lds_cc[tid].m1 = (double)S.m1[gt];
       lds_cc[tid].es = (double)S.es;
@@ -558,6 +572,11 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   const float kr = scaled_kr(S.r[gg]);
   CraterConst& cc = lds_cc[lcr];
   const int cap = S.newton_cap;
+  int nres = kRescaleMax;  // blocks per rescale: the workgroup's smallest
+  for (int i = 0; i < P; ++i) nres = min(nres, lds_nres[i]);
+  nres = __builtin_amdgcn_readfirstlane(nres);
+  int bph = 0;   // blocks since the last rescale
+  int rpar = 0;  // rescales so far, mod 2
   const float esia = S.esia;
   const int ncols = S.ncols;
 
@@ -622,13 +641,15 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
       store8(Tout + (size_t)b * bstride + lane_off(), out);
     }
   };
-  // U *= f = fp32(r0 sig^kend) at the block end; the residual r0 sig^kend / f
-  // is carried (cc.r0 is double buffered by block parity: every lane reads one
-  // half, sub-facet 0 writes the other, and the block's barriers order them)
-  auto block_scale = [&](int b, int kend) {
-    const double pb = cc.r0[b & 1] * cc.pk[kend - 1];
+  // U *= f = fp32(r0 sig^n) at a rescale, n the steps since the last one; the
+  // residual r0 sig^n / f is carried (cc.r0 is double buffered by rescale
+  // parity: every lane reads one half, sub-facet 0 writes the other, and the
+  // barriers of the blocks in between order them)
+  auto block_scale = [&](int n) {
+    const double pb = cc.r0[rpar] * cc.pk[n - 1];
     const float f = (float)pb;
-    if (j == 0) cc.r0[(b + 1) & 1] = fma(pb - (double)f, (double)__builtin_amdgcn_rcpf(f), 1.0);
+    if (j == 0) cc.r0[rpar ^ 1] = fma(pb - (double)f, (double)__builtin_amdgcn_rcpf(f), 1.0);
+    rpar ^= 1;
     return f;
   };
   auto after_step = [&](int k, float tp, float t) {
@@ -650,6 +671,8 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
     constexpr bool INST = decltype(inst_tag)::value;
     constexpr bool FULL = decltype(full_tag)::value;
     const int kend = FULL ? BTPM_BLK : min(BTPM_BLK, ntau - b * BTPM_BLK);
+    // this block's row of the scale table (steps since the last rescale)
+    const float2* sk8 = &cc.sk[bph * BTPM_BLK];
 #pragma unroll
     for (int k = 0; k < BTPM_BLK; ++k) {
       const float tp = lds_p[k][tid];
@@ -661,7 +684,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
         } else {
           // the per-step conversions leave out the residual (within 6e-8 of
           // 1): a bounded perturbation of T_1 and T_0, not a gain on the profile
-          const float2 sk = cc.sk[k];
+          const float2 sk = sk8[k];
           t = (float)conduct_step<NX>(u, kr, sk.x, sk.y, tot, cc.m1, cc.es, Trefd, cc.tol, cap,
                                       (double)(tp + (2.0f * dl - dl2)));
         }
@@ -671,9 +694,16 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
     }
     asm volatile("" ::: "memory");
     if constexpr (!INST) {
-      const float f = block_scale(b, kend);
+      // every nres blocks, and at the end of the rotation (the profile goes
+      // out unscaled)
+      if (!FULL || bph + 1 == nres) {
+        const float f = block_scale(bph * BTPM_BLK + kend);
 #pragma unroll
-      for (int i = 0; i <= NX; ++i) u[i] *= f;
+        for (int i = 0; i <= NX; ++i) u[i] *= f;
+        bph = 0;
+      } else {
+        ++bph;
+      }
     }
   };
 
