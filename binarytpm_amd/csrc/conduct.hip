@@ -50,17 +50,21 @@ namespace {
 // with its inverse, and folded back into U once per 8-step block (U grows by
 // at most c^-8 < 1e32 in between for r <= BTPM_R_MAX); the crater kernel
 // folds every 1 to 3 blocks, as rarely as the FP32 range allows for its r.
-// The zero-flux bottom keeps the copy node u[NX] = u[NX-1] of the reference.
+// The zero-flux bottom is the reference's copy node u[NX] = u[NX-1]: node
+// NX-1 sees its own old value in its place, and u[NX] itself is not carried
+// (a profile enters with u[NX] = u[NX-1], which every FD step leaves so, and
+// leaves through store_profile with the copy).
 template <int NX>
 __device__ __forceinline__ void ftcs_scaled(float (&u)[NX + 1], float kr) {
   float before = u[0];
 #pragma unroll
-  for (int i = 1; i < NX; ++i) {
+  for (int i = 1; i < NX - 1; ++i) {
     const float cur = u[i];
     u[i] = fmaf(kr, u[i + 1] + before, cur);
     before = cur;
   }
-  u[NX] = u[NX - 1];
+  const float cur = u[NX - 1];
+  u[NX - 1] = fmaf(kr, cur + before, cur);
 }
 
 // Per-column scale state of the scaled recurrence: real offset = sd * U.
@@ -125,7 +129,7 @@ template <int NX>
 __device__ __forceinline__ void rescale(float (&u)[NX + 1], Scale& sc) {
   const float f = (float)sc.sd;
 #pragma unroll
-  for (int i = 0; i <= NX; ++i) u[i] *= f;
+  for (int i = 0; i < NX; ++i) u[i] *= f;
   sc.isd *= (double)f;
   sc.sd = 2.0 - sc.isd;
 }
@@ -168,7 +172,7 @@ __device__ __forceinline__ double conduct_step(float (&u)[NX + 1], float kr, SC 
   double x = fmax(x0, 1.0);
   const double d0 = btpm_dev::newton_step(K, m1, es, x);
 #pragma unroll
-  for (int i = 2; i <= NX; ++i) asm volatile("" : "+v"(u[i]));
+  for (int i = 2; i < NX; ++i) asm volatile("" : "+v"(u[i]));
   x = btpm_dev::newton_finish(K, m1, es, x, d0, tol, cap);
   if constexpr (std::is_same<SC, double>::value)
     u[0] = (float)((x - Tref) * is);
@@ -194,14 +198,16 @@ __device__ __forceinline__ float load_profile(const gfloat* rec, float (&u)[NX +
   return tref;
 }
 
+// (node NX is written as the copy of node NX-1)
 template <int NX>
 __device__ __forceinline__ void store_profile(gfloat* rec, const float (&u)[NX + 1], float tref) {
+  auto node = [&](int i) { return tref + u[i < NX ? i : NX - 1]; };
 #pragma unroll
   for (int i = 0; i + 3 <= NX; i += 4)
     *reinterpret_cast<GLOBAL_AS float4_t*>(rec + i) =
-        float4_t{tref + u[i], tref + u[i + 1], tref + u[i + 2], tref + u[i + 3]};
+        float4_t{node(i), node(i + 1), node(i + 2), node(i + 3)};
 #pragma unroll
-  for (int i = (NX + 1) & ~3; i <= NX; ++i) rec[i] = tref + u[i];
+  for (int i = (NX + 1) & ~3; i <= NX; ++i) rec[i] = node(i);
 }
 
 __device__ __forceinline__ void store8(gfloat* p, const float (&v)[BTPM_BLK]) {
@@ -699,7 +705,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
       if (!FULL || bph + 1 == nres) {
         const float f = block_scale(bph * BTPM_BLK + kend);
 #pragma unroll
-        for (int i = 0; i <= NX; ++i) u[i] *= f;
+        for (int i = 0; i < NX; ++i) u[i] *= f;
         bph = 0;
       } else {
         ++bph;
