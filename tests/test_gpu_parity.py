@@ -203,15 +203,16 @@ def rough80():
     return inputs.rough_primary(subdiv=1)          # 80 facets x 100 sub-facets
 
 
-def _oracle_rough(case, crater, ti, tacc=0.01):
-    p = O.params(650, case.rot_period, tacc=tacc)
+def _oracle_rough(case, crater, ti, tacc=0.01, ntau=650):
+    p = O.params(ntau, case.rot_period, tacc=tacc)
     nf = case.illum.shape[0]
+    nfc = crater["shadow"].shape[0]   # crater parents: the first nfc facets
     F = O.facet_scatter(case.illum, 0.1, case.csr, multiple=True, acc=1e-4)
     sm = O.smooth_solve(p, F, 0.1, ti, csr=case.csr)
-    rgi = O.rgi(case.csr, crater["drf"], F, np.full(nf, 0.1), nf)
+    rgi = O.rgi(case.csr, crater["drf"], F, np.full(nf, 0.1), nfc)
     il = O.crater_scatter(crater["vf"], 0.1, crater["shadow"], crater["cosang"], rgi,
                           crater["solar"], multiple=True, acc=1e-4)
-    rgh = O.rgh(case.csr, crater["drf"], sm.stemp, ES, nf)
+    rgh = O.rgh(case.csr, crater["drf"], sm.stemp, ES, nfc)
     cr = O.crater_solve(p, crater["vf"], 0.1, ti, il, rgh)
     return F, sm, il, cr
 
@@ -232,6 +233,27 @@ def test_crater_parity(ctx, rough80, ti):
     assert info.converged and info.crater_converged
     assert _report(f"crater TI={ti}", s.crater_temps(), cr.stemp) <= TOL_K
     assert abs(info.crater_rotations - cr.rotations) <= 1
+
+
+@pytest.mark.parametrize("ntau", [616, 630])
+def test_crater_parity_rescale_interval(ctx, ntau):
+    """The crater kernel folds its FTCS scale back into the profile every 1, 2
+    or 3 blocks, as rarely as the FP32 range allows for r = dtau/dz^2:
+    ntau = 616 gives r = 0.49980 (every block), ntau = 630 r = 0.48870 (every
+    2 blocks); the 650-step cases above (r = 0.47366) take every 3."""
+    case, crater = inputs.rough_primary(subdiv=1, ntau=ntau)
+    p = th.params(ntau, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
+                        solar=crater["solar"], multiple=True, scatter_acc=1e-4,
+                        drf=crater["drf"])
+    shape = th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2, crater=cm)
+    s = th.Solve(ctx, shape, 200.0, with_craters=True).run()
+    F, sm, il, cr = _oracle_rough(case, crater, 200.0, ntau=ntau)
+    info = s.info()
+    assert info.converged and info.crater_converged
+    assert _report(f"crater ntau={ntau}", s.crater_temps(), cr.stemp) <= TOL_K
+    assert abs(info.crater_rotations - cr.rotations) <= 1
+    shape.close()
 
 
 def test_crater_ragged_workgroup(ctx, rough80):
