@@ -7,12 +7,16 @@
            facets see most of the body)
 
 for each kernel forced through BTPM_SH_LAYOUT: csr (lane per receiver over
-the per-chunk CSR), ell (lane per receiver over the lane-interleaved ELL) and
-rows (lane per step over the per-chunk CSR).  Prints the average kernel time
-per launch from the library's own event timing.
+the per-chunk CSR), ell / ell32 (lane per receiver over the lane-interleaved
+ELL, 64 / 32 steps per workgroup), grp (8 receivers per wave over the
+receiver-grouped layout) and rows (lane per step over the per-chunk CSR).  Prints the average kernel time per launch
+from the library's own event timing.
 
 Usage (one GPU): python -m binarytpm_amd.bench_tools.selfheat_layouts
+[--pattern uniform|skewed] [--layouts csr,ell,...] (default: both patterns,
+every layout)
 """
+import argparse
 import os
 
 import numpy as np
@@ -39,14 +43,28 @@ def skewed_viewfactors(nf, seed=11, every=97, k=20):
             np.concatenate(vals).astype(np.float32))
 
 
+LAYOUTS = ("csr", "ell", "ell32", "grp", "rows")
+
+
 def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pattern", choices=("uniform", "skewed"), default=None)
+    ap.add_argument("--layouts", default=",".join(LAYOUTS))
+    args = ap.parse_args()
+    layouts = [x for x in args.layouts.split(",") if x]
+    for x in layouts:
+        if x not in LAYOUTS:
+            ap.error(f"unknown layout {x}")
     case, _ = inputs.serving_shape(4, 7)
     nf = case.illum.shape[0]
     p = th.params(case.ntau, case.rot_period, tacc=0.01)
-    patterns = (("uniform", inputs.sparse_viewfactors(nf, 0.03, 0.2, 1)[0]),
-                ("skewed", skewed_viewfactors(nf)))
-    for name, csr in patterns:
-        for lay in ("csr", "ell", "rows"):
+    patterns = (("uniform", lambda: inputs.sparse_viewfactors(nf, 0.03, 0.2, 1)[0]),
+                ("skewed", lambda: skewed_viewfactors(nf)))
+    for name, make in patterns:
+        if args.pattern and name != args.pattern:
+            continue
+        csr = make()
+        for lay in layouts:
             os.environ["BTPM_SH_LAYOUT"] = lay
             with th.Context(0) as ctx:
                 shape = th.Shape(ctx, p, case.illum, 0.1, csr=csr, scatter=0)

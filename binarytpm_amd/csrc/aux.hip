@@ -608,6 +608,27 @@ extern "C" __global__ __launch_bounds__(256) void btpm_spmm_blocked(
 // pairs; cptr[c][f] = first entry of row f in chunk c).  The staged emission is reused by every receiver of the block,
 // instead of being gathered from L2 once per view factor.
 namespace {
+// Global-memory access to a set's arrays.  Through a generic pointer (the
+// set table holds generic ones) loads compile to FLAT instructions, which
+// count on the LDS / scalar-load wait counter as well: every wait for an LDS
+// read or a scalar load would then also wait for the next chunk's emission
+// rows from HBM, the prefetch meant to overlap it.
+typedef float gvec4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gload4(const float* p) {
+  const gvec4 v = *reinterpret_cast<const __attribute__((address_space(1))) gvec4*>(
+      (const __attribute__((address_space(1))) float*)p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore4(float* p, float4 v) {
+  *reinterpret_cast<__attribute__((address_space(1))) gvec4*>(
+      (__attribute__((address_space(1))) float*)p) = gvec4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ float gload1(const float* p) {
+  return *(const __attribute__((address_space(1))) float*)p;
+}
+__device__ __forceinline__ void gstore1(float* p, float v) {
+  *(__attribute__((address_space(1))) float*)p = v;
+}
 constexpr int kSHRows = BTPM_SH_ROWS;    // receivers per workgroup (one per lane)
 constexpr int kSHChunk = BTPM_SH_CHUNK;  // emitters per staged chunk
 constexpr int kSHSteps = BTPM_SH_STEPS;  // steps per workgroup
@@ -640,7 +661,7 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
       const int gl = e / (kSHSteps / 4), q4 = 4 * (e % (kSHSteps / 4));
       float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (g0 + gl < nf && s0 + q4 < npad)
-        v = *reinterpret_cast<const float4*>(emis + (size_t)(g0 + gl) * npad + s0 + q4);
+        v = gload4(emis + (size_t)(g0 + gl) * npad + s0 + q4);
       *reinterpret_cast<float4*>(&E[gl * kSHStride + q4]) = v;
     }
     __syncthreads();
@@ -668,13 +689,13 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_tiled(
   for (int bq = 0; bq < kSHSteps / BTPM_BLK; ++bq) {
     if (s0 + bq * BTPM_BLK >= npad) break;
     const size_t o = ((size_t)(s0 / BTPM_BLK + bq) * nf + f) * BTPM_BLK;
-    const float4 a = *reinterpret_cast<const float4*>(src + o);
-    const float4 b = *reinterpret_cast<const float4*>(src + o + 4);
+    const float4 a = gload4(src + o);
+    const float4 b = gload4(src + o + 4);
     const int q = bq * BTPM_BLK;
-    *reinterpret_cast<float4*>(y + o) =
-        make_float4(a.x + acc[q], a.y + acc[q + 1], a.z + acc[q + 2], a.w + acc[q + 3]);
-    *reinterpret_cast<float4*>(y + o + 4) =
-        make_float4(b.x + acc[q + 4], b.y + acc[q + 5], b.z + acc[q + 6], b.w + acc[q + 7]);
+    gstore4(y + o,
+            make_float4(a.x + acc[q], a.y + acc[q + 1], a.z + acc[q + 2], a.w + acc[q + 3]));
+    gstore4(y + o + 4,
+            make_float4(b.x + acc[q + 4], b.y + acc[q + 5], b.z + acc[q + 6], b.w + acc[q + 7]));
   }
 }
 
@@ -723,7 +744,7 @@ __device__ __forceinline__ void selfheat_ell_body(const int2* __restrict__ wseg,
     for (int i = 0; i < kPer; ++i) {
       const int g = g0 + srow + i * (kSHRows / kQ4);
       pre[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (g < nf && scol) pre[i] = *reinterpret_cast<const float4*>(emis + (size_t)g * npad + s0 + sq4);
+      if (g < nf && scol) pre[i] = gload4(emis + (size_t)g * npad + s0 + sq4);
     }
   };
   float acc[H];
@@ -766,13 +787,13 @@ __device__ __forceinline__ void selfheat_ell_body(const int2* __restrict__ wseg,
   for (int bq = 0; bq < H / BTPM_BLK; ++bq) {
     if (s0 + bq * BTPM_BLK >= npad) break;
     const size_t o = ((size_t)(s0 / BTPM_BLK + bq) * nf + f) * BTPM_BLK;
-    const float4 a = *reinterpret_cast<const float4*>(src + o);
-    const float4 b = *reinterpret_cast<const float4*>(src + o + 4);
+    const float4 a = gload4(src + o);
+    const float4 b = gload4(src + o + 4);
     const int q = bq * BTPM_BLK;
-    *reinterpret_cast<float4*>(y + o) =
-        make_float4(a.x + acc[q], a.y + acc[q + 1], a.z + acc[q + 2], a.w + acc[q + 3]);
-    *reinterpret_cast<float4*>(y + o + 4) =
-        make_float4(b.x + acc[q + 4], b.y + acc[q + 5], b.z + acc[q + 6], b.w + acc[q + 7]);
+    gstore4(y + o,
+            make_float4(a.x + acc[q], a.y + acc[q + 1], a.z + acc[q + 2], a.w + acc[q + 3]));
+    gstore4(y + o + 4,
+            make_float4(b.x + acc[q + 4], b.y + acc[q + 5], b.z + acc[q + 6], b.w + acc[q + 7]));
   }
 }
 
@@ -785,6 +806,176 @@ extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_ell_h32(
     const int2* wseg, const int2* ent, int nf, int nchunk, const int* slots,
     const btpm_set_dev* sets, const btpm_status_dev* status) {
   selfheat_ell_body<32>(wseg, ent, nf, nchunk, slots, sets, status);
+}
+
+// The same product with a wave's 64 receivers taken 8 at a time.  In pass p
+// (0..7) of wave w the lanes 8g .. 8g+7 work for the receiver in slot
+// 64w + 8p + g of the block, lane s = l % 8 on steps 4s + 32j .. 4s + 32j + 3
+// (j < H/32) of the workgroup's H steps.  Each LDS read of a group is one
+// 128-byte run of a staged row (the one-receiver-per-lane kernels read 64
+// random rows at once and collide on banks), and a pass walks the longest of
+// 8 receivers' entries in the chunk instead of the longest of 64.  A block's receivers sit in slots by decreasing entry count
+// (gperm[rb * 256 + slot] = receiver, nf for an empty slot), so long rows
+// share passes.  For receiver block rb, chunk c, wave w and pass p,
+// gseg[((rb * nchunk + c) * 4 + w) * 8 + p] = (first, n): n is the longest
+// group's entry count in the chunk rounded up to even, and entries k, k + 1
+// of group g are the int4 gent[first + 8 (k / 2) + g] = (emitter within the
+// chunk, view-factor bits) twice, shorter groups padded with (0, +0.0f).
+// Each receiver still adds its entries in emitter order (a padded entry adds
+// +0 x E >= 0, which leaves the sum unchanged), so the sums equal
+// btpm_selfheat_tiled's bit for bit.
+template <int H>
+__device__ __forceinline__ void selfheat_grp_body(const int2* __restrict__ gseg,
+                                                  const int4* __restrict__ gent,
+                                                  const int* __restrict__ gperm, int nf,
+                                                  int nchunk, const int* __restrict__ slots,
+                                                  const btpm_set_dev* __restrict__ sets,
+                                                  const btpm_status_dev* __restrict__ status) {
+  constexpr int kJ = H / 32;                      // 128-byte runs of a staged row
+  constexpr int kQ4 = H / 4;                      // float4 per staged row
+  constexpr int kPer = kSHChunk * kQ4 / kSHRows;  // float4 staged per lane
+  constexpr int kPass = 8;                        // receivers per lane: 64 per wave / 8 groups
+  static_assert(H % 32 == 0 && kSHChunk * kQ4 % kSHRows == 0 && kSHRows % kQ4 == 0,
+                "staging shape");
+  __shared__ __align__(16) float E[kSHChunk * H];
+  const int slot = slots[blockIdx.z];
+  if (status[slot].done) return;
+  const btpm_set_dev& S = sets[slot];
+  const int npad = S.nb * BTPM_BLK;
+  const int s0 = blockIdx.y * H;
+  if (s0 >= npad) return;
+  const int tid = (int)threadIdx.x;
+  const int rb = (int)blockIdx.x;
+  const uint8_t* __restrict__ frozen = S.frozen;
+  {
+    const int f = gperm[rb * kSHRows + tid];
+    const bool live = f < nf && (S.inst || !frozen[f]);
+    if (__syncthreads_count(live) == 0) return;  // every receiver here is frozen
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, g = lane >> 3, sl = lane & 7;
+  int off[kJ];  // this lane's steps in the staged rows
+#pragma unroll
+  for (int j = 0; j < kJ; ++j) off[j] = 32 * j + 4 * sl;
+  int fr[kPass];
+  unsigned mine = 0, pass_live = 0;  // this lane's live passes; the wave's
+#pragma unroll
+  for (int p = 0; p < kPass; ++p) {
+    fr[p] = gperm[rb * kSHRows + 64 * wave + 8 * p + g];
+    const bool live = fr[p] < nf && (S.inst || !frozen[fr[p]]);
+    mine |= (unsigned)live << p;
+    if (__ballot(live) != 0) pass_live |= 1u << p;
+  }
+  const float* __restrict__ emis = S.emis;
+  // staging: lane tid holds rows tid / kQ4 + i * (kSHRows / kQ4), float4 column tid % kQ4
+  const int srow = tid / kQ4, sq4 = 4 * (tid % kQ4);
+  const bool scol = s0 + sq4 < npad;
+  float4 pre[kPer];
+  auto fetch = [&](int c) {
+    const int g0 = c * kSHChunk;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = g0 + srow + i * (kSHRows / kQ4);
+      pre[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (e < nf && scol) pre[i] = gload4(emis + (size_t)e * npad + s0 + sq4);
+    }
+  };
+  float acc[kPass][kJ][4];
+#pragma unroll
+  for (int p = 0; p < kPass; ++p)
+#pragma unroll
+    for (int j = 0; j < kJ; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[p][j][q] = 0.0f;
+  // the wave's 8 segment descriptors of a chunk, lane l holding pass l % 8,
+  // loaded one chunk ahead (read with readlane where they are used)
+  const int2* __restrict__ sgb =
+      gseg + (size_t)rb * nchunk * (BTPM_SH_WAVES * kPass) + wave * kPass + (lane & 7);
+  int2 dn = sgb[0];
+  fetch(0);
+  for (int c = 0; c < nchunk; ++c) {
+    __syncthreads();  // the previous chunk has been consumed
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+      *reinterpret_cast<float4*>(&E[(srow + i * (kSHRows / kQ4)) * H + sq4]) = pre[i];
+    if (c + 1 < nchunk) fetch(c + 1);
+    const int2 dc = dn;
+    dn = sgb[(size_t)min(c + 1, nchunk - 1) * (BTPM_SH_WAVES * kPass)];
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < kPass; ++p) {
+      const int n = (pass_live >> p) & 1 ? __builtin_amdgcn_readlane(dc.y, p) : 0;
+      if (n == 0) continue;
+      const int4* __restrict__ e = gent + (size_t)__builtin_amdgcn_readlane(dc.x, p) + g;
+      // pairs are prefetched three ahead by unconditional loads (the index
+      // clamped to the segment's last pair) into the registers of the pair
+      // just consumed (the loop is unrolled by three, so no copy waits for a
+      // load); the empty asm with a memory clobber keeps each load where it
+      // is written (otherwise the compiler sinks it to its use and waits for
+      // it there) without waiting for it
+      const int last = n / 2 - 1;
+      auto step = [&](const int4& q) {
+        // the empty asm makes the pair's emitter indices opaque until here,
+        // so their address arithmetic (and the wait for the pair's load)
+        // stays at this step instead of being hoisted to the loop's top
+        int xa = q.x, xb = q.z;
+        asm volatile("" : "+v"(xa), "+v"(xb));
+        float4 a[kJ], b[kJ];
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+          a[j] = *reinterpret_cast<const float4*>(&E[xa * H + off[j]]);
+          b[j] = *reinterpret_cast<const float4*>(&E[xb * H + off[j]]);
+        }
+        const float va = __int_as_float(q.y), vb = __int_as_float(q.w);
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+          acc[p][j][0] = fmaf(vb, b[j].x, fmaf(va, a[j].x, acc[p][j][0]));
+          acc[p][j][1] = fmaf(vb, b[j].y, fmaf(va, a[j].y, acc[p][j][1]));
+          acc[p][j][2] = fmaf(vb, b[j].z, fmaf(va, a[j].z, acc[p][j][2]));
+          acc[p][j][3] = fmaf(vb, b[j].w, fmaf(va, a[j].w, acc[p][j][3]));
+        }
+      };
+      int4 qa = e[0], qb = e[min(1, last) * 8], qc = e[min(2, last) * 8];
+      asm volatile("" ::: "memory");
+      int kp = 0;
+      for (; kp + 3 <= last + 1; kp += 3) {
+        step(qa);
+        qa = e[min(kp + 3, last) * 8];
+        asm volatile("" ::: "memory");
+        step(qb);
+        qb = e[min(kp + 4, last) * 8];
+        asm volatile("" ::: "memory");
+        step(qc);
+        qc = e[min(kp + 5, last) * 8];
+        asm volatile("" ::: "memory");
+      }
+      if (kp <= last) step(qa);
+      if (kp + 1 <= last) step(qb);
+    }
+  }
+  const float* __restrict__ src = S.src;
+  float* __restrict__ y = S.qext;
+#pragma unroll
+  for (int p = 0; p < kPass; ++p) {
+    if (!((mine >> p) & 1)) continue;
+    const int f = fr[p];
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+      const int st = s0 + off[j];  // a multiple of 4: st < npad covers st + 3
+      if (st >= npad) continue;
+      const size_t o = ((size_t)(st / BTPM_BLK) * nf + f) * BTPM_BLK + (st % BTPM_BLK);
+      const float4 a = gload4(src + o);
+      gstore4(y + o,
+            make_float4(a.x + acc[p][j][0], a.y + acc[p][j][1],
+                                                      a.z + acc[p][j][2], a.w + acc[p][j][3]));
+    }
+  }
+}
+
+extern "C" __global__ __launch_bounds__(kSHRows) void btpm_selfheat_grp(
+    const int2* gseg, const int4* gent, const int* gperm, int nf, int nchunk, const int* slots,
+    const btpm_set_dev* sets, const btpm_status_dev* status) {
+  selfheat_grp_body<32>(gseg, gent, gperm, nf, nchunk, slots, sets, status);
 }
 
 // The same product with the lanes along time: lane l of a wave holds step
@@ -838,7 +1029,7 @@ extern "C" __global__ __launch_bounds__(kRowsWaves * 64) void btpm_selfheat_rows
       const int row = idx / (kRowsH / 4), q4 = 4 * (idx % (kRowsH / 4));
       pre[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (g0 + row < nf && s0 + q4 < npad)
-        pre[i] = *reinterpret_cast<const float4*>(emis + (size_t)(g0 + row) * npad + s0 + q4);
+        pre[i] = gload4(emis + (size_t)(g0 + row) * npad + s0 + q4);
     }
   };
   float acc[kRowsR];
@@ -885,7 +1076,7 @@ extern "C" __global__ __launch_bounds__(kRowsWaves * 64) void btpm_selfheat_rows
   for (int r = 0; r < kRowsR; ++r) {
     if (!((live >> r) & 1ull)) continue;
     const size_t o = ((size_t)(s / BTPM_BLK) * nf + (f0 + r)) * BTPM_BLK + (s % BTPM_BLK);
-    y[o] = src[o] + acc[r];
+    gstore1(y + o, gload1(src + o) + acc[r]);
   }
 }
 

@@ -70,6 +70,10 @@ extern "C" __global__ void btpm_selfheat_ell_h32(const int2* wseg, const int2* e
                                                  int nchunk, const int* slots,
                                                  const btpm_set_dev* sets,
                                                  const btpm_status_dev* status);
+extern "C" __global__ void btpm_selfheat_grp(const int2* gseg, const int4* gent, const int* gperm,
+                                             int nf, int nchunk, const int* slots,
+                                             const btpm_set_dev* sets,
+                                             const btpm_status_dev* status);
 extern "C" __global__ void btpm_selfheat_rows(const int* cptr, const int2* ent, int nf,
                                               int nchunk, const int* slots,
                                               const btpm_set_dev* sets,

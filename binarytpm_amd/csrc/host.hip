@@ -243,9 +243,16 @@ struct btpm_shape {
   // stays within BTPM_SH_ELL_MAX_PAD times the view-factor count
   DBuf wseg, ell;
   bool use_ell = false;
-  // BTPM_SH_LAYOUT forcing one kernel (no tuning): -1 time-lane, 1 the
-  // lane-per-receiver kernel of the layout (ELL or per-chunk CSR), 0 tuned
-  int force_heat = 0;
+  // the same entries per (receiver block, chunk, wave, pass of 8 receivers)
+  // for btpm_selfheat_grp: gseg = (first pair, entries rounded up to even),
+  // gent = int4 pairs of (emitter within the chunk, VF bits), gperm = the
+  // block's receivers by decreasing entry count (nf for an empty slot)
+  DBuf gseg, gent, gperm;
+  bool use_grp = false;
+  size_t grp_slots = 0;  // padded entries of the grouped layout
+  // BTPM_SH_LAYOUT forcing one self-heating kernel (no tuning): the kernel
+  // code (see heat_kernel_code) or -1 for tuned
+  int force_heat = -1;
   DBuf cvf32;      // [100][100] crater VF (fp32, vf[i*100+j] = VF(j->i))
   DBuf cvfA;       // [2][112][112] fp16 hi/lo of VF^T, zero padded
   DBuf cillum;     // [nb][nf*100][BLK] crater illumination (after scattering)
@@ -564,12 +571,33 @@ struct Sched {
     btpm_shape* shape;
     int nsets;
     std::unique_ptr<DBuf> slots;
-    // kernel: 0 = btpm_selfheat_rows (time along the lanes); 64 / 32 = steps
-    // per workgroup of the lane-per-receiver kernel (ELL, or the per-chunk
-    // CSR btpm_selfheat_tiled when the shape has no ELL layout)
-    int steps = 0;
+    // kernel code: kHeatRows = btpm_selfheat_rows (time along the lanes);
+    // 64 / 32 = steps per workgroup of the lane-per-receiver kernel (ELL, or
+    // the per-chunk CSR btpm_selfheat_tiled when the shape has no ELL
+    // layout); kHeatGrp + 32 = the receiver-grouped kernel
+    int code = 0;
     bool trial = false;
   };
+  static constexpr int kHeatRows = 0, kHeatGrp = 1000;
+  static int heat_steps(int code) {
+    return code == kHeatRows ? BTPM_SH_STEPS_T : code % kHeatGrp;
+  }
+  // self-heating kernels a shape's layouts allow, for the tuner
+  static std::vector<int> heat_candidates(const btpm_shape* sh) {
+    std::vector<int> c{kHeatRows, 64};
+    if (sh->use_ell) c.push_back(32);
+    if (sh->use_grp) c.push_back(kHeatGrp + 32);
+    return c;
+  }
+  // untuned: the grouped kernel while its padding stays within
+  // BTPM_SH_ELL_MAX_PAD times the entry count, else ELL under the same
+  // bound, else the time-lane kernel (skewed rows: a few receivers see most
+  // of the shape)
+  static int heat_default(const btpm_shape* sh) {
+    if (sh->use_grp && sh->grp_slots <= (size_t)BTPM_SH_ELL_MAX_PAD * sh->nnz)
+      return kHeatGrp + 32;
+    return sh->use_ell ? BTPM_SH_STEPS : kHeatRows;
+  }
   std::vector<HeatGroup> heat;
 
   // ---- autotuner ----------------------------------------------------------
@@ -886,10 +914,9 @@ struct Sched {
           want_trial(smooth_key(kv.second.second), {256, 128, 64});
         }
       for (auto& kv : by_shape)
-        if (!kv.first->force_heat) {
+        if (kv.first->force_heat < 0) {
           present.insert(heat_key(kv.first));
-          want_trial(heat_key(kv.first),
-                     kv.first->use_ell ? std::vector<int>{0, 64, 32} : std::vector<int>{0, 64});
+          want_trial(heat_key(kv.first), heat_candidates(kv.first));
         }
       while (!trials.empty() && !present.count(trials.front().key)) trials.pop_front();
       if (!trials.empty()) trials.front().units = 0.0;
@@ -936,13 +963,15 @@ struct Sched {
     for (auto& kv : by_shape) {
       HeatGroup g{kv.first, (int)kv.second.size(), std::make_unique<DBuf>()};
       upload(*g.slots, kv.second.data(), kv.second.size(), ctx->stream);
-      if (kv.first->force_heat) {
-        g.steps = kv.first->force_heat < 0 ? 0 : BTPM_SH_STEPS;
+      if (kv.first->force_heat >= 0) {
+        g.code = kv.first->force_heat;
       } else {
         const TuneKey k = heat_key(kv.first);
-        // untuned: ELL when its padding is bounded, else the time-lane kernel
-        // (skewed rows: a few receivers see most of the shape)
-        g.steps = choice(k, kv.first->use_ell ? BTPM_SH_STEPS : 0);
+        g.code = choice(k, heat_default(kv.first));
+        // a choice tuned on another shape of the class that this shape's
+        // layouts cannot run
+        const std::vector<int> ok = heat_candidates(kv.first);
+        if (std::find(ok.begin(), ok.end(), g.code) == ok.end()) g.code = heat_default(kv.first);
         g.trial = on_trial(k);
         if (g.trial)
           for (int sl : kv.second) trials.front().units += ctx->h_status[sl].norder;
@@ -986,7 +1015,7 @@ struct Sched {
       HeatGroup g{kv.first, (int)kv.second.size(), std::make_unique<DBuf>()};
       upload(*g.slots, kv.second.data(), kv.second.size(), ctx->stream);
       for (const HeatGroup& m : heat)
-        if (m.shape == kv.first) g.steps = m.steps;
+        if (m.shape == kv.first) g.code = m.code;
       fheat.push_back(std::move(g));
     }
     fn_tiles_c = (int)tc.size();
@@ -1012,17 +1041,21 @@ struct Sched {
       for (const HeatGroup& g : fast ? fheat : heat) {
         btpm_shape* sh = g.shape;
         const dim3 grid(grid1(sh->nf, BTPM_SH_ROWS),
-                        grid1((size_t)sh->nb * BTPM_BLK, g.steps > 0 ? g.steps : BTPM_SH_STEPS),
-                        g.nsets);  // the lane-per-receiver kernels
+                        grid1((size_t)sh->nb * BTPM_BLK, heat_steps(g.code)),
+                        g.nsets);  // the lane-per-receiver and grouped kernels
         if (g.trial) trial_event(st, 0);
-        if (g.steps == 0)
+        if (g.code == kHeatRows)
           hipLaunchKernelGGL(btpm_selfheat_rows,
                              dim3(grid1(sh->nf, BTPM_SH_ROWS_T),
                                   grid1((size_t)sh->nb * BTPM_BLK, BTPM_SH_STEPS_T), g.nsets),
                              dim3(BTPM_SH_ROWS_T / 32 * 64), 0, st, sh->cptr.as<int>(),
                              sh->sent.as<int2>(), sh->nf, sh->nchunk, g.slots->as<int>(),
                              ctx->d_sets, ctx->d_status);
-        else if (sh->use_ell && g.steps == 32)
+        else if (g.code == kHeatGrp + 32)
+          hipLaunchKernelGGL(btpm_selfheat_grp, grid, dim3(BTPM_SH_ROWS), 0, st,
+                             sh->gseg.as<int2>(), sh->gent.as<int4>(), sh->gperm.as<int>(),
+                             sh->nf, sh->nchunk, g.slots->as<int>(), ctx->d_sets, ctx->d_status);
+        else if (sh->use_ell && g.code == 32)
           hipLaunchKernelGGL(btpm_selfheat_ell_h32, grid, dim3(BTPM_SH_ROWS), 0, st,
                              sh->wseg.as<int2>(), sh->ell.as<int2>(), sh->nf, sh->nchunk,
                              g.slots->as<int>(), ctx->d_sets, ctx->d_status);
@@ -1542,18 +1575,86 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
             ws[2 * i + 1] = n;
             total += (size_t)n * 64;
           }
-      // BTPM_SH_LAYOUT=csr|ell|rows forces one kernel (diagnostics and tests)
+      // receiver-grouped copy: per block, the receivers in slots by
+      // decreasing entry count; per (block, chunk, wave, pass of 8 slots) the
+      // longest entry count rounded up to even, entries in int4 pairs
+      constexpr int kPass = 8;
+      std::vector<int> gp((size_t)nrb * BTPM_SH_ROWS, nf);
+      for (int rb = 0; rb < nrb; ++rb) {
+        const int f0 = rb * BTPM_SH_ROWS, f1 = std::min(nf, f0 + BTPM_SH_ROWS);
+        int* b = gp.data() + f0;
+        for (int f = f0; f < f1; ++f) b[f - f0] = f;
+        std::stable_sort(b, b + (f1 - f0), [&](int x, int y) {
+          return body->rowptr[x + 1] - body->rowptr[x] > body->rowptr[y + 1] - body->rowptr[y];
+        });
+      }
+      std::vector<int> gs(2 * (size_t)nrb * nc * BTPM_SH_WAVES * kPass);
+      size_t gpairs = 0;
+      for (int rb = 0; rb < nrb; ++rb)
+        for (int c = 0; c < nc; ++c)
+          for (int w = 0; w < BTPM_SH_WAVES; ++w)
+            for (int p = 0; p < kPass; ++p) {
+              int n = 0;
+              for (int g = 0; g < 8; ++g) {
+                const int f = gp[(size_t)rb * BTPM_SH_ROWS + 64 * w + 8 * p + g];
+                if (f < nf)
+                  n = std::max(n, cp[(size_t)(c + 1) * nf + f] - cp[(size_t)c * nf + f]);
+              }
+              n = (n + 1) & ~1;
+              const size_t i = (((size_t)rb * nc + c) * BTPM_SH_WAVES + w) * kPass + p;
+              gs[2 * i] = (int)std::min(gpairs, (size_t)INT32_MAX);
+              gs[2 * i + 1] = n;
+              gpairs += (size_t)(n / 2) * 8;
+            }
+      // BTPM_SH_LAYOUT forces one kernel (diagnostics and tests): csr (the
+      // per-chunk CSR, lane per receiver), ell / ell32 (ELL, 64 / 32 steps per
+      // workgroup), grp (grouped, 32 steps), rows (time-lane)
       const char* lay = std::getenv("BTPM_SH_LAYOUT");
       const bool fits = total < (size_t)INT32_MAX;
+      const bool gfits = gpairs < (size_t)INT32_MAX;
+      sh->grp_slots = 2 * gpairs;
+      // the grouped layout is kept while it stays within 16 entries per view
+      // factor (the tuner times it; the untuned choice needs the ELL bound)
+      sh->use_grp = gfits && sh->grp_slots <= 16 * (size_t)body->nnz;
       if (lay && std::strcmp(lay, "csr") == 0) {
         sh->use_ell = false;
-        sh->force_heat = 1;
-      } else if (lay && std::strcmp(lay, "ell") == 0) {
+        sh->force_heat = 64;
+      } else if (lay && (std::strcmp(lay, "ell") == 0 || std::strcmp(lay, "ell32") == 0)) {
         sh->use_ell = fits;
-        sh->force_heat = 1;
+        sh->force_heat = lay[3] ? 32 : 64;
+        if (!fits) sh->force_heat = 64;  // the per-chunk CSR
       } else {
         sh->use_ell = fits && total <= (size_t)BTPM_SH_ELL_MAX_PAD * body->nnz;
-        if (lay && std::strcmp(lay, "rows") == 0) sh->force_heat = -1;
+        if (lay && std::strcmp(lay, "rows") == 0) sh->force_heat = 0;
+        if (lay && std::strcmp(lay, "grp") == 0) {
+          if (!gfits) return fail(BTPM_ERR_ARG, "BTPM_SH_LAYOUT=grp: grouped layout too large");
+          sh->use_grp = true;
+          sh->force_heat = 1032;  // kHeatGrp + 32
+        }
+      }
+      if (sh->use_grp) {
+        std::vector<int> ge(4 * gpairs, 0);
+        for (int rb = 0; rb < nrb; ++rb)
+          for (int c = 0; c < nc; ++c)
+            for (int w = 0; w < BTPM_SH_WAVES; ++w)
+              for (int p = 0; p < kPass; ++p) {
+                const size_t i = (((size_t)rb * nc + c) * BTPM_SH_WAVES + w) * kPass + p;
+                const size_t first = (size_t)gs[2 * i];
+                for (int g = 0; g < 8; ++g) {
+                  const int f = gp[(size_t)rb * BTPM_SH_ROWS + 64 * w + 8 * p + g];
+                  if (f >= nf) continue;
+                  const int k0 = cp[(size_t)c * nf + f], k1 = cp[(size_t)(c + 1) * nf + f];
+                  for (int k = k0; k < k1; ++k) {
+                    const size_t o = first + (size_t)((k - k0) / 2) * 8 + g;
+                    const int h = 2 * ((k - k0) & 1);
+                    ge[4 * o + h] = se[2 * (size_t)k] - c * kChunk;  // emitter within the chunk
+                    ge[4 * o + h + 1] = se[2 * (size_t)k + 1];
+                  }
+                }
+              }
+        upload(sh->gperm, gp.data(), gp.size(), st);
+        upload(sh->gseg, gs.data(), gs.size(), st);
+        upload(sh->gent, ge.data(), ge.size(), st);
       }
       // the per-chunk CSR serves the time-lane kernel and btpm_selfheat_tiled
       upload(sh->sent, se.data(), se.size(), st);

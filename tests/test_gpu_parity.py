@@ -81,12 +81,13 @@ def test_smooth_inst_parity(ctx):
     assert _report("smooth TI=0", s.surface_temps(), ref.stemp) <= TOL_K
 
 
-@pytest.mark.parametrize("layout", ["ell", "csr", "rows"])
+@pytest.mark.parametrize("layout", ["ell", "ell32", "csr", "rows", "grp"])
 def test_self_heating_super_cycle_parity(ctx, monkeypatch, layout):
     """Binary-like body: 2:1 super-cycle (numrevs 2 and 1 on ntau = 1300),
     sparse facet view factors, facet multiple scattering; every self-heating
     kernel (lane per receiver over the lane-interleaved ELL or the per-chunk
-    CSR, and lane per step over the per-chunk CSR)."""
+    CSR, lane per step over the per-chunk CSR, and 8 receivers per wave over
+    the receiver-grouped layout)."""
     monkeypatch.setenv("BTPM_SH_LAYOUT", layout)
     nt = 1300
     Vp, Fp = inputs.ellipsoid(1, (0.6, 0.5, 0.45))
@@ -139,7 +140,7 @@ def test_self_heating_layouts_bitwise(ctx, monkeypatch):
     csr = _skewed_viewfactors(nf)
     p = th.params(case.ntau, case.rot_period, tacc=0.01)
     out = {}
-    for layout in ("csr", "ell", "rows", None):
+    for layout in ("csr", "ell", "ell32", "rows", "grp", None):
         if layout is None:
             monkeypatch.delenv("BTPM_SH_LAYOUT", raising=False)
         else:
@@ -149,11 +150,45 @@ def test_self_heating_layouts_bitwise(ctx, monkeypatch):
         out[layout] = s.surface_temps()
         s.close()
         shape.close()
-    for layout in ("ell", "rows", None):
+    for layout in ("ell", "ell32", "rows", "grp", None):
         assert np.array_equal(out["csr"], out[layout]), layout
     F = O.facet_scatter(case.illum, 0.1, csr, multiple=False)
     ref = O.smooth_solve(O.params(case.ntau, case.rot_period, tacc=0.01), F, 0.1, 200.0, csr=csr)
     assert _report("skewed self-heating", out["ell"], ref.stemp) <= TOL_K
+
+
+def test_self_heating_grouped_ragged(ctx, monkeypatch):
+    """333 facets (a last receiver block of 77: partial groups of 8 and empty
+    slots) with dense rows of uneven length: the grouped kernel equals the
+    per-chunk CSR kernel bit for bit and matches the oracle."""
+    case, _ = inputs.serving_shape(3, 5)
+    nf = 333
+    D = np.ascontiguousarray(case.illum[:nf])
+    rng = np.random.default_rng(5)
+    rowptr, cols, vals = [0], [], []
+    for f in range(nf):
+        k = int(rng.integers(0, 60)) if f % 7 else 0
+        c = np.sort(rng.choice(nf - 1, size=k, replace=False))
+        c = np.where(c >= f, c + 1, c)
+        cols.append(c)
+        vals.append(rng.uniform(0.001, 0.01, size=k))
+        rowptr.append(rowptr[-1] + k)
+    csr = (np.asarray(rowptr, np.int32), np.concatenate(cols).astype(np.int32),
+           np.concatenate(vals).astype(np.float32))
+    p = th.params(case.ntau, case.rot_period, tacc=0.01)
+    out = {}
+    for layout in ("csr", "grp"):
+        monkeypatch.setenv("BTPM_SH_LAYOUT", layout)
+        shape = th.Shape(ctx, p, D, 0.1, csr=csr, scatter=1)
+        s = th.Solve(ctx, shape, 100.0).run()
+        out[layout] = s.surface_temps()
+        s.close()
+        shape.close()
+    monkeypatch.delenv("BTPM_SH_LAYOUT", raising=False)
+    assert np.array_equal(out["csr"], out["grp"])
+    F = O.facet_scatter(D, 0.1, csr, multiple=False)
+    ref = O.smooth_solve(O.params(case.ntau, case.rot_period, tacc=0.01), F, 0.1, 100.0, csr=csr)
+    assert _report("grouped self-heating, ragged", out["grp"], ref.stemp) <= TOL_K
 
 
 def test_config1_binary_full_size(ctx):
@@ -334,8 +369,9 @@ def test_crater_sun_table_and_flux_scale(ctx, rough80):
 def test_autotune_identical_results(rough80):
     """The autotuner's trial rotations run candidate kernels on the live
     solves (smooth conduction 256/128/64 columns per workgroup; self-heating
-    time-lane, ELL 64 and ELL 32 steps per workgroup, each candidate twice in
-    palindromic order); every candidate does the same arithmetic in the same
+    time-lane, ELL 64 and ELL 32 steps per workgroup and receiver-grouped, each
+    candidate twice in palindromic order: 6 + 8 trial rotations); every
+    candidate does the same arithmetic in the same
     order, so a tuned batch equals an untuned one bit for bit.  Decisions are
     kept in the context: a second batch on it runs no trials."""
     case, crater = rough80
@@ -353,7 +389,7 @@ def test_autotune_identical_results(rough80):
             print(f"[tune] autotune={tune}: {st.tune_decisions} decisions, "
                   f"{st.tune_rotations} trial rotations", flush=True)
             if tune:
-                assert st.tune_decisions == 2 and st.tune_rotations == 12
+                assert st.tune_decisions == 2 and st.tune_rotations == 14
                 again = [th.Solve(c2, shape, 120.0, with_craters=True)]
                 st2 = th.batch_run(c2, again, th.batch_opts(autotune=True))
                 assert st2.tune_decisions == 0 and st2.tune_rotations == 0
