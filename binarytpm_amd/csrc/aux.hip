@@ -209,8 +209,11 @@ extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev 
 // shape -> crater coupling (load_GPU_roughness_globalillumination /
 // _globalheating, T175706/thermal.cpp:2693-2803) as a per-facet GEMM
 //   out[j][s] = sum_k drf[k][j] w[g_k][s]        (j < 100, s in a 64-step tile)
-// w = weights in blocked layout over the smooth columns:
-//   RGI: w = A_g F_g(s);  RGH: w = es T_g(s)^4 (mode 1 computes it from T).
+// w = weights in facet-major rows w[g][s] (s < nb*BLK, btpm_weights_fm):
+//   RGI: w = A_g F_g(s);  RGH: w = es T_g(s)^4.
+// A wave's gather of one entry's weights is then one 256-byte run (64
+// consecutive steps of one row) instead of eight 32-byte pieces of the
+// blocked series a quarter of a cache line each.
 // The result is written blocked over crater columns c = f*NS+j as
 //   dst[b][c][k] = base_gain * base_scale_f * base[b][c][k] + out   (base may be null).
 // Workgroup: 256 lanes = 4 waves per (facet, 64-step tile), on a 1-D grid
@@ -260,8 +263,8 @@ __device__ __forceinline__ void split8(const float (&v)[8], float sc, rhalf8& hi
 extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
     const int* __restrict__ rowptr, const int* __restrict__ colidx,
     const _Float16* __restrict__ drfA, const int* __restrict__ cbase,
-    const float* __restrict__ drf_scale, const float* __restrict__ w, int mode, float es,
-    int nsmooth, int nfac, int ntau, int nb, const float* __restrict__ base,
+    const float* __restrict__ drf_scale, const float* __restrict__ w, int nfac, int ntau,
+    int nb, const float* __restrict__ base,
     const float* __restrict__ base_scale, float base_gain, float* __restrict__ dst) {
   constexpr int kMT = kRM / 16;
   constexpr int kAPieces = kRDrfTile / 8;  // 16-byte pieces of a packed drf^T tile: 896
@@ -286,7 +289,7 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
   // staging roles: drf^T tile pieces p = tid + 256 q (a straight copy of the
   // packed tile); w^T row tid % 64 (a step), group tid / 64 = wave
   const int sb = min(s0 + (tid & (kRN - 1)), ntau - 1);
-  const size_t wofs = (size_t)(sb / BTPM_BLK) * nsmooth * BTPM_BLK + (sb % BTPM_BLK);
+  const int npad = nb * BTPM_BLK;
   rhalf8 ra[kAPer];
   float rb[8];
   auto fetch = [&](int kb) {
@@ -301,7 +304,7 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
     for (int i = 0; i < 8; ++i) {
       const int k = 8 * wave + i;
       rb[i] = 0.0f;
-      if (k < kn) rb[i] = w[wofs + (size_t)colidx[kb + k] * BTPM_BLK];
+      if (k < kn) rb[i] = w[(size_t)colidx[kb + k] * npad + sb];
     }
   };
   rfloat4 acc[kMT];
@@ -321,7 +324,7 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
     {
       float v[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = mode == 1 ? es * (rb[i] * rb[i]) * (rb[i] * rb[i]) : rb[i];
+      for (int i = 0; i < 8; ++i) v[i] = rb[i];
       rhalf8 hi, lo;
       split8(v, 1.0f, hi, lo);
       const int n = tid & (kRN - 1);
@@ -358,6 +361,30 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
       dst[o] = v;
     }
   }
+}
+
+// Weights of btpm_shape_to_crater as facet-major rows out[c][s] (s < nb*BLK)
+// from a blocked [nb][ncols][BLK] series x: mode 0 gives (gain * scale[c]) x
+// (RGI: A_c F_c), mode 1 gives es x^4 (RGH: the emission of the smooth
+// surface temperatures), each value computed once here instead of once per
+// view-factor entry that gathers it.  One thread per (block, column).
+extern "C" __global__ __launch_bounds__(256) void btpm_weights_fm(const float* __restrict__ x,
+                                                                 const float* __restrict__ scale,
+                                                                 float gain, int mode, float es,
+                                                                 int ncols, int nb,
+                                                                 float* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;  // b * ncols + c
+  if (i >= (size_t)nb * ncols) return;
+  const int b = (int)(i / ncols), c = (int)(i % ncols);
+  const float4 a0 = *reinterpret_cast<const float4*>(x + i * BTPM_BLK);
+  const float4 a1 = *reinterpret_cast<const float4*>(x + i * BTPM_BLK + 4);
+  float v[BTPM_BLK] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  const float sc = mode == 1 ? 0.0f : gain * scale[c];
+#pragma unroll
+  for (int k = 0; k < BTPM_BLK; ++k) v[k] = mode == 1 ? es * (v[k] * v[k]) * (v[k] * v[k]) : sc * v[k];
+  float* o = out + (size_t)c * nb * BTPM_BLK + (size_t)b * BTPM_BLK;
+  *reinterpret_cast<float4*>(o) = make_float4(v[0], v[1], v[2], v[3]);
+  *reinterpret_cast<float4*>(o + 4) = make_float4(v[4], v[5], v[6], v[7]);
 }
 
 // Packs the shape->crater view factors once per shape for btpm_shape_to_crater:

@@ -43,10 +43,12 @@ extern "C" __global__ void btpm_pack_drf(const int* rowptr, const float* drf, co
                                          float* drf_scale, _Float16* drfA);
 extern "C" __global__ void btpm_shape_to_crater(const int* rowptr, const int* colidx,
                                                 const _Float16* drfA, const int* cbase,
-                                                const float* drf_scale, const float* w, int mode,
-                                                float es, int nsmooth, int nfac, int ntau, int nb,
-                                                const float* base, const float* base_scale,
-                                                float base_gain, float* dst);
+                                                const float* drf_scale, const float* w, int nfac,
+                                                int ntau, int nb, const float* base,
+                                                const float* base_scale, float base_gain,
+                                                float* dst);
+extern "C" __global__ void btpm_weights_fm(const float* x, const float* scale, float gain,
+                                           int mode, float es, int ncols, int nb, float* out);
 extern "C" __global__ void btpm_crater_scatter(const float* vf32, const float* albedo,
                                                const float* shadow, const float* cosang,
                                                const float* sun, const float* vis_table,

@@ -261,6 +261,9 @@ struct btpm_shape {
   DBuf drfA;       // [chunks][2][112][32] halves
   DBuf cbase;      // [nf] first chunk of each facet
   DBuf drf_scale;  // [nf] power of two putting the facet's largest |drf| in [0.5, 1)
+  // [nf][nb*BLK] facet-major weights of btpm_shape_to_crater (RGI at shape
+  // creation, then each solve's RGH; every use is on the context's stream)
+  DBuf wfm;
   float vf_unscale = 1.0f;
   std::vector<float> h_albedo, h_numrevs;
 };
@@ -472,11 +475,13 @@ void build_crater(btpm_solve* s, hipStream_t st) {
   d.tile_cols = BTPM_CRATERS_WG;
   // src = (1-A) illum + RGH (RGH from the converged smooth T)
   if (sh->nnz > 0 && sh->drfA.p) {
+    hipLaunchKernelGGL(btpm_weights_fm, dim3(grid1((size_t)d.nb * sh->nf, 256)), dim3(256), 0,
+                       st, s->smooth->T.as<float>(), (const float*)nullptr, 1.0f, 1,
+                       sh->p.eps * sh->p.sigma, sh->nf, d.nb, sh->wfm.as<float>());
     dim3 g(8 * grid1(sh->nf, 8) * grid1((size_t)d.nb * BTPM_BLK, 64));  // XCD-grouped facets
     hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                        sh->colidx.as<int>(), sh->drfA.as<_Float16>(), sh->cbase.as<int>(),
-                       sh->drf_scale.as<float>(), s->smooth->T.as<float>(), 1,
-                       sh->p.eps * sh->p.sigma, sh->nf, sh->nf, sh->p.ntau, d.nb,
+                       sh->drf_scale.as<float>(), sh->wfm.as<float>(), sh->nf, sh->p.ntau, d.nb,
                        sh->cillum.as<float>(), sh->one_minus_a.as<float>(), s->flux_scale,
                        S->src.as<float>());
   } else {
@@ -1756,6 +1761,7 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
       hipLaunchKernelGGL(btpm_pack_drf, dim3(nf), dim3(256), 0, st, sh->rowptr.as<int>(),
                          raw.as<float>(), sh->cbase.as<int>(), sh->drf_scale.as<float>(),
                          sh->drfA.as<_Float16>());
+      sh->wfm.alloc((size_t)nf * nb * BTPM_BLK * sizeof(float));
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(st));
     }
@@ -1780,17 +1786,15 @@ int btpm_shape_create(btpm_ctx* ctx, const btpm_params* p, const btpm_body* body
       if (sh->drfA.p) {
         // RGI = sum_g drf * A_g F_g(s): blocked over crater columns, then
         // unpacked to the reference layout the scattering kernel reads
-        DBuf AF;
-        AF.alloc(nser * 4);
-        hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st,
-                           AF.as<float>(), sh->F.as<float>(), sh->albedo.as<float>(), 1.0f, nf,
-                           nser);
+        hipLaunchKernelGGL(btpm_weights_fm, dim3(grid1((size_t)nb * nf, 256)), dim3(256), 0, st,
+                           sh->F.as<float>(), sh->albedo.as<float>(), 1.0f, 0, 0.0f, nf, nb,
+                           sh->wfm.as<float>());
         rgi_blk.alloc(ncser * 4);
         dim3 g(8 * grid1(nf, 8) * grid1((size_t)nb * BTPM_BLK, 64));  // XCD-grouped facets
         hipLaunchKernelGGL(btpm_shape_to_crater, g, dim3(256), 0, st, sh->rowptr.as<int>(),
                            sh->colidx.as<int>(), sh->drfA.as<_Float16>(), sh->cbase.as<int>(),
-                           sh->drf_scale.as<float>(), AF.as<float>(), 0, 0.0f, nf,
-                           nf, nt, nb, (const float*)nullptr, (const float*)nullptr, 1.0f,
+                           sh->drf_scale.as<float>(), sh->wfm.as<float>(), nf, nt, nb,
+                           (const float*)nullptr, (const float*)nullptr, 1.0f,
                            rgi_blk.as<float>());
         rgi.alloc((size_t)nf * nt * BTPM_NS * 4);
         hipLaunchKernelGGL(btpm_unpack_crater, dim3(grid1((size_t)nf * BTPM_NS * nb, 256)),
