@@ -292,8 +292,21 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
   const int npad = nb * BTPM_BLK;
   rhalf8 ra[kAPer];
   float rb[8];
+  // the wave's 8 entry indices of a chunk are wave-uniform; read through an
+  // opaque zero VGPR they come by vector loads, on the vector memory
+  // counter: as scalar loads they would share the LDS counter, and every
+  // wait for the products' LDS reads would wait for them too (scalar loads
+  // complete out of order).  Indices run one chunk ahead of the gathers that
+  // use them; every index is clamped into the facet's row, and gathers past
+  // its end read a valid weight that is zeroed where it is used
+  int vz;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+  int ci[8], cn[8];
+  auto load_idx = [&](int kb, int (&dst)[8]) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dst[i] = colidx[min(kb + 8 * wave + i, k1 - 1) + vz];
+  };
   auto fetch = [&](int kb) {
-    const int kn = min(kRK, k1 - kb);
     const rhalf8* __restrict__ tile = tiles + (size_t)((kb - k0) / kRK) * (kRDrfTile / 8);
 #pragma unroll
     for (int q = 0; q < kAPer; ++q) {
@@ -301,16 +314,16 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
       if (p < kAPieces) ra[q] = tile[p];
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int k = 8 * wave + i;
-      rb[i] = 0.0f;
-      if (k < kn) rb[i] = w[(size_t)colidx[kb + k] * npad + sb];
-    }
+    for (int i = 0; i < 8; ++i) rb[i] = w[(size_t)ci[i] * npad + sb];
+    load_idx(kb + kRK, cn);
   };
   rfloat4 acc[kMT];
 #pragma unroll
   for (int t = 0; t < kMT; ++t) acc[t] = rfloat4{0.0f, 0.0f, 0.0f, 0.0f};
-  if (k0 < k1) fetch(k0);
+  if (k0 < k1) {
+    load_idx(k0, ci);
+    fetch(k0);
+  }
   for (int kb = k0; kb < k1; kb += kRK) {
     __syncthreads();  // the previous chunk's products have read LDS
 #pragma unroll
@@ -322,9 +335,10 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
       }
     }
     {
+      const int kn = min(kRK, k1 - kb);
       float v[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = rb[i];
+      for (int i = 0; i < 8; ++i) v[i] = 8 * wave + i < kn ? rb[i] : 0.0f;
       rhalf8 hi, lo;
       split8(v, 1.0f, hi, lo);
       const int n = tid & (kRN - 1);
@@ -332,7 +346,11 @@ extern "C" __global__ __launch_bounds__(256) void btpm_shape_to_crater(
       *reinterpret_cast<rhalf8*>(&lb[1][n * kRS + 8 * wave]) = lo;
     }
     __syncthreads();
-    if (kb + kRK < k1) fetch(kb + kRK);
+    if (kb + kRK < k1) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) ci[i] = cn[i];
+      fetch(kb + kRK);
+    }
     const int bn = 16 * wave + r;
     const rhalf8 bhi = *reinterpret_cast<const rhalf8*>(&lb[0][bn * kRS + 8 * g]);
     const rhalf8 blo = *reinterpret_cast<const rhalf8*>(&lb[1][bn * kRS + 8 * g]);
