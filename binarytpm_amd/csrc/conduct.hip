@@ -585,6 +585,10 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
   int rpar = 0;  // rescales so far, mod 2
   const float esia = S.esia;
   const int ncols = S.ncols;
+  // held in a register, so the block loop has no scalar load (read through S
+  // there, it is reloaded after every barrier, and a pending scalar load
+  // makes the product's first LDS wait a full drain of the shared counter)
+  const float vf_unscale = S.vf_unscale;
 
   float u[NX + 1];
   const float Tref = load_profile<NX>((const gfloat*)S.prof + (size_t)c * BTPM_PSTRIDE, u);
@@ -636,7 +640,7 @@ extern "C" __global__ __launch_bounds__(BTPM_CRATER_THREADS) void btpm_conduct_c
     for (int k = 0; k < BTPM_BLK; ++k) lds_p[k][tid] = nprev[k];
     if (more) load8(Tin + (size_t)(b + 1) * bstride + lane_off(), nprev);
     __syncthreads();
-    crater_product(lds_a, lds_b, lds_q, wave, lane, S.vf_unscale);
+    crater_product(lds_a, lds_b, lds_q, wave, lane, vf_unscale);
     __syncthreads();
   };
   auto block_tail = [&](int b) {
