@@ -795,6 +795,14 @@ __device__ __forceinline__ void selfheat_ell_body(const int2* __restrict__ wseg,
   float acc[H];
 #pragma unroll
   for (int q = 0; q < H; ++q) acc[q] = 0.0f;
+  // the wave's segment (first, n) of a chunk is wave-uniform; read through
+  // an opaque zero VGPR it comes by a vector load, one chunk ahead (a scalar
+  // load would share the LDS wait counter, and the chunk's first LDS wait
+  // would drain it)
+  int vz;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+  const int2* __restrict__ wsb = wseg + (size_t)blockIdx.x * nchunk * BTPM_SH_WAVES + wave;
+  int2 dn = wsb[vz];
   fetch(0);
   for (int c = 0; c < nchunk; ++c) {
     __syncthreads();  // the previous chunk has been consumed
@@ -802,19 +810,24 @@ __device__ __forceinline__ void selfheat_ell_body(const int2* __restrict__ wseg,
     for (int i = 0; i < kPer; ++i)
       *reinterpret_cast<float4*>(&E[(srow + i * (kSHRows / kQ4)) * kStride + sq4]) = pre[i];
     if (c + 1 < nchunk) fetch(c + 1);
+    const int2 dc = dn;
+    dn = wsb[(size_t)min(c + 1, nchunk - 1) * BTPM_SH_WAVES + vz];
     __syncthreads();
-    const int2 sg = wseg[((size_t)blockIdx.x * nchunk + c) * BTPM_SH_WAVES + wave];
-    const int n = wave_live ? sg.y : 0;
-    const int2* __restrict__ e = ent + (size_t)sg.x + lane;
-    int2 g1 = make_int2(0, 0), g2 = make_int2(0, 0);
-    if (n > 0) g1 = e[0];
-    if (n > 1) g2 = e[64];
-    for (int k = 0; k < n; ++k) {
-      const int2 gv = g1;
-      g1 = g2;
-      if (k + 2 < n) g2 = e[(size_t)(k + 2) * 64];
+    const int n = wave_live ? __builtin_amdgcn_readfirstlane(dc.y) : 0;
+    if (n == 0) continue;
+    const int2* __restrict__ e = ent + (size_t)__builtin_amdgcn_readfirstlane(dc.x) + lane;
+    // entries are prefetched three ahead by unconditional loads (the index
+    // clamped to the segment's last entry) into the registers of the entry
+    // just consumed (the loop is unrolled by three, so no copy waits for a
+    // load); the empty asm with a memory clobber keeps each load where it is
+    // written instead of next to its use, and the one on the emitter index
+    // keeps that entry's address arithmetic (and its wait) at its own step
+    const int last = n - 1;
+    auto step = [&](const int2& gv) {
+      int gx = gv.x;
+      asm volatile("" : "+v"(gx));
       const float v = __int_as_float(gv.y);
-      const float* er = &E[gv.x * kStride];
+      const float* er = &E[gx * kStride];
 #pragma unroll
       for (int q = 0; q < H; q += 4) {
         const float4 e4 = *reinterpret_cast<const float4*>(er + q);
@@ -823,7 +836,23 @@ __device__ __forceinline__ void selfheat_ell_body(const int2* __restrict__ wseg,
         acc[q + 2] = fmaf(v, e4.z, acc[q + 2]);
         acc[q + 3] = fmaf(v, e4.w, acc[q + 3]);
       }
+    };
+    int2 qa = e[0], qb = e[(size_t)min(1, last) * 64], qc = e[(size_t)min(2, last) * 64];
+    asm volatile("" ::: "memory");
+    int k = 0;
+    for (; k + 3 <= n; k += 3) {
+      step(qa);
+      qa = e[(size_t)min(k + 3, last) * 64];
+      asm volatile("" ::: "memory");
+      step(qb);
+      qb = e[(size_t)min(k + 4, last) * 64];
+      asm volatile("" ::: "memory");
+      step(qc);
+      qc = e[(size_t)min(k + 5, last) * 64];
+      asm volatile("" ::: "memory");
     }
+    if (k < n) step(qa);
+    if (k + 1 < n) step(qb);
   }
   if (!live) return;
   const float* __restrict__ src = S.src;
