@@ -168,12 +168,33 @@ extern "C" __global__ __launch_bounds__(128) void btpm_init_crater(btpm_set_dev 
   const int c = f * BTPM_NS + (ok ? j : 0);
   double sum = 0.0;
   if (ok) {
-    for (int b = 0; b < S.nb; ++b) {
-      const float4* p = reinterpret_cast<const float4*>(S.src + ((size_t)b * S.ncols + c) * BTPM_BLK);
-      const float4 x = p[0], y = p[1];
-      const float v[BTPM_BLK] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
-      const int kend = min(BTPM_BLK, S.ntau - b * BTPM_BLK);
-      for (int k = 0; k < kend; ++k) sum += (double)v[k];
+    // the steps in order, block by block; the full blocks' loads are issued
+    // eight blocks at a time (sixteen loads in flight per lane) ahead of the
+    // additions, the partial last block is added step by step
+    const float* srcc = S.src + (size_t)c * BTPM_BLK;
+    const size_t bstr = (size_t)S.ncols * BTPM_BLK;
+    const int nfull = S.ntau / BTPM_BLK;
+    for (int b0 = 0; b0 < nfull; b0 += 8) {
+      float4 x[8][2];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (b0 + j < nfull) {
+          const float4* p = reinterpret_cast<const float4*>(srcc + (size_t)(b0 + j) * bstr);
+          x[j][0] = p[0];
+          x[j][1] = p[1];
+        }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (b0 + j < nfull) {
+          sum += (double)x[j][0].x; sum += (double)x[j][0].y;
+          sum += (double)x[j][0].z; sum += (double)x[j][0].w;
+          sum += (double)x[j][1].x; sum += (double)x[j][1].y;
+          sum += (double)x[j][1].z; sum += (double)x[j][1].w;
+        }
+    }
+    if (nfull < S.nb) {
+      const float* p = srcc + (size_t)nfull * bstr;
+      for (int k = 0; k < S.ntau - nfull * BTPM_BLK; ++k) sum += (double)p[k];
     }
   }
   const double mean = sum / S.ntau;
