@@ -2087,6 +2087,33 @@ static int fetch_series(btpm_solve* s, SetHost* S, float* out, int crater) {
   API_END
 }
 
+extern "C++" {
+namespace btpm {
+// A finished solve's temperatures unpacked on the device for the Observe
+// synthesis: surface [nf][ntau] into stemp and, unless ctemp is null, crater
+// [nf][ntau][100] into ctemp, floored at +0 K like every readback.  The caller
+// holds the context's lock and runs this inside its API_BEGIN (HIP failures
+// throw); the kernels go on the context's stream.
+int solve_temps_on_device(btpm_solve* s, btpm_ctx* ctx, int nf, int ntau, float* stemp,
+                          float* ctemp) {
+  if (s->ctx != ctx) return fail(BTPM_ERR_ARG, "the solve belongs to another context");
+  if (s->phase != 3 || !s->smooth) return fail(BTPM_ERR_STATE, "solve not finished");
+  if (s->shape->nf != nf || s->shape->p.ntau != ntau)
+    return fail(BTPM_ERR_ARG, "solve of %d facets x %d steps, observation of %d x %d",
+                s->shape->nf, s->shape->p.ntau, nf, ntau);
+  if (ctemp && !s->crater) return fail(BTPM_ERR_STATE, "no finished crater solve");
+  const int nb = s->smooth->d.nb;
+  hipLaunchKernelGGL(btpm_unpack_blocked, dim3(grid1((size_t)nf * nb, 256)), dim3(256), 0,
+                     ctx->stream, s->smooth->T.as<float>(), stemp, nf, ntau, nb);
+  if (ctemp)
+    hipLaunchKernelGGL(btpm_unpack_crater, dim3(grid1((size_t)nf * BTPM_NS * nb, 256)),
+                       dim3(256), 0, ctx->stream, s->crater->T.as<float>(), ctemp, nf, ntau, nb);
+  HIPCHK(hipGetLastError());
+  return BTPM_OK;
+}
+}  // namespace btpm
+}  // extern "C++"
+
 int btpm_solve_surface_temps(btpm_solve* s, float* out) {
   if (!s || !out) return fail(BTPM_ERR_ARG, "null argument");
   if (s->phase != 3 || !s->smooth) return fail(BTPM_ERR_STATE, "solve not finished");

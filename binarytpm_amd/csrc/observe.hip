@@ -270,10 +270,13 @@ extern "C" __global__ __launch_bounds__(1024) void btpm_obs_sum(
 
 using namespace btpm;
 
-extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* smooth,
-                            double* rough) {
+namespace {
+// btpm_observe and btpm_observe_solve: the temperatures come from the host
+// arrays of `in`, or with `solve` from that finished solve's device memory
+int observe_run(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve, double* smooth,
+                double* rough) {
   if (!ctx || !in || !smooth || !rough || !in->equpos || !in->facets || !in->visible ||
-      !in->stemp || !in->colour || !in->wavelength_um)
+      (!solve && !in->stemp) || !in->colour || !in->wavelength_um)
     return fail(BTPM_ERR_ARG, "null argument");
   if (in->nv < 3 || in->nf < 1 || in->ntau < 1 || in->nv_primary < 0 || in->nv_primary > in->nv)
     return fail(BTPM_ERR_ARG, "bad sizes: nv %d nf %d ntau %d nv_primary %d", in->nv, in->nf,
@@ -286,9 +289,13 @@ extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* sm
       return fail(BTPM_ERR_ARG, "facet %d: vertex %d out of range", i / 3, in->facets[i]);
   const int rough_on = in->rough != 0;
   if (rough_on) {
-    if (!in->crater_vtx || !in->crater_facets || !in->crater_visible || !in->crater_temp)
+    if (!in->crater_vtx || !in->crater_facets || !in->crater_visible ||
+        (!solve && !in->crater_temp))
       return fail(BTPM_ERR_ARG, "null crater arrays");
     if (in->ncv < 3 || in->ncf < 1) return fail(BTPM_ERR_ARG, "bad crater sizes");
+    if (solve && in->ncf != BTPM_NS)
+      return fail(BTPM_ERR_ARG, "the solve's crater model has %d facets, not %d", BTPM_NS,
+                  in->ncf);
     for (int i = 0; i < 3 * in->ncf; ++i)
       if (in->crater_facets[i] < 0 || in->crater_facets[i] >= in->ncv)
         return fail(BTPM_ERR_ARG, "crater facet %d: vertex out of range", i / 3);
@@ -378,12 +385,19 @@ extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* sm
   upload(d_trig, trig.data(), trig.size(), st);
   upload(d_frame, frame, 15, st);
   upload(d_vis, in->visible, nser, st);
-  upload(d_st, in->stemp, nser, st);
   if (rough_on) {  // the large inputs (4 B per sub-facet and step each) through pinned slices
     d_cvis.alloc(nser * ncf * sizeof(float));
     upload_staged(ctx, d_cvis.p, in->crater_visible, nser * ncf * sizeof(float));
     d_ct.alloc(nser * ncf * sizeof(float));
-    upload_staged(ctx, d_ct.p, in->crater_temp, nser * ncf * sizeof(float));
+  }
+  if (solve) {  // the temperatures never leave the device
+    d_st.alloc(nser * sizeof(float));
+    const int rc = solve_temps_on_device(solve, ctx, nf, ntau, d_st.as<float>(),
+                                         rough_on ? d_ct.as<float>() : nullptr);
+    if (rc != BTPM_OK) return rc;
+  } else {
+    upload(d_st, in->stemp, nser, st);
+    if (rough_on) upload_staged(ctx, d_ct.p, in->crater_temp, nser * ncf * sizeof(float));
   }
   upload(d_cs, csub.data(), csub.size(), st);
   upload(d_pl, pl, 9, st);
@@ -428,4 +442,16 @@ extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* sm
                         st));
   HIPCHK(hipStreamSynchronize(st));
   API_END
+}
+}  // namespace
+
+extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* smooth,
+                            double* rough) {
+  return observe_run(ctx, in, nullptr, smooth, rough);
+}
+
+extern "C" int btpm_observe_solve(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve,
+                                  double* smooth, double* rough) {
+  if (!solve) return fail(BTPM_ERR_ARG, "null argument");
+  return observe_run(ctx, in, solve, smooth, rough);
 }

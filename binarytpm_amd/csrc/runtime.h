@@ -22,6 +22,10 @@ namespace btpm {
 // message of the last failure on this thread (btpm_last_error)
 std::string& last_error();
 
+// a finished solve's temperatures unpacked on the device (host.hip)
+int solve_temps_on_device(btpm_solve* s, btpm_ctx* ctx, int nf, int ntau, float* stemp,
+                          float* ctemp);
+
 inline int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;

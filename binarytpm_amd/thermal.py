@@ -488,22 +488,26 @@ WISE_UM = (4.6028, 11.0984, 22.6405)    # observe_generator.cpp:43
 
 def observe(ctx: Context, equpos, facets, ntau: int, hel, obs, rot, visible, stemp, colour,
             nv_primary=None, rev_primary=1.0, rev_secondary=1.0, crater=None, eps=0.95,
-            wavelengths=WISE_UM, au_m=AU_M, tmin=100):
+            wavelengths=WISE_UM, au_m=AU_M, tmin=100, solve=None):
     """Observed fluxes ([ntau][3] smooth, [ntau][3] rough) on the GPU (the
     Observe stage's flux synthesis): the shape and its motion as for
     geometry_illumination, hel / obs = (r AU, ecliptic latitude, longitude)
     of the system from the Sun / from the observer, visible [nf][ntau] facet
     flags, stemp [nf][ntau]; crater = (vertices [ncv][3], facets [ncf][3],
     visible fractions [nf][ntau][ncf], temperatures [nf][ntau][ncf]) or None;
-    colour [3][tcount] corrections from tmin K up."""
+    colour [3][tcount] corrections from tmin K up.  With solve= a finished
+    Solve, the surface and crater temperatures are that solve's, read in
+    device memory (btpm_observe_solve): stemp and the crater temperatures
+    may then be None."""
     eq = np.ascontiguousarray(equpos, dtype=np.float64)
     fv = np.ascontiguousarray(facets, dtype=np.int32)
     vis = np.ascontiguousarray(visible, dtype=np.int32)
-    st = _f32(stemp)
     col = np.ascontiguousarray(colour, dtype=np.float64)
     wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
     nv, nf = eq.shape[0], fv.shape[0]
-    if vis.shape != (nf, ntau) or st.shape != (nf, ntau) or wl.shape != (3,) or col.shape[0] != 3:
+    st = _f32(stemp) if stemp is not None or solve is None else None
+    if (vis.shape != (nf, ntau) or (st is not None and st.shape != (nf, ntau))
+            or wl.shape != (3,) or col.shape[0] != 3):
         raise ValueError("visible/stemp must be [nf][ntau], wavelengths [3], colour [3][n]")
     q = ObserveIn()
     q.nv, q.nf, q.ntau = nv, nf, int(ntau)
@@ -515,25 +519,35 @@ def observe(ctx: Context, equpos, facets, ntau: int, hel, obs, rot, visible, ste
     q.obs[:] = [float(x) for x in obs]
     q.rot[:] = [float(x) for x in rot]
     q.au_m = float(au_m)
-    q.visible, q.stemp = _ptr(vis, _ip), _ptr(st)
+    q.visible = _ptr(vis, _ip)
+    if st is not None:
+        q.stemp = _ptr(st)
     keep = []
     if crater is not None:
         cv = _f32(crater[0])
         cf = np.ascontiguousarray(crater[1], dtype=np.int32)
-        cvis, ct = _f32(crater[2]), _f32(crater[3])
-        if cvis.shape != (nf, ntau, cf.shape[0]) or ct.shape != cvis.shape:
+        cvis = _f32(crater[2])
+        ct = _f32(crater[3]) if crater[3] is not None or solve is None else None
+        if cvis.shape != (nf, ntau, cf.shape[0]) or (ct is not None and ct.shape != cvis.shape):
             raise ValueError("crater arrays must be [nf][ntau][ncf]")
         keep += [cv, cf, cvis, ct]
         q.rough, q.ncv, q.ncf = 1, cv.shape[0], cf.shape[0]
         q.crater_vtx, q.crater_facets = _ptr(cv), _ptr(cf, _ip)
-        q.crater_visible, q.crater_temp = _ptr(cvis), _ptr(ct)
+        q.crater_visible = _ptr(cvis)
+        if ct is not None:
+            q.crater_temp = _ptr(ct)
     q.wavelength_um = wl.ctypes.data_as(C.POINTER(C.c_double))
     q.colour = col.ctypes.data_as(C.POINTER(C.c_double))
     q.tmin, q.tcount, q.eps = int(tmin), col.shape[1], float(eps)
     sm = np.empty((ntau, 3), np.float64)
     ro = np.empty((ntau, 3), np.float64)
-    check(lib().btpm_observe(ctx.handle, C.byref(q), sm.ctypes.data_as(C.POINTER(C.c_double)),
-                             ro.ctypes.data_as(C.POINTER(C.c_double))))
+    if solve is None:
+        check(lib().btpm_observe(ctx.handle, C.byref(q), sm.ctypes.data_as(C.POINTER(C.c_double)),
+                                 ro.ctypes.data_as(C.POINTER(C.c_double))))
+    else:
+        check(lib().btpm_observe_solve(ctx.handle, C.byref(q), solve.handle,
+                                       sm.ctypes.data_as(C.POINTER(C.c_double)),
+                                       ro.ctypes.data_as(C.POINTER(C.c_double))))
     return sm, ro
 
 

@@ -481,6 +481,17 @@ typedef struct {
 } btpm_observe_in;
 /* smooth, rough: [ntau][3] (host memory)                                    */
 int btpm_observe(btpm_ctx *ctx, const btpm_observe_in *in, double *smooth, double *rough);
+/* The same synthesis with the temperatures taken from a finished solve in
+ * device memory (no host copy of its [nf][ntau][100] crater temperatures):
+ * the solve's shape has in->nf facets and in->ntau steps, and with in->rough
+ * its crater model the in->ncf = 100 sub-facets; in->stemp and
+ * in->crater_temp are not read (may be null).  Surface and crater
+ * temperatures enter floored at +0 K, as btpm_solve_surface_temps and
+ * btpm_solve_crater_temps return them, so the fluxes equal btpm_observe's on
+ * those readbacks.  (No counterpart in the reference, whose Observe stage
+ * reads the Thermal stage's files.)                                         */
+int btpm_observe_solve(btpm_ctx *ctx, const btpm_observe_in *in, btpm_solve *solve,
+                       double *smooth, double *rough);
 
 #ifdef __cplusplus
 }

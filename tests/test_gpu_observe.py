@@ -86,3 +86,51 @@ def test_observe_fg3_full_cycle(ctx):
     for s in (0, 333, 649):
         o = O.observe(V, F, ntau, HEL, OBS, ROT, vis, st, cc, crater=crater, steps=(s, 1), **kw)
         _check(f"fg3 rough step {s}", (g[0][s:s + 1], g[1][s:s + 1]), o)
+
+
+def test_observe_from_solve_on_device(ctx):
+    """btpm_observe_solve: the Observe synthesis reading a finished crater
+    solve's temperatures in device memory equals btpm_observe on that solve's
+    readbacks bit for bit (both floored at +0 K, same kernels), with and
+    without the crater model; an unfinished solve, a solve of another step
+    count and a crater model that is not the solver's 100 sub-facets are
+    refused."""
+    case, crater = inputs.rough_primary(subdiv=1)      # 80 facets x 100 sub-facets
+    V, F = inputs.ellipsoid(1, (0.6, 0.5, 0.45))
+    ntau = case.ntau
+    c = np.load(os.path.join(GOLD, "new_crater2.npz"))
+    cc = np.load(os.path.join(GOLD, "fg3_observe.npz"))["colour"]
+    p = th.params(ntau, case.rot_period, tacc=0.01)
+    cm = th.CraterModel(vf=crater["vf"], shadow=crater["shadow"], cosang=crater["cosang"],
+                        solar=crater["solar"], multiple=True, scatter_acc=1e-4,
+                        drf=crater["drf"])
+    shape = th.Shape(ctx, p, case.illum, 0.1, csr=case.csr, scatter=2, scatter_acc=1e-4,
+                     crater=cm)
+    s = th.Solve(ctx, shape, 150.0, with_craters=True).run()
+    vis, _, cvis, _ = _inputs(len(F), ntau, 100, 3)
+    st, ct = s.surface_temps(), s.crater_temps()
+    kw = dict(nv_primary=len(V))
+    t0 = time.perf_counter()
+    host = th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, st, cc,
+                      crater=(c["V"], c["F"], cvis, ct), **kw)
+    t1 = time.perf_counter()
+    dev = th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, None, cc,
+                     crater=(c["V"], c["F"], cvis, None), solve=s, **kw)
+    t2 = time.perf_counter()
+    print(f"[observe] from readbacks {t1 - t0:.3f} s, from the solve {t2 - t1:.3f} s", flush=True)
+    assert np.array_equal(host[0], dev[0]) and np.array_equal(host[1], dev[1])
+    assert np.isfinite(dev[1]).all() and dev[1].any()
+    sm_only = th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, None, cc, solve=s, **kw)
+    assert np.array_equal(sm_only[0], host[0]) and not sm_only[1].any()
+    pending = th.Solve(ctx, shape, 300.0, with_craters=True)
+    with pytest.raises(th.BtpmError):
+        th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, None, cc, solve=pending, **kw)
+    with pytest.raises(th.BtpmError):
+        th.observe(ctx, V, F, ntau - 10, HEL, OBS, ROT, vis[:, :ntau - 10], None, cc,
+                   solve=s, **kw)
+    with pytest.raises(th.BtpmError):
+        th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, None, cc,
+                   crater=(c["V"], c["F"][:50], cvis[:, :, :50], None), solve=s, **kw)
+    pending.close()
+    s.close()
+    shape.close()
