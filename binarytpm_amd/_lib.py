@@ -198,8 +198,8 @@ PROTOTYPES = [
                                            C.c_int, C.c_int]),
     ("btpm_observe", C.c_int, [_vp, C.POINTER(ObserveIn), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]),
-    ("btpm_observe_solve", C.c_int, [_vp, C.POINTER(ObserveIn), _vp, C.POINTER(C.c_double),
-                                     C.POINTER(C.c_double)]),
+    ("btpm_observe_solve", C.c_int, [_vp, C.POINTER(ObserveIn), _vp, _fp, C.c_int, C.c_float,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 
 _LIB = None

@@ -271,10 +271,21 @@ extern "C" __global__ __launch_bounds__(1024) void btpm_obs_sum(
 using namespace btpm;
 
 namespace {
+// Where btpm_observe_solve takes the crater visibility from: with obs_facet
+// (the observer's direction in each facet frame, [nf][ntau][3]) it is
+// computed on the device, the crater sunlight kernel with the observer as the
+// source and in->visible as the facet flags, with m x m rim samples per
+// sub-facet and the crater radius.
+struct ObsVis {
+  const float* obs_facet = nullptr;
+  int m = 0;
+  float radius = 0.0f;
+};
+
 // btpm_observe and btpm_observe_solve: the temperatures come from the host
 // arrays of `in`, or with `solve` from that finished solve's device memory
-int observe_run(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve, double* smooth,
-                double* rough) {
+int observe_run(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve, const ObsVis& ov,
+                double* smooth, double* rough) {
   if (!ctx || !in || !smooth || !rough || !in->equpos || !in->facets || !in->visible ||
       (!solve && !in->stemp) || !in->colour || !in->wavelength_um)
     return fail(BTPM_ERR_ARG, "null argument");
@@ -289,9 +300,12 @@ int observe_run(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve, dou
       return fail(BTPM_ERR_ARG, "facet %d: vertex %d out of range", i / 3, in->facets[i]);
   const int rough_on = in->rough != 0;
   if (rough_on) {
-    if (!in->crater_vtx || !in->crater_facets || !in->crater_visible ||
+    if (!in->crater_vtx || !in->crater_facets || (!ov.obs_facet && !in->crater_visible) ||
         (!solve && !in->crater_temp))
       return fail(BTPM_ERR_ARG, "null crater arrays");
+    if (ov.obs_facet && (in->ncf > 128 || ov.m < 1 || ov.m > 64 || !(ov.radius > 0.0f)))
+      return fail(BTPM_ERR_ARG, "crater visibility: ncf %d, m %d, radius %g", in->ncf, ov.m,
+                  (double)ov.radius);
     if (in->ncv < 3 || in->ncf < 1) return fail(BTPM_ERR_ARG, "bad crater sizes");
     if (solve && in->ncf != BTPM_NS)
       return fail(BTPM_ERR_ARG, "the solve's crater model has %d facets, not %d", BTPM_NS,
@@ -387,7 +401,11 @@ int observe_run(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve, dou
   upload(d_vis, in->visible, nser, st);
   if (rough_on) {  // the large inputs (4 B per sub-facet and step each) through pinned slices
     d_cvis.alloc(nser * ncf * sizeof(float));
-    upload_staged(ctx, d_cvis.p, in->crater_visible, nser * ncf * sizeof(float));
+    if (ov.obs_facet)
+      crater_fractions_on_device(ctx, in->visible, ov.obs_facet, (long long)nser, in->crater_vtx,
+                                 in->crater_facets, ncf, ov.m, ov.radius, d_cvis.as<float>());
+    else
+      upload_staged(ctx, d_cvis.p, in->crater_visible, nser * ncf * sizeof(float));
     d_ct.alloc(nser * ncf * sizeof(float));
   }
   if (solve) {  // the temperatures never leave the device
@@ -447,11 +465,16 @@ int observe_run(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve, dou
 
 extern "C" int btpm_observe(btpm_ctx* ctx, const btpm_observe_in* in, double* smooth,
                             double* rough) {
-  return observe_run(ctx, in, nullptr, smooth, rough);
+  return observe_run(ctx, in, nullptr, ObsVis{}, smooth, rough);
 }
 
 extern "C" int btpm_observe_solve(btpm_ctx* ctx, const btpm_observe_in* in, btpm_solve* solve,
+                                  const float* obs_facet, int m, float crater_radius,
                                   double* smooth, double* rough) {
   if (!solve) return fail(BTPM_ERR_ARG, "null argument");
-  return observe_run(ctx, in, solve, smooth, rough);
+  ObsVis ov;
+  ov.obs_facet = obs_facet;
+  ov.m = m;
+  ov.radius = crater_radius;
+  return observe_run(ctx, in, solve, ov, smooth, rough);
 }

@@ -25,6 +25,10 @@ std::string& last_error();
 // a finished solve's temperatures unpacked on the device (host.hip)
 int solve_temps_on_device(btpm_solve* s, btpm_ctx* ctx, int nf, int ntau, float* stemp,
                           float* ctemp);
+// crater lit / visible fractions computed into device memory (viewfactor.hip)
+void crater_fractions_on_device(btpm_ctx* ctx, const int* lit, const float* src, long long n,
+                                const float* cvtx, const int* cfv, int ncf, int m, float radius,
+                                float* frac);
 
 inline int fail(int code, const char* fmt, ...) {
   char buf[512];

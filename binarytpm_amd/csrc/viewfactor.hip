@@ -600,7 +600,7 @@ extern "C" __global__ __launch_bounds__(256) void btpm_crater_sun(
   for (int i = threadIdx.x; i < rows * ncf; i += blockDim.x) {
     const int e = i / ncf, j = i - e * ncf;
     shadow[q0 * ncf + i] = tsh[e * (ncf + 1) + j];
-    cosang[q0 * ncf + i] = tco[e * (ncf + 1) + j];
+    if (cosang) cosang[q0 * ncf + i] = tco[e * (ncf + 1) + j];  // null: fractions only
   }
 }
 
@@ -900,6 +900,46 @@ extern "C" int btpm_crater_viewfactors(btpm_ctx* ctx, const btpm_drf_in* in, flo
   download_staged(ctx, drf, d_out.p, (size_t)n * sizeof(float));
   API_END
 }
+
+extern "C++" {
+namespace btpm {
+// The fractions of btpm_crater_sunlight (lit, or visible with the observer as
+// the source) of n facet-steps straight into device memory frac[n][ncf], from
+// host flags lit[n] and source vectors src[n][3] in the facet frames (no
+// cosines, no readback).  The caller holds the context's lock inside its
+// API_BEGIN (HIP failures throw); the work goes on the context's stream.
+void crater_fractions_on_device(btpm_ctx* ctx, const int* lit, const float* src, long long n,
+                                const float* cvtx, const int* cfv, int ncf, int m, float radius,
+                                float* frac) {
+  hipStream_t st = ctx->stream;
+  const int nsamp = m * m;
+  std::vector<float4> csub, samp;
+  std::vector<float> vtab;
+  crater_samples(cvtx, cfv, ncf, m, csub, samp, vtab);
+  DBuf d_cs, d_samp, d_vt, d_lit, d_src;
+  upload(d_cs, csub.data(), csub.size(), st);
+  upload(d_samp, samp.data(), samp.size(), st);
+  upload(d_vt, vtab.data(), vtab.size(), st);
+  const size_t shm = (128 * ((size_t)ncf + 1) + (size_t)nsamp + 1) * sizeof(float);
+  const long long chunk = 1ll << 22;
+  const long long first = std::min(n, chunk);
+  d_lit.alloc((size_t)first * sizeof(int));
+  d_src.alloc((size_t)first * 3 * sizeof(float));
+  for (long long q0 = 0; q0 < n; q0 += chunk) {
+    const long long k = std::min(chunk, n - q0);
+    HIPCHK(hipMemcpyAsync(d_lit.p, lit + q0, (size_t)k * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_src.p, src + 3 * q0, (size_t)k * 3 * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(btpm_crater_sun, dim3((unsigned)((k + 63) / 64)), dim3(256), shm, st,
+                       d_lit.as<int>(), d_src.as<float>(), d_cs.as<float4>(), d_samp.as<float4>(),
+                       d_vt.as<float>(), ncf, nsamp, radius, k, frac + q0 * ncf,
+                       (float*)nullptr);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(st));  // the chunk buffers are freed on return
+}
+}  // namespace btpm
+}  // extern "C++"
 
 extern "C" int btpm_crater_sunlight(btpm_ctx* ctx, const btpm_crater_sun_in* in, float* shadow,
                                     float* cosang) {

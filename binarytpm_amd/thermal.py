@@ -488,7 +488,8 @@ WISE_UM = (4.6028, 11.0984, 22.6405)    # observe_generator.cpp:43
 
 def observe(ctx: Context, equpos, facets, ntau: int, hel, obs, rot, visible, stemp, colour,
             nv_primary=None, rev_primary=1.0, rev_secondary=1.0, crater=None, eps=0.95,
-            wavelengths=WISE_UM, au_m=AU_M, tmin=100, solve=None):
+            wavelengths=WISE_UM, au_m=AU_M, tmin=100, solve=None, observer_facet=None,
+            crater_m: int = 10, crater_radius: float = 10.0):
     """Observed fluxes ([ntau][3] smooth, [ntau][3] rough) on the GPU (the
     Observe stage's flux synthesis): the shape and its motion as for
     geometry_illumination, hel / obs = (r AU, ecliptic latitude, longitude)
@@ -498,7 +499,10 @@ def observe(ctx: Context, equpos, facets, ntau: int, hel, obs, rot, visible, ste
     colour [3][tcount] corrections from tmin K up.  With solve= a finished
     Solve, the surface and crater temperatures are that solve's, read in
     device memory (btpm_observe_solve): stemp and the crater temperatures
-    may then be None."""
+    may then be None; with observer_facet ([nf][ntau][3], the observer's
+    direction in each facet frame) as well, the crater visible fractions are
+    computed on the device (crater_sunlight with the observer as the source,
+    crater_m x crater_m rim samples) and may be None too."""
     eq = np.ascontiguousarray(equpos, dtype=np.float64)
     fv = np.ascontiguousarray(facets, dtype=np.int32)
     vis = np.ascontiguousarray(visible, dtype=np.int32)
@@ -526,14 +530,17 @@ def observe(ctx: Context, equpos, facets, ntau: int, hel, obs, rot, visible, ste
     if crater is not None:
         cv = _f32(crater[0])
         cf = np.ascontiguousarray(crater[1], dtype=np.int32)
-        cvis = _f32(crater[2])
+        on_dev = solve is not None and observer_facet is not None
+        cvis = _f32(crater[2]) if crater[2] is not None or not on_dev else None
         ct = _f32(crater[3]) if crater[3] is not None or solve is None else None
-        if cvis.shape != (nf, ntau, cf.shape[0]) or (ct is not None and ct.shape != cvis.shape):
+        want = (nf, ntau, cf.shape[0])
+        if (cvis is not None and cvis.shape != want) or (ct is not None and ct.shape != want):
             raise ValueError("crater arrays must be [nf][ntau][ncf]")
         keep += [cv, cf, cvis, ct]
         q.rough, q.ncv, q.ncf = 1, cv.shape[0], cf.shape[0]
         q.crater_vtx, q.crater_facets = _ptr(cv), _ptr(cf, _ip)
-        q.crater_visible = _ptr(cvis)
+        if cvis is not None:
+            q.crater_visible = _ptr(cvis)
         if ct is not None:
             q.crater_temp = _ptr(ct)
     q.wavelength_um = wl.ctypes.data_as(C.POINTER(C.c_double))
@@ -545,7 +552,9 @@ def observe(ctx: Context, equpos, facets, ntau: int, hel, obs, rot, visible, ste
         check(lib().btpm_observe(ctx.handle, C.byref(q), sm.ctypes.data_as(C.POINTER(C.c_double)),
                                  ro.ctypes.data_as(C.POINTER(C.c_double))))
     else:
-        check(lib().btpm_observe_solve(ctx.handle, C.byref(q), solve.handle,
+        ob = None if observer_facet is None else _f32(observer_facet, (nf, int(ntau), 3))
+        check(lib().btpm_observe_solve(ctx.handle, C.byref(q), solve.handle, _ptr(ob),
+                                       int(crater_m), float(crater_radius),
                                        sm.ctypes.data_as(C.POINTER(C.c_double)),
                                        ro.ctypes.data_as(C.POINTER(C.c_double))))
     return sm, ro

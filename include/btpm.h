@@ -488,10 +488,17 @@ int btpm_observe(btpm_ctx *ctx, const btpm_observe_in *in, double *smooth, doubl
  * in->crater_temp are not read (may be null).  Surface and crater
  * temperatures enter floored at +0 K, as btpm_solve_surface_temps and
  * btpm_solve_crater_temps return them, so the fluxes equal btpm_observe's on
- * those readbacks.  (No counterpart in the reference, whose Observe stage
- * reads the Thermal stage's files.)                                         */
+ * those readbacks.  With obs_facet ([nf][ntau][3] floats, the observer's
+ * direction in each facet's frame) the crater visibility is computed on the
+ * device too, as btpm_crater_sunlight's fractions with the observer as the
+ * source, in->visible as the facet flags, m x m rim samples and
+ * crater_radius; in->crater_visible is then not read (may be null).  With
+ * obs_facet null, in->crater_visible is used and m, crater_radius are not.
+ * (No counterpart in the reference, whose Observe stage reads the Thermal
+ * and Visibility stages' files.)                                           */
 int btpm_observe_solve(btpm_ctx *ctx, const btpm_observe_in *in, btpm_solve *solve,
-                       double *smooth, double *rough);
+                       const float *obs_facet, int m, float crater_radius, double *smooth,
+                       double *rough);
 
 #ifdef __cplusplus
 }

@@ -92,9 +92,10 @@ def test_observe_from_solve_on_device(ctx):
     """btpm_observe_solve: the Observe synthesis reading a finished crater
     solve's temperatures in device memory equals btpm_observe on that solve's
     readbacks bit for bit (both floored at +0 K, same kernels), with and
-    without the crater model; an unfinished solve, a solve of another step
-    count and a crater model that is not the solver's 100 sub-facets are
-    refused."""
+    without the crater model, and with the crater visibility computed on the
+    device from the observer's directions; an unfinished solve, a solve of
+    another step count and a crater model that is not the solver's 100
+    sub-facets are refused."""
     case, crater = inputs.rough_primary(subdiv=1)      # 80 facets x 100 sub-facets
     V, F = inputs.ellipsoid(1, (0.6, 0.5, 0.45))
     ntau = case.ntau
@@ -122,6 +123,19 @@ def test_observe_from_solve_on_device(ctx):
     assert np.isfinite(dev[1]).all() and dev[1].any()
     sm_only = th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, None, cc, solve=s, **kw)
     assert np.array_equal(sm_only[0], host[0]) and not sm_only[1].any()
+    # crater visibility from the observer's direction in each facet frame,
+    # computed on the device: equals crater_sunlight's fractions fed to the
+    # host path
+    rng = np.random.default_rng(9)
+    ob = rng.normal(size=(len(F), ntau, 3)).astype(np.float32)
+    ob /= np.linalg.norm(ob, axis=2, keepdims=True)
+    cvis2, _ = th.crater_sunlight(ctx, vis, ob, c["V"], c["F"])
+    host2 = th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, st, cc,
+                       crater=(c["V"], c["F"], cvis2, ct), **kw)
+    dev2 = th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, None, cc,
+                      crater=(c["V"], c["F"], None, None), solve=s, observer_facet=ob, **kw)
+    assert np.array_equal(host2[0], dev2[0]) and np.array_equal(host2[1], dev2[1])
+    assert dev2[1].any() and not np.array_equal(dev2[1], dev[1])
     pending = th.Solve(ctx, shape, 300.0, with_craters=True)
     with pytest.raises(th.BtpmError):
         th.observe(ctx, V, F, ntau, HEL, OBS, ROT, vis, None, cc, solve=pending, **kw)

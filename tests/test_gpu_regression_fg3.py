@@ -161,11 +161,13 @@ def test_fg3_rough_light_curves(ctx, fg3):
     def check(s, name, key, via_host=False):
         # smooth fluxes from the six-digit temperatures of the text file the
         # reference's Observe reads; rough fluxes (crater temperatures only)
-        # from the solve in device memory, once also through the host readback
+        # from the solve in device memory with the crater visibility computed
+        # there from the observer's directions, once also through the host
         sm, _ = _flux(fg3, ctx, s.surface_temps())
         t0 = time.perf_counter()
         _, ro = th.observe(ctx, fg3["V"], fg3["F"], NTAU, HEL, OBS, ROT, seen.shadow, None,
-                           fg3["cc"], crater=(cV, cF, cvis, None), solve=s, **KW)
+                           fg3["cc"], crater=(cV, cF, None, None), solve=s,
+                           observer_facet=seen.sun_facet, **KW)
         t1 = time.perf_counter()
         if via_host:
             _, ro_h = _flux(fg3, ctx, s.surface_temps(), crater=(cV, cF, cvis, s.crater_temps()))
