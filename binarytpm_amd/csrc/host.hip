@@ -699,7 +699,8 @@ struct Sched {
   // set run opt.fast_extra more rotations after every rotation of all sets,
   // from lists of their own, so a small solve does not advance in lock-step
   // with the big sets beside it (sets are independent: results are the same).
-  // Off in profiled runs (their per-launch timings) and in tuning trials.
+  // Off in tuning trials; in profiled runs the fast rotations are launched
+  // after each timed rotation, outside its events.
   std::vector<SmoothList> fsmooth;
   std::vector<HeatGroup> fheat;
   DBuf fd_tiles_c, fd_ids;
@@ -993,7 +994,7 @@ struct Sched {
     fheat.clear();
     fn_tiles_c = fn_ids = 0;
     fast_active = false;
-    if (opt.fast_extra <= 0 || opt.profile || tuning()) return;
+    if (opt.fast_extra <= 0 || tuning()) return;
     int max_cols = 0;
     for (auto& kv : slots) max_cols = std::max(max_cols, kv.second.second->d.ncols);
     auto fast = [&](const SetHost* S) { return 8LL * S->d.ncols <= max_cols; };
@@ -1140,6 +1141,9 @@ struct Sched {
       }
       if (ev) HIPCHK(hipEventRecord(ev[1], st));
     }
+    if (fast_active)  // the small sets' extra rotations, not timed
+      for (int k = 0; k < opt.fast_extra; ++k)
+        for (int ph = 0; ph < 4; ++ph) enqueue_phase(st, ph, true);
   }
 
   void capture(hipGraph_t* g, hipGraphExec_t* x, int R, int phase) {
@@ -1212,7 +1216,7 @@ struct Sched {
                           hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     stats.syncs++;
-    if (fast_active && !opt.profile) stats.fast_rotations += R * opt.fast_extra;
+    if (fast_active) stats.fast_rotations += R * opt.fast_extra;
     schedule_sample(
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count(),
         R);

@@ -147,7 +147,8 @@ typedef struct {
                                     all sets, for sets at most 1/8 the columns
                                     of the largest resident set (they finish
                                     in fewer shared rotations; results do not
-                                    change).  Not in profiled runs.  Default 3 */
+                                    change).  In profiled runs they are not
+                                    timed.  Default 3                          */
 } btpm_batch_opts;
 
 typedef struct {
