@@ -1180,10 +1180,12 @@ extern "C" __global__ __launch_bounds__(kRowsWaves * 64) void btpm_selfheat_rows
 extern "C" __global__ void btpm_scale_cols(float* __restrict__ y, const float* __restrict__ x,
                                            const float* __restrict__ scale, float gain, int ncols,
                                            size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = (int)((i / BTPM_BLK) % ncols);
-  y[i] = (gain * scale[c]) * x[i];
+  // grid-stride: a crater series can hold more than 2^32 elements
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int c = (int)((i / BTPM_BLK) % ncols);
+    y[i] = (gain * scale[c]) * x[i];
+  }
 }
 
 // p_new = alpha*(d + q) has been written to y; record max |y - p| per

@@ -51,10 +51,12 @@ __global__ void compat_source(const float* __restrict__ illum, const float* __re
 // thermal_GPU.cu:522-523), on blocked arrays
 __global__ void compat_keep_previous(const float* __restrict__ T, float* __restrict__ Tprev,
                                      const uint8_t* __restrict__ frozen, int ncols, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int c = (int)((i / BTPM_BLK) % ncols);
-  if (!frozen[c / NS]) Tprev[i] = T[i];
+  // grid-stride: a crater series can hold more than 2^32 elements
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int c = (int)((i / BTPM_BLK) % ncols);
+    if (!frozen[c / NS]) Tprev[i] = T[i];
+  }
 }
 
 // converged kernel (thermal_GPU.cu:648-692) from the rotation's max change
@@ -263,7 +265,7 @@ void rotation(float EPS, float SIGMA, float IR_ABS, float tol, int inst) {
                         "limit)", c.r_bad, (double)c.r_bad_val, BTPM_R_MAX)};
   ensure_source();
   const size_t n = blk();
-  hipLaunchKernelGGL(compat_keep_previous, dim3(grid1(n, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(compat_keep_previous, dim3(grid_stride(n, 256)), dim3(256), 0, st,
                      c.T.as<float>(), c.Tprev.as<float>(), c.frozen.as<uint8_t>(), c.nf * NS, n);
   btpm_set_dev d = make_set(EPS * SIGMA, IR_ABS * EPS * SIGMA, tol, inst);
   HIPCHK(hipMemcpyAsync(c.d_set.p, &d, sizeof d, hipMemcpyHostToDevice, st));

@@ -102,12 +102,22 @@ void* pool_alloc(size_t bytes) {
   hipError_t e = over_limit() ? hipErrorOutOfMemory : hipMalloc(&p, want);
   size_t fr0 = 0, fr1 = 0, tot = 0, trimmed = 0;
   if (e != hipSuccess) {
+    // return cached blocks, largest first, until the driver can hold the
+    // request (an eighth spare); only if that fails, the whole cache: each
+    // returned block costs a driver call, and after a large batch the cache
+    // can hold a hundred of them
     (void)hipGetLastError();
     (void)hipMemGetInfo(&fr0, &tot);
-    trimmed = pool_cached(dev);
-    pool_trim(dev);
-    (void)hipMemGetInfo(&fr1, &tot);
+    const size_t need = want + want / 8;
+    trimmed = pool_trim_bytes(dev, need > fr0 ? need - fr0 : 0);
     e = over_limit() ? hipErrorOutOfMemory : hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      trimmed += pool_cached(dev);
+      pool_trim(dev);
+      e = over_limit() ? hipErrorOutOfMemory : hipMalloc(&p, want);
+    }
+    (void)hipMemGetInfo(&fr1, &tot);
   }
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -173,6 +183,31 @@ size_t pool_limit(int device) {
 }
 
 // hipFree waits for the device's work on the block, so trimming needs no tag
+// return cached blocks of the device to the driver, largest first, until at
+// least `bytes` are returned; the bytes returned
+size_t pool_trim_bytes(int device, size_t bytes) {
+  Pool& P = the_pool();
+  std::vector<void*> blocks;
+  size_t got = 0;
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto& c = P.cache[device];
+    while (got < bytes && !c.empty()) {
+      auto it = std::prev(c.end());
+      blocks.push_back(it->second.p);
+      got += it->first;
+      P.cached[device] -= it->first;
+      c.erase(it);
+    }
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(device);
+  for (void* b : blocks) (void)hipFree(b);
+  (void)hipSetDevice(cur);
+  return got;
+}
+
 void pool_trim(int device) {
   Pool& P = the_pool();
   std::vector<void*> blocks;
@@ -417,7 +452,7 @@ void build_smooth(btpm_solve* s, hipStream_t st) {
   d.ntiles = S->ntiles;
   d.tile_cols = BTPM_SMOOTH_WG;
   // src = (1-A) F
-  hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st, S->src.as<float>(),
+  hipLaunchKernelGGL(btpm_scale_cols, dim3(grid_stride(nser, 256)), dim3(256), 0, st, S->src.as<float>(),
                      sh->F.as<float>(), sh->one_minus_a.as<float>(), s->flux_scale, d.ncols, nser);
   const SetHost* W = warm_smooth_source(s);
   if (W) {
@@ -491,7 +526,7 @@ void build_crater(btpm_solve* s, hipStream_t st) {
       for (int j = 0; j < BTPM_NS; ++j) oma[(size_t)f * BTPM_NS + j] = 1.0f - sh->h_albedo[f];
     DBuf tmp;
     upload(tmp, oma.data(), oma.size(), st);
-    hipLaunchKernelGGL(btpm_scale_cols, dim3(grid1(nser, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(btpm_scale_cols, dim3(grid_stride(nser, 256)), dim3(256), 0, st,
                        S->src.as<float>(), sh->cillum.as<float>(), tmp.as<float>(), s->flux_scale,
                        d.ncols, nser);
     HIPCHK(hipStreamSynchronize(st));
@@ -609,9 +644,10 @@ struct Sched {
   // Tile choices per size class, measured on the device:
   //   kind 0: smooth conduction columns per workgroup {256, 128, 64}, per
   //           column-count class (ceil log2 of the set's columns);
-  //   kind 1: facet self-heating kernel {time-lane (0), ELL 64 and 32 steps
-  //           per workgroup}, per (log2 nnz, log2 facets, step blocks) class
-  //           (shapes without an ELL layout: {time-lane, per-chunk CSR}).
+  //   kind 1: facet self-heating kernel {time-lane (0), per-chunk CSR or ELL
+  //           64, ELL 32, receiver-grouped} as the shape's layouts allow, per
+  //           (log2 nnz, log2 facets, step blocks) class.
+  // A candidate replaces the first one only if it is kTuneMargin cheaper.
   // Every candidate computes bit-identical results (only the launch shape
   // differs), so a trial is an ordinary rotation of the running solves:
   // while a class is untuned the scheduler syncs after every rotation, runs
@@ -620,6 +656,7 @@ struct Sched {
   // palindromic order (A B C C B A), which cancels a linear drift of the
   // per-rotation cost as the solves converge.  The cheapest is kept in the
   // context for every later run, and the graphs are re-captured with it.
+  static constexpr double kTuneMargin = 0.02;
   using TuneKey = std::pair<int, long long>;
   struct Trial {
     TuneKey key;
@@ -685,6 +722,12 @@ struct Sched {
       size_t best = 0;
       for (size_t i = 1; i < t.cand.size(); ++i)
         if (t.cost[i] / t.n[i] < t.cost[best] / t.n[best]) best = i;
+      // the first candidate (the untuned width of the smooth conduction) is
+      // kept unless another is at least kTuneMargin cheaper: near-ties are
+      // timing noise, and a decision flipped by noise carries over to every
+      // later batch and server on the context
+      if (best != 0 && t.cost[best] / t.n[best] > (1.0 - kTuneMargin) * (t.cost[0] / t.n[0]))
+        best = 0;
       ctx->tuned[t.key] = t.cand[best];
       stats.tune_decisions++;
       if (trace_on())
@@ -2352,6 +2395,11 @@ int btpm_server_start(btpm_ctx* ctx, const btpm_batch_opts* opts, btpm_server** 
     CtxLock lk(ctx);
     if (ctx->servers > 0) return fail(BTPM_ERR_STATE, "a server is already running on this context");
     ctx->servers++;
+    // the idle blocks an earlier batch left in the device cache go back to
+    // the driver before the first request: admission then budgets real free
+    // memory, and a request's first large allocation does not wait for the
+    // cache to be returned
+    pool_trim(ctx->device);
   }
   auto srv = std::make_unique<btpm_server>();
   srv->ctx = ctx;

@@ -1,6 +1,8 @@
 // runtime.h -- host-side helpers shared by libbtpm's translation units:
 // error reporting, RAII device buffers and uploads.
 #pragma once
+
+#include <stdexcept>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,7 +74,20 @@ struct HipError {
   return BTPM_OK;
 
 inline int nblocks(int ntau) { return (ntau + BTPM_BLK - 1) / BTPM_BLK; }
-inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+// Workgroups covering n items, b per workgroup.  A dispatch's grid size is a
+// 32-bit count of work-items per dimension: a launch past it would be cut
+// short without an error, so a count that could reach it is refused here
+// (element-wise passes over whole series use grid_stride instead).
+inline unsigned grid1(size_t n, unsigned b) {
+  if (n > (size_t)UINT32_MAX - 1024u) throw std::length_error("launch over 2^32 work-items");
+  return (unsigned)((n + b - 1) / b);
+}
+// Workgroups of a grid-stride launch over n items (b lanes per workgroup):
+// at most 16384, which already gives every CU 64 workgroups.
+inline unsigned grid_stride(size_t n, unsigned b) {
+  const size_t g = (n + b - 1) / b;
+  return (unsigned)(g < 16384 ? (g > 0 ? g : 1) : 16384);
+}
 
 // device buffer owned by RAII
 // Device memory cache.  Blocks released by DBuf stay cached per device and
@@ -93,6 +108,7 @@ void pool_free(void* p);
 size_t pool_cached(int device);   // bytes held in the cache
 size_t pool_live(int device);     // bytes handed out and not yet released
 void pool_trim(int device);       // return the device's cache to the driver
+size_t pool_trim_bytes(int device, size_t bytes);  // at least `bytes` of it, largest first
 void pool_set_limit(int device, size_t bytes);  // 0 = no limit
 size_t pool_limit(int device);
 

@@ -313,7 +313,8 @@ int btpm_batch_run(btpm_ctx *ctx, btpm_solve **solves, int n, const btpm_batch_o
 /* ---- server (continuous batching) -------------------------------------------
  * A request loop on its own host thread: submitted solves are admitted as
  * memory and slots free up, advanced together with the resident ones, and
- * completed with their latency.                                              */
+ * completed with their latency.  Starting a server returns the device
+ * blocks the library keeps cached (from earlier batches) to the driver.     */
 int btpm_server_start(btpm_ctx *ctx, const btpm_batch_opts *opts, btpm_server **out);
 int btpm_server_submit(btpm_server *srv, btpm_solve *s);
 /* blocks until s completes (or timeout_s elapses: BTPM_ERR_STATE)          */
